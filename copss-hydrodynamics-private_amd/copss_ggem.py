@@ -1,0 +1,391 @@
+"""ctypes binding of libcopss_ggem.so, the MI355X GGEM particle path.
+
+Mirrors the reference's GGEM host interface over the C-ABI declared in
+include/copss_ggem.h:
+
+  PointMesh<3>::reinit               -> Context.set_particles + build_neighbors
+  GGEMStokes::local_solution_bead    -> Context.local_velocity_beads
+  GGEMStokes::local_solution_field   -> Context.local_velocity_points
+  GGEMStokes::global_self_exclusion  -> Context.global_self_exclusion
+  PMSystemStokes::compute_point_velocity -> Context.combine_velocity
+  AssembleStokes::compute_element_rhs (spreading) -> Context.spread
+  _MatMult_Stokes (device part)      -> Context.mobility_apply
+  BrownianSystem::chebyshev_polynomial_approximation + hi_ewald
+                                     -> Context.cheb_brownian
+  BrownianSystem::compute_eigenvalues -> Context.lanczos_bounds
+
+Errors raise GGEMError carrying the C error code and ggem_last_error() text
+(the reference aborts through libmesh_error() at the same points).  There is
+no fallback: without the built library every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcopss_ggem.so")
+
+POLYMER_BEAD, POINT_PARTICLE, LAGRANGIAN_POINT = 0, 1, 2
+FP64, FP32C = 0, 1
+CHEB_REFERENCE, CHEB_FIXED = 0, 1
+COMPAT_ELEM_HYSTERESIS = 1
+
+ERRORS = {
+    -1: "GGEM_ERR_INVALID_ARG", -2: "GGEM_ERR_OUT_OF_DOMAIN", -3: "GGEM_ERR_INVALID_ELEM",
+    -4: "GGEM_ERR_HIP", -5: "GGEM_ERR_NOT_READY", -6: "GGEM_ERR_STENCIL_TIE",
+    -7: "GGEM_ERR_NO_DEVICE", -8: "GGEM_ERR_CAPACITY",
+}
+
+EXPORTS = [
+    "ggem_create", "ggem_destroy", "ggem_last_error", "ggem_get_consts", "ggem_set_precision",
+    "ggem_sync", "ggem_set_particles", "ggem_set_forces", "ggem_build_neighbors",
+    "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
+    "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
+    "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
+    "ggem_cheb_brownian", "ggem_lanczos_bounds", "ggem_util_reference_noise",
+    "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
+    "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
+]
+
+
+class GGEMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("mu", C.c_double), ("alpha", C.c_double), ("ksi", C.c_double), ("br0", C.c_double),
+        ("ibm_beta", C.c_double), ("hmin", C.c_double),
+        ("box_min", C.c_double * 3), ("box_max", C.c_double * 3),
+        ("periodic", C.c_int32 * 3), ("n_elem", C.c_int32 * 3),
+        ("search_radius_e", C.c_double), ("point_type", C.c_int32), ("precision", C.c_int32),
+        ("compat_flags", C.c_uint32), ("device", C.c_int32),
+    ]
+
+
+class Consts(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "one_eight_pi_mu", "alpha", "alpha2", "alpha3_pi_23", "two_alpha_sqrt_pi", "coeff_1",
+        "ksi", "ksi2", "two_ksi_sqrt_pi", "coeff_2", "search_radius_e", "search_radius_p")] + [
+        ("h", C.c_double * 3), ("L", C.c_double * 3),
+        ("stencil_rows", C.c_int32), ("stencil_cells", C.c_int32)]
+
+
+class ChebInfo(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("terms", C.c_int32), ("applies", C.c_int32),
+                ("pad", C.c_int32), ("err1", C.c_double), ("err2", C.c_double)]
+
+
+APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+class Op(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("apply", APPLY_FN)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcopss_ggem.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P, D, I32, I64, U64 = C.c_void_p, C.c_double, C.c_int32, C.c_int64, C.c_uint64
+        sig = {
+            "ggem_create": [P, C.POINTER(C.c_void_p)],
+            "ggem_destroy": [P],
+            "ggem_get_consts": [P, P],
+            "ggem_set_precision": [P, I32],
+            "ggem_sync": [P],
+            "ggem_set_particles": [P, I64, P, P],
+            "ggem_set_forces": [P, P],
+            "ggem_build_neighbors": [P],
+            "ggem_get_elem_ids": [P, P],
+            "ggem_neighbor_pairs": [P, P],
+            "ggem_get_neighbor_csr": [P, P, P],
+            "ggem_get_elem_point_csr": [P, I64, I64, P, P],
+            "ggem_local_velocity_beads": [P, P],
+            "ggem_global_self_exclusion": [P, P],
+            "ggem_combine_velocity": [P, P, P],
+            "ggem_local_velocity_points": [P, I64, P, P, P],
+            "ggem_spread": [P, I32, I32, P, P, P],
+            "ggem_mobility_apply": [P, P, P],
+            "ggem_set_noise": [P, I32, P],
+            "ggem_cheb_brownian": [P, P, I32, D, D, D, I32, I32, P, P, P, P],
+            "ggem_lanczos_bounds": [P, P, I32, U64, P, P],
+            "ggem_util_reference_noise": [I64, U64, D, I64, P],
+            "ggem_util_rsa_positions": [P, I64, D, U64, P],
+            "ggem_util_normal": [I64, U64, P],
+            "ggem_util_hex20_if_table": [P, I32, P],
+            "ggem_profile_enable": [P, I32],
+            "ggem_profile_reset": [P],
+            "ggem_profile_get": [P, C.c_char_p, P, P],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        L.ggem_last_error.restype = C.c_char_p
+        L.ggem_last_error.argtypes = [P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def make_params(cfg):
+    p = Params()
+    p.mu = cfg.get("mu", 1.0 / (6.0 * np.pi))
+    p.alpha = cfg["alpha"]
+    p.ksi = cfg.get("ksi", 0.0)
+    p.br0 = cfg.get("br0", 1.0)
+    p.ibm_beta = cfg.get("ibm_beta", 0.75)
+    p.hmin = cfg.get("hmin", 0.0)
+    for d in range(3):
+        p.box_min[d] = cfg["box_min"][d]
+        p.box_max[d] = cfg["box_max"][d]
+        p.periodic[d] = int(bool(cfg["periodic"][d]))
+        p.n_elem[d] = int(cfg["n_elem"][d])
+    p.search_radius_e = cfg.get("search_radius_e", 0.0)
+    p.point_type = cfg.get("point_type", POLYMER_BEAD)
+    p.precision = cfg.get("precision", FP64)
+    p.compat_flags = cfg.get("compat_flags", COMPAT_ELEM_HYSTERESIS)
+    p.device = cfg.get("device", 0)
+    return p
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = lib().ggem_last_error(ctx)
+        raise GGEMError(rc, msg.decode() if msg else "")
+
+
+# ---- host utilities ------------------------------------------------------
+def reference_noise(n, seed, dt, skip_vectors=0):
+    out = np.zeros(n)
+    _check(lib().ggem_util_reference_noise(n, seed, dt, skip_vectors, _p(out)))
+    return out
+
+
+def rsa_positions(cfg, n, min_dist, seed):
+    p = make_params(cfg)
+    out = np.zeros((n, 3))
+    _check(lib().ggem_util_rsa_positions(C.byref(p), n, min_dist, seed, _p(out)))
+    return out
+
+
+def normal(n, seed):
+    out = np.zeros(n)
+    _check(lib().ggem_util_normal(n, seed, _p(out)))
+    return out
+
+
+def hex20_if_table(h, nqp_1d):
+    h = _f64(h)
+    IF = np.zeros(20 * nqp_1d ** 3)
+    _check(lib().ggem_util_hex20_if_table(_p(h), nqp_1d, _p(IF)))
+    return IF
+
+
+class Context:
+    """One GGEM context on one HIP device (PointMesh + GGEMStokes + BrownianSystem state)."""
+
+    def __init__(self, cfg):
+        self.cfg = dict(cfg)
+        self._params = make_params(cfg)
+        h = C.c_void_p()
+        _check(lib().ggem_create(C.byref(self._params), C.byref(h)))
+        self._h = h
+        self.n = 0
+        self.n_elem = int(np.prod(cfg["n_elem"]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ggem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    def consts(self):
+        c = Consts()
+        self._c(lib().ggem_get_consts(self._h, C.byref(c)))
+        return c
+
+    def set_precision(self, precision):
+        self._c(lib().ggem_set_precision(self._h, precision))
+
+    def sync(self):
+        self._c(lib().ggem_sync(self._h))
+
+    def set_particles(self, xyz, f=None):
+        x = _f64(xyz, (-1, 3))
+        self.n = x.shape[0]
+        ff = None if f is None else _f64(f, (-1, 3))
+        self._c(lib().ggem_set_particles(self._h, self.n, _p(x), _p(ff)))
+
+    def set_forces(self, f):
+        self._c(lib().ggem_set_forces(self._h, _p(_f64(f, (-1, 3)))))
+
+    def build_neighbors(self):
+        self._c(lib().ggem_build_neighbors(self._h))
+
+    def reinit(self, xyz, f=None):
+        """PointMesh<3>::reinit with a neighbour-list update."""
+        self.set_particles(xyz, f)
+        self.build_neighbors()
+
+    def elem_ids(self):
+        e = np.zeros(self.n, np.int32)
+        self._c(lib().ggem_get_elem_ids(self._h, _p(e)))
+        return e
+
+    def neighbor_pairs(self):
+        v = C.c_int64()
+        self._c(lib().ggem_neighbor_pairs(self._h, C.byref(v)))
+        return v.value
+
+    def neighbor_csr(self):
+        off = np.zeros(self.n + 1, np.int64)
+        self._c(lib().ggem_get_neighbor_csr(self._h, _p(off), None))
+        ids = np.zeros(max(int(off[-1]), 1), np.int32)
+        self._c(lib().ggem_get_neighbor_csr(self._h, _p(off), _p(ids)))
+        return off, ids[: int(off[-1])]
+
+    def elem_point_csr(self, e0=0, ne=None):
+        ne = self.n_elem - e0 if ne is None else ne
+        off = np.zeros(ne + 1, np.int64)
+        self._c(lib().ggem_get_elem_point_csr(self._h, e0, ne, _p(off), None))
+        ids = np.zeros(max(int(off[-1]), 1), np.int32)
+        self._c(lib().ggem_get_elem_point_csr(self._h, e0, ne, _p(off), _p(ids)))
+        return off, ids[: int(off[-1])]
+
+    def elem_point_pairs(self):
+        """sum over elements of the element-point list length (spreading pairs / nqp)."""
+        off = np.zeros(self.n_elem + 1, np.int64)
+        self._c(lib().ggem_get_elem_point_csr(self._h, 0, self.n_elem, _p(off), None))
+        return int(off[-1])
+
+    def local_velocity_beads(self, out=True):
+        u = np.zeros((self.n, 3)) if out else None
+        self._c(lib().ggem_local_velocity_beads(self._h, _p(u)))
+        return u
+
+    def global_self_exclusion(self):
+        u = np.zeros((self.n, 3))
+        self._c(lib().ggem_global_self_exclusion(self._h, _p(u)))
+        return u
+
+    def combine_velocity(self, u_glob=None, out=True):
+        U = np.zeros((self.n, 3)) if out else None
+        ug = None if u_glob is None else _f64(u_glob, (-1, 3))
+        self._c(lib().ggem_combine_velocity(self._h, _p(ug), _p(U)))
+        return U
+
+    def local_velocity_points(self, pts, elem=None):
+        pts = _f64(pts, (-1, 3))
+        el = None if elem is None else np.ascontiguousarray(elem, dtype=np.int32)
+        u = np.zeros_like(pts)
+        self._c(lib().ggem_local_velocity_points(self._h, pts.shape[0], _p(pts), _p(el), _p(u)))
+        return u
+
+    def spread(self, nqp_1d, IF, want_rho=True, want_fe=True):
+        nqp = nqp_1d ** 3
+        IF = _f64(IF).ravel()
+        n_u = IF.size // nqp
+        rho = np.zeros((self.n_elem, nqp, 3)) if want_rho else None
+        Fe = np.zeros((self.n_elem, 3, n_u)) if want_fe else None
+        self._c(lib().ggem_spread(self._h, nqp_1d, n_u, _p(IF), _p(rho), _p(Fe)))
+        return rho, Fe
+
+    def mobility_apply(self, f):
+        f = _f64(f, (-1, 3))
+        u = np.zeros_like(f)
+        self._c(lib().ggem_mobility_apply(self._h, _p(f), _p(u)))
+        return u
+
+    def set_noise(self, dw):
+        """Keep nrhs noise vectors on the device (used when dw is omitted)."""
+        dw = _f64(dw)
+        nrhs = dw.size // (3 * self.n)
+        self._c(lib().ggem_set_noise(self._h, nrhs, _p(dw)))
+        self._noise_nrhs = nrhs
+
+    def cheb_brownian(self, N, eig_min, eig_max, dw=None, tol=0.1, mode=CHEB_REFERENCE, op=None,
+                      want_y=True, want_disp=True):
+        if dw is None:
+            nrhs = self._noise_nrhs
+        else:
+            dw = _f64(dw)
+            nrhs = dw.size // (3 * self.n)
+            dw = dw.reshape(nrhs, self.n, 3)
+        shape = (nrhs, self.n, 3)
+        y = np.zeros(shape) if want_y else None
+        disp = np.zeros(shape) if want_disp else None
+        info = (ChebInfo * nrhs)()
+        opp = None
+        if op is not None:
+            opp = self._make_op(op)
+        self._c(lib().ggem_cheb_brownian(self._h, C.byref(opp) if opp is not None else None, N,
+                                         eig_min, eig_max, tol, mode, nrhs, _p(dw), _p(y),
+                                         _p(disp), info))
+        return y, disp, [info[i] for i in range(nrhs)]
+
+    def lanczos_bounds(self, iters=30, seed=7, op=None):
+        lo, hi = C.c_double(), C.c_double()
+        opp = None if op is None else self._make_op(op)
+        self._c(lib().ggem_lanczos_bounds(self._h, C.byref(opp) if opp is not None else None,
+                                          iters, seed, C.byref(lo), C.byref(hi)))
+        return lo.value, hi.value
+
+    def _make_op(self, fn):
+        n3 = 3 * self.n
+
+        def cb(user, f, u):
+            try:
+                fa = np.ctypeslib.as_array(f, shape=(n3,))
+                ua = np.ctypeslib.as_array(u, shape=(n3,))
+                ua[:] = np.asarray(fn(fa.copy().reshape(-1, 3)), dtype=np.float64).ravel()
+                return 0
+            except Exception:
+                return 1
+
+        self._cb = APPLY_FN(cb)
+        return Op(None, self._cb)
+
+    # profiling
+    def profile(self, enable=True):
+        self._c(lib().ggem_profile_enable(self._h, int(enable)))
+
+    def profile_reset(self):
+        self._c(lib().ggem_profile_reset(self._h))
+
+    def profile_get(self, name):
+        ms, cnt = C.c_double(), C.c_int64()
+        self._c(lib().ggem_profile_get(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,362 @@
+// kernels_build.hip -- (a) the neighbour build: element binning, counting
+// sort of the particles into cell order (stable by particle id), and the
+// CSR emission of the reference's lists for export.
+//
+// Reference: PointMesh<3>::reinit (point_mesh.C:245-289) with
+// build_elem_point_containing_list (:573-648), build_elem_point_neighbor_list
+// (:652-799) and build_point_point_neighbor_list (:803-848).  The element
+// stencil replaces build_elem_elem_neighbor_list (:293-443); it is computed
+// on the host once per context.
+#include "ggem_device.h"
+
+namespace ggem {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(int64_t n, int b = kBlock) { return unsigned((n + b - 1) / b); }
+
+// Canonical element of a point: per axis floor((x - min)/h) clamped to
+// [0, n-1]; -1 outside [min, max].
+__device__ __forceinline__ int64_t locate_dev(const Grid& g, double x, double y, double z) {
+  const double v[3] = {x, y, z};
+  int64_t c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!(v[d] >= g.bmin[d]) || !(v[d] <= g.bmin[d] + g.L[d])) return -1;
+    int64_t q = int64_t(floor((v[d] - g.bmin[d]) / g.h[d]));
+    q = q < 0 ? 0 : q;
+    q = q > g.n[d] - 1 ? g.n[d] - 1 : q;
+    c[d] = q;
+  }
+  return c[0] + int64_t(g.n[0]) * (c[1] + int64_t(g.n[1]) * c[2]);
+}
+
+// Elem::contains_point of an axis-aligned hex with libMesh's 1e-6 tolerance
+// on the reference element.
+__device__ __forceinline__ bool contains_dev(const Grid& g, int64_t e, double x, double y,
+                                             double z) {
+  int ci, cj, ck;
+  cell_ijk(g, e, ci, cj, ck);
+  const double v[3] = {x, y, z};
+  const int ijk[3] = {ci, cj, ck};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double cen = g.bmin[d] + (double(ijk[d]) + 0.5) * g.h[d];
+    const double xi = 2.0 * (v[d] - cen) / g.h[d];
+    if (!(fabs(xi) <= 1.0 + 1e-6)) return false;
+  }
+  return true;
+}
+
+__global__ void k_bin(Grid g, int64_t n, const double* __restrict__ x, int32_t* __restrict__ elem,
+                      int use_prev, int32_t* __restrict__ count, int32_t* __restrict__ flag) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+  int64_t e = -1;
+  if (use_prev) {
+    const int32_t old = elem[i];
+    if (old >= 0 && old < g.n_elem && contains_dev(g, old, px, py, pz)) e = old;
+  }
+  if (e < 0) e = locate_dev(g, px, py, pz);
+  if (e < 0) {
+    atomicOr(flag, 1);
+    e = 0;  // keep later kernels in bounds; the host reports the error
+  }
+  elem[i] = int32_t(e);
+  atomicAdd(&count[e], 1);
+}
+
+// Exclusive scan of int32 counts in three phases: per-block scan with block
+// totals, a single-block scan of the totals, and the carry-in add.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, int32_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int32_t add = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  total = sh[kScanThreads - 1];
+  const int32_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_tiles(const int32_t* __restrict__ in, int64_t len, int32_t* __restrict__ out,
+                             int32_t* __restrict__ tile_sums) {
+  __shared__ int32_t sh[kScanThreads];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  int32_t v[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    v[q] = (base + q < len) ? in[base + q] : 0;
+    s += v[q];
+  }
+  int32_t total;
+  int32_t run = block_exclusive_scan(s, sh, total);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    if (base + q < len) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_sums(int32_t* __restrict__ sums, int64_t ntiles) {
+  __shared__ int32_t sh[kScanThreads];
+  int32_t carry = 0;
+  for (int64_t c0 = 0; c0 < ntiles; c0 += kScanThreads) {
+    const int64_t i = c0 + threadIdx.x;
+    const int32_t v = (i < ntiles) ? sums[i] : 0;
+    int32_t total;
+    const int32_t ex = block_exclusive_scan(v, sh, total);
+    if (i < ntiles) sums[i] = ex + carry;
+    carry += total;
+    __syncthreads();
+  }
+}
+
+__global__ void k_scan_add(int32_t* __restrict__ out, int64_t len, const int32_t* __restrict__ sums,
+                           int32_t total_n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len) out[i] += sums[i / kScanTile];
+  if (i == 0) out[len] = total_n;
+}
+
+// Place every particle into its cell's slot range (arbitrary order within
+// the cell), then sort each cell by particle id and fill the slot arrays.
+__global__ void k_place(int64_t n, const int32_t* __restrict__ elem,
+                        const int32_t* __restrict__ cell_start, int32_t* __restrict__ cursor,
+                        int32_t* __restrict__ perm) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t e = elem[i];
+  const int32_t s = cell_start[e] + atomicAdd(&cursor[e], 1);
+  perm[s] = int32_t(i);
+}
+
+__global__ void k_sort_cells(int64_t n_elem, int64_t n, const int32_t* __restrict__ cell_start,
+                             int32_t* __restrict__ perm, int32_t* __restrict__ slot_of,
+                             int32_t* __restrict__ slot_cell, const double* __restrict__ x,
+                             double* __restrict__ xs) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  const int32_t a = cell_start[e], b = cell_start[e + 1];
+  for (int32_t s = a + 1; s < b; ++s) {  // insertion sort by particle id
+    const int32_t v = perm[s];
+    int32_t t = s - 1;
+    while (t >= a && perm[t] > v) {
+      perm[t + 1] = perm[t];
+      --t;
+    }
+    perm[t + 1] = v;
+  }
+  for (int32_t s = a; s < b; ++s) {
+    const int32_t p = perm[s];
+    slot_of[p] = s;
+    slot_cell[s] = int32_t(e);
+    xs[s] = x[3 * int64_t(p)];
+    xs[n + s] = x[3 * int64_t(p) + 1];
+    xs[2 * n + s] = x[3 * int64_t(p) + 2];
+  }
+}
+
+// Forces (nrhs blocks of interleaved particle-order vectors) -> slot SoA.
+__global__ void k_gather_forces(int64_t n, int nrhs, const int32_t* __restrict__ perm,
+                                const double* __restrict__ f, int64_t stride_in,
+                                double* __restrict__ fs, float* __restrict__ fs32,
+                                int64_t stride_out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t p = perm[s];
+  for (int r = 0; r < nrhs; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double v = f[r * stride_in + 3 * p + c];
+      if (fs) fs[r * stride_out + c * n + s] = v;
+      if (fs32) fs32[r * stride_out + c * n + s] = float(v);
+    }
+  }
+}
+
+// Neighbour count of every particle (own element + stencil, minus itself).
+__global__ void k_count_pairs(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                              const int32_t* __restrict__ cell_start,
+                              const int32_t* __restrict__ slot_cell,
+                              const int32_t* __restrict__ perm, int64_t n,
+                              int64_t* __restrict__ counts, unsigned long long* __restrict__ total) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t cnt = 0;
+  if (s < n) {
+    int ci, cj, ck;
+    cell_ijk(g, slot_cell[s], ci, cj, ck);
+    for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
+                     [&](int32_t a, int32_t b) { cnt += b - a; });
+    cnt -= 1;
+    if (counts) counts[perm[s]] = cnt;
+  }
+  // wave reduction then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, (unsigned long long)cnt);
+}
+
+// Reference order of a particle list: own element's particles (ascending id,
+// itself removed), then the other stencil elements by ascending element id,
+// each by ascending particle id.  Slots are sorted by (element, id), so the
+// second part is the ascending sequence of slots outside the own element.
+__device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows,
+                          const int32_t* cell_start, int64_t e, int32_t self_slot,
+                          const int32_t* perm, int32_t* out) {
+  int ci, cj, ck;
+  cell_ijk(g, e, ci, cj, ck);
+  const int32_t oa = cell_start[e], ob = cell_start[e + 1];
+  int64_t w = 0;
+  for (int32_t t = oa; t < ob; ++t)
+    if (t != self_slot) out[w++] = t;
+  const int64_t head = w;
+  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b) {
+    for (int32_t t = a; t < b; ++t)
+      if (t < oa || t >= ob) out[w++] = t;
+  });
+  for (int64_t q = head + 1; q < w; ++q) {  // insertion sort of the tail
+    const int32_t v = out[q];
+    int64_t t = q - 1;
+    while (t >= head && out[t] > v) {
+      out[t + 1] = out[t];
+      --t;
+    }
+    out[t + 1] = v;
+  }
+  for (int64_t q = 0; q < w; ++q) out[q] = perm[out[q]];
+}
+
+__global__ void k_fill_csr(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                           const int32_t* __restrict__ cell_start,
+                           const int32_t* __restrict__ slot_cell, const int32_t* __restrict__ perm,
+                           int64_t n, const int64_t* __restrict__ offsets, int32_t* __restrict__ ids) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int32_t p = perm[s];
+  emit_list(g, rows, nrows, cell_start, slot_cell[s], int32_t(s), perm, ids + offsets[p]);
+}
+
+__global__ void k_elem_point(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                             const int32_t* __restrict__ cell_start,
+                             const int32_t* __restrict__ perm, int64_t e0, int64_t ne,
+                             int64_t* __restrict__ offsets, int32_t* __restrict__ ids, int pass) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= ne) return;
+  const int64_t e = e0 + q;
+  if (pass == 0) {
+    int ci, cj, ck;
+    cell_ijk(g, e, ci, cj, ck);
+    int64_t cnt = 0;
+    for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
+                     [&](int32_t a, int32_t b) { cnt += b - a; });
+    offsets[q + 1] = cnt;
+  } else {
+    emit_list(g, rows, nrows, cell_start, e, -1, perm, ids + offsets[q]);
+  }
+}
+
+__global__ void k_locate(Grid g, int64_t m, const double* __restrict__ pts, int32_t* __restrict__ elem) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  elem[q] = int32_t(locate_dev(g, pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]));
+}
+
+}  // namespace
+
+hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis) {
+  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_elem, ctx->stream);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(ctx->flag, 0, sizeof(int32_t), ctx->stream);
+  if (e != hipSuccess) return e;
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "bin");
+  hipLaunchKernelGGL(k_bin, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid, ctx->n,
+                     ctx->x_in, ctx->elem, hysteresis ? 1 : 0, ctx->cell_count, ctx->flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(ggem_ctx* ctx) {
+  const int64_t len = ctx->grid.n_elem;
+  const int64_t ntiles = (len + kScanTile - 1) / kScanTile;
+  ProfScope ps(ctx, "scan");
+  hipLaunchKernelGGL(k_scan_tiles, dim3(unsigned(ntiles)), dim3(kScanThreads), 0, ctx->stream,
+                     ctx->cell_count, len, ctx->cell_start, ctx->scan_tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, ctx->stream, ctx->scan_tmp, ntiles);
+  hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(len)), dim3(kBlock), 0, ctx->stream, ctx->cell_start,
+                     len, ctx->scan_tmp, int32_t(ctx->n));
+  return hipGetLastError();
+}
+
+hipError_t launch_place(ggem_ctx* ctx) {
+  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_elem, ctx->stream);
+  if (e != hipSuccess) return e;
+  ProfScope ps(ctx, "place_sort");
+  if (ctx->n > 0)
+    hipLaunchKernelGGL(k_place, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
+                       ctx->elem, ctx->cell_start, ctx->cell_count, ctx->perm);
+  hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
+                     ctx->grid.n_elem, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
+                     ctx->slot_cell, ctx->x_in, ctx->xs);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_forces(ggem_ctx* ctx, const double* f, double* fs, float* fs32, int nrhs,
+                                int64_t stride_in, int64_t stride_out) {
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "gather_forces");
+  hipLaunchKernelGGL(k_gather_forces, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
+                     nrhs, ctx->perm, f, stride_in, fs, fs32, stride_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* total_d, int64_t* counts_d) {
+  hipError_t e = hipMemsetAsync(total_d, 0, sizeof(int64_t), ctx->stream);
+  if (e != hipSuccess) return e;
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "count_pairs");
+  hipLaunchKernelGGL(k_count_pairs, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->n,
+                     counts_d, reinterpret_cast<unsigned long long*>(total_d));
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_csr(ggem_ctx* ctx, const int64_t* offsets_d, int32_t* ids_d) {
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "fill_csr");
+  hipLaunchKernelGGL(k_fill_csr, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->n,
+                     offsets_d, ids_d);
+  return hipGetLastError();
+}
+
+hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets_d,
+                                 int32_t* ids_d, int pass) {
+  if (ne == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_elem_point, dim3(blocks_for(ne)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->perm, e0, ne, offsets_d, ids_d,
+                     pass);
+  return hipGetLastError();
+}
+
+hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* elem) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_locate, dim3(blocks_for(m)), dim3(kBlock), 0, ctx->stream, ctx->grid, m, pts,
+                     elem);
+  return hipGetLastError();
+}
+
+}  // namespace ggem

@@ -1,0 +1,129 @@
+// kernels_vec.hip -- vector algebra of the Chebyshev recurrence on
+// slot-ordered SoA vectors: layout permutes, deterministic dot products
+// (fixed two-stage reduction, no atomics), axpby and fill.
+//
+// Reference: the PETSc Vec calls of BrownianSystem::chebyshev_polynomial_approximation
+// (brownian_system.C:1030-1131): VecAXPBY(y,a,b,x) = a x + b y, VecAXPY,
+// VecDot, VecNorm.
+#include "ggem_device.h"
+
+namespace ggem {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kDotBlocks = 512;
+
+inline unsigned blocks_for(int64_t n, int b = kBlock) { return unsigned((n + b - 1) / b); }
+
+// interleaved particle order [r][3p + c]  ->  slot SoA [r][c][s]
+__global__ void k_permute_in(int64_t n, int nrhs, const int32_t* __restrict__ perm,
+                             const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t p = perm[s];
+  for (int r = 0; r < nrhs; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[r * 3 * n + c * n + s] = in[r * 3 * n + 3 * p + c];
+}
+
+__global__ void k_permute_out(int64_t n, int nrhs, const int32_t* __restrict__ perm,
+                              const double* __restrict__ in, double* __restrict__ out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t p = perm[s];
+  for (int r = 0; r < nrhs; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[r * 3 * n + 3 * p + c] = in[r * 3 * n + c * n + s];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+// stage 1: partials[r][b] over a fixed grid-stride partition
+__global__ void k_dot_partial(int64_t len, int nrhs, const double* __restrict__ a,
+                              const double* __restrict__ b, double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  for (int r = 0; r < nrhs; ++r) {
+    double s = 0.0;
+    const double* ar = a + r * len;
+    const double* br = b + r * len;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len;
+         i += int64_t(gridDim.x) * blockDim.x)
+      s += ar[i] * br[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[r * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// stage 2: one block per right-hand side
+__global__ void k_dot_final(int nparts, const double* __restrict__ partials, double* __restrict__ out) {
+  __shared__ double sh[kBlock / 64];
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partials[r * nparts + i];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) out[r] = t;
+}
+
+__global__ void k_axpby(int64_t len, double a, const double* __restrict__ x, double b,
+                        double* __restrict__ y) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len) y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void k_fill(int64_t len, double v, double* __restrict__ y) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len) y[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in, double* out) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute_in, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, nrhs,
+                     ctx->perm, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_out(ggem_ctx* ctx, int nrhs, const double* in, double* out) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute_out, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
+                     nrhs, ctx->perm, in, out);
+  return hipGetLastError();
+}
+
+// partials must hold nrhs * kDotBlocks doubles
+hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
+                      double* partials) {
+  const int64_t len = 3 * ctx->n;
+  ProfScope ps(ctx, "dot");
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, ctx->stream, len, nrhs, a, b,
+                     partials);
+  hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kBlock), 0, ctx->stream, kDotBlocks, partials, out_d);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(ggem_ctx* ctx, int64_t len, double a, const double* x, double b, double* y) {
+  if (len == 0) return hipSuccess;
+  ProfScope ps(ctx, "axpby");
+  hipLaunchKernelGGL(k_axpby, dim3(blocks_for(len)), dim3(kBlock), 0, ctx->stream, len, a, x, b, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(ggem_ctx* ctx, int64_t len, double v, double* y) {
+  if (len == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(len)), dim3(kBlock), 0, ctx->stream, len, v, y);
+  return hipGetLastError();
+}
+
+}  // namespace ggem

@@ -1,0 +1,239 @@
+/*
+ * copss_ggem.h -- C-ABI of the MI355X (gfx950) GGEM particle path.
+ *
+ * Drop-in boundary for the reference's GGEM hot path.  Each entry point below
+ * names the reference interface it replaces (file:line, relative to the
+ * reference source root).  Per-bead and per-element reference calls are
+ * batched into whole-array calls.
+ *
+ * Conventions
+ *  - Vectors crossing the ABI use the reference's global layout: 3 doubles
+ *    per particle, interleaved [3*i + j] in particle-id order
+ *    (brownian_system.C:253-263, pm_system_stokes.C:494-497).
+ *  - Host pointers unless the call says otherwise.  Inputs are copied to the
+ *    device; outputs are copied back.  An output pointer may be NULL: the
+ *    result then stays in the context's device buffers (see the
+ *    ggem_get_* calls), which is how callers keep a pipeline on the GPU.
+ *  - Every call returns 0 on success and a negative GGEM_ERR_* code on
+ *    failure; ggem_last_error() describes the last failure.  Nothing throws
+ *    across the ABI (the reference aborts through libmesh_error(); a C++
+ *    caller turns a non-zero return into that).
+ *  - A context is bound to one HIP device and one stream.  Calls on one
+ *    context are not reentrant; different contexts are independent.  Calls
+ *    are stream ordered and return after their outputs are on the host
+ *    (or, with NULL outputs, after the work is queued).
+ */
+#ifndef COPSS_GGEM_H
+#define COPSS_GGEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define GGEM_OK 0
+#define GGEM_ERR_INVALID_ARG (-1)
+#define GGEM_ERR_OUT_OF_DOMAIN (-2)  /* a particle lies outside the mesh       */
+#define GGEM_ERR_INVALID_ELEM (-3)   /* ggem_stokes.C:369-380                   */
+#define GGEM_ERR_HIP (-4)            /* HIP runtime failure (text in last_error) */
+#define GGEM_ERR_NOT_READY (-5)      /* call order: particles / lists missing   */
+#define GGEM_ERR_STENCIL_TIE (-6)    /* R_e within 1e-9 of a cell-centre distance */
+#define GGEM_ERR_NO_DEVICE (-7)      /* no HIP device visible                   */
+#define GGEM_ERR_CAPACITY (-8)       /* a size exceeds what this build supports */
+
+/* ---- parameters ------------------------------------------------------- */
+/* PointType, point_particle.h:36-41 */
+enum { GGEM_POLYMER_BEAD = 0, GGEM_POINT_PARTICLE = 1, GGEM_LAGRANGIAN_POINT = 2 };
+/* arithmetic of the pair sum and the spreading */
+enum { GGEM_FP64 = 0, GGEM_FP32C = 1 };
+/* compat_flags */
+#define GGEM_COMPAT_ELEM_HYSTERESIS 1u /* keep a particle's element while it still
+                                          contains it (point_mesh.C:589-596)       */
+
+/*
+ * Reference sources of each field: viscosity_0 / alpha / br0 / ibm_beta /
+ * minimum mesh size (AssembleStokes::init_ggem_stokes, assemble_stokes.C:1061-1085;
+ * values from copss_point_particle_system.C:178-317); the slit box,
+ * periodicity and n_mesh (copss.C:971-1127); the element search radius
+ * R_e = 0.5 h_max + 4/alpha (copss.C:1074-1075).
+ */
+typedef struct ggem_params {
+  double mu;              /* viscosity; 1/(6 pi) in the reference's units        */
+  double alpha;           /* GGEM splitting parameter                            */
+  double ksi;             /* > 0: this xi; <= 0: GGEMStokes::set_ksi's rule       */
+  double br0;             /* bead radius (1)                                     */
+  double ibm_beta;        /* LAGRANGIAN_POINT: xi = 1/(ibm_beta * hmin)           */
+  double hmin;
+  double box_min[3];
+  double box_max[3];
+  int32_t periodic[3];
+  int32_t n_elem[3];      /* uniform hex grid; element id = i + nx*(j + ny*k)     */
+  double search_radius_e; /* > 0: this R_e; <= 0: 0.5*h_max + 4/alpha            */
+  int32_t point_type;     /* GGEM_POLYMER_BEAD ...                                */
+  int32_t precision;      /* GGEM_FP64 or GGEM_FP32C                              */
+  uint32_t compat_flags;  /* GGEM_COMPAT_*                                        */
+  int32_t device;         /* HIP device ordinal                                  */
+} ggem_params;
+
+/* Derived constants, GGEMStokes::set_mu/set_alpha/set_ksi (ggem_stokes.C:39-114) */
+typedef struct ggem_consts {
+  double one_eight_pi_mu, alpha, alpha2, alpha3_pi_23, two_alpha_sqrt_pi, coeff_1;
+  double ksi, ksi2, two_ksi_sqrt_pi, coeff_2;
+  double search_radius_e, search_radius_p;
+  double h[3], L[3];
+  int32_t stencil_rows;   /* (dj,dk) rows of the element stencil              */
+  int32_t stencil_cells;  /* elements within R_e of an interior element, self incl. */
+} ggem_consts;
+
+typedef struct ggem_ctx ggem_ctx;
+
+/* Context: owns all device buffers and one HIP stream.  Replaces the
+ * construction of PointMesh (point_mesh.C:139-156) and the GGEMStokes set-up
+ * (assemble_stokes.C:1061-1085).  *out is NULL on failure. */
+int ggem_create(const ggem_params* params, ggem_ctx** out);
+int ggem_destroy(ggem_ctx* ctx);
+const char* ggem_last_error(const ggem_ctx* ctx); /* never NULL; ctx may be NULL */
+int ggem_get_consts(const ggem_ctx* ctx, ggem_consts* out);
+int ggem_set_precision(ggem_ctx* ctx, int32_t precision);
+int ggem_sync(ggem_ctx* ctx);
+
+/* ---- particles -------------------------------------------------------- */
+/* Positions (and forces, may be NULL = zero) of n particles, copied to the
+ * device.  Replaces the PointParticle store read by PointMesh::reinit
+ * (point_mesh.C:245-289; point_particle.h:51-363).  Positions must already be
+ * wrapped into the box on periodic axes (PMPeriodicBoundary::correct_position,
+ * pm_periodic_boundary.C:305-328). */
+int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double* f);
+/* Forces only: the "assign" branch of BrownianSystem::extract_particle_vector
+ * used by _MatMult_Stokes (brownian_system.C:76). */
+int ggem_set_forces(ggem_ctx* ctx, const double* f);
+
+/* ---- (a) neighbour build ---------------------------------------------- */
+/* PointMesh<3>::reinit(bool& update_flag) (point_mesh.h:387,
+ * point_mesh.C:245-289): bins particles into elements, builds the
+ * element-point lists and the particle lists.  Always rebuilds. */
+int ggem_build_neighbors(ggem_ctx* ctx);
+/* Element of each particle: PointParticle::elem_id after
+ * build_elem_point_containing_list (point_mesh.C:573-648). */
+int ggem_get_elem_ids(ggem_ctx* ctx, int32_t* elem /* n */);
+/* Number of (ordered) neighbour pairs sum_i |nbr(i)| of the last build. */
+int ggem_neighbor_pairs(ggem_ctx* ctx, int64_t* npairs);
+/* Particle neighbour lists in the reference's order as CSR
+ * (build_point_point_neighbor_list, point_mesh.C:803-848): offsets[n+1],
+ * ids[npairs].  ids may be NULL to fetch offsets only. */
+int ggem_get_neighbor_csr(ggem_ctx* ctx, int64_t* offsets, int32_t* ids);
+/* Element-point neighbour list of elements [e0, e0+ne): offsets[ne+1] (from
+ * 0) and ids; build_elem_point_neighbor_list (point_mesh.C:652-799). */
+int ggem_get_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets, int32_t* ids);
+
+/* ---- (b) local velocity ----------------------------------------------- */
+/* GGEMStokes::local_solution_bead for every bead (ggem_stokes.h:187-189,
+ * ggem_stokes.C:427-503): u = sum_nbr G_reg(r_ij) f_j (+ the self term for
+ * Lagrangian points).  u: 3n or NULL. */
+int ggem_local_velocity_beads(ggem_ctx* ctx, double* u);
+/* GGEMStokes::global_self_exclusion for every bead (ggem_stokes.C:506-523). */
+int ggem_global_self_exclusion(ggem_ctx* ctx, double* uex);
+/* PMSystemStokes::compute_point_velocity("disturbed") combination
+ * (pm_system_stokes.C:441-457): U = U_glob + U_loc - coeff_1 f + f for
+ * polymer beads, U_glob + U_loc for Lagrangian points, 0 for point
+ * particles (both reference branches skip that type).  u_glob may be NULL
+ * (= 0); recomputes U_loc from the current forces. */
+int ggem_combine_velocity(ggem_ctx* ctx, const double* u_glob, double* U);
+/* GGEMStokes::local_solution_field at m points (ggem_stokes.h:176-179,
+ * ggem_stokes.C:360-424), each with the element that contains it.  elem may
+ * be NULL: the element is then located with the particle binning rule. */
+int ggem_local_velocity_points(ggem_ctx* ctx, int64_t m, const double* pts, const int32_t* elem,
+                               double* u);
+
+/* ---- (c) spreading ---------------------------------------------------- */
+/* AssembleStokes::compute_element_rhs point-force branch for every element
+ * (assemble_stokes.h:134-141, assemble_stokes.C:643-672 via
+ * assemble_global_F :473-516).  nqp_1d = 2 or 3 Gauss points per axis
+ * (qp = a + n (b + n c)); IF[n_u*nqp] with IF[l*nqp + qp] = JxW[qp]*phi_l(qp)
+ * (assemble_int_force, assemble_stokes.C:733-757), identical for every
+ * element of the uniform grid.  rho[n_e][nqp][3] and Fe[n_e][3][n_u] may each
+ * be NULL. */
+int ggem_spread(ggem_ctx* ctx, int32_t nqp_1d, int32_t n_u, const double* IF, double* rho,
+                double* Fe);
+
+/* ---- (d) Brownian step ------------------------------------------------ */
+/* Mobility operator u = M f: _MatMult_Stokes (brownian_system.h:54-59,
+ * brownian_system.C:44-140) with the host FEM part U_glob left out, i.e.
+ * the device part of compute_point_velocity.  f, u: 3n host arrays. */
+int ggem_mobility_apply(ggem_ctx* ctx, const double* f, double* u);
+
+/* Brownian noise kept on the device: nrhs vectors of 3n values (particle
+ * order, interleaved), used by ggem_cheb_brownian when its dw is NULL.  The
+ * noise vector dw of fixman_integrate (copss.C:1569-1572). */
+int ggem_set_noise(ggem_ctx* ctx, int32_t nrhs, const double* dw);
+
+/* Host-assisted operator for a FEM-inclusive mobility: u = M f on host
+ * arrays (3n, particle order).  Returns 0 on success. */
+typedef struct ggem_op {
+  void* user;
+  int (*apply)(void* user, const double* f, double* u);
+} ggem_op;
+
+typedef struct ggem_cheb_info {
+  int32_t converged;  /* reference-compat mode: the stopping rule fired on err1 */
+  int32_t terms;      /* highest Chebyshev index used                          */
+  int32_t applies;    /* operator applications                                 */
+  int32_t pad;
+  double err1, err2;  /* estimators of brownian_system.C:1076-1131             */
+} ggem_cheb_info;
+
+enum { GGEM_CHEB_REFERENCE = 0, GGEM_CHEB_FIXED = 1 };
+
+/* BrownianSystem::chebyshev_polynomial_approximation (brownian_system.h:239-243,
+ * brownian_system.C:967-1178) followed by hi_ewald (copss.C:1686):
+ *   y = M^{-1/2} dw by the three-term Chebyshev recurrence on
+ *   [eig_min, eig_max] (coefficients: chebyshev.C:57-151), disp = M y.
+ * mode GGEM_CHEB_REFERENCE: the reference's estimators and stopping rule
+ * (err1 <= tol or err2 <= 1e-9, at most N terms); GGEM_CHEB_FIXED: exactly N
+ * terms and no estimator applications.  nrhs noise vectors are processed
+ * together (dw, y, disp: nrhs blocks of 3n); dw == NULL uses the noise set
+ * by ggem_set_noise (nrhs must match).  op == NULL: device M; else the
+ * recurrence runs on the device and op->apply is called per application.
+ * y and disp may be NULL; info (nrhs entries) may be NULL. */
+int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, double eig_min,
+                       double eig_max, double tol, int32_t mode, int32_t nrhs, const double* dw,
+                       double* y, double* disp, ggem_cheb_info* info);
+
+/* Extreme-eigenvalue estimates of M by Lanczos from a seeded start vector,
+ * replacing the SLEPc Krylov-Schur bounds of BrownianSystem::compute_eigenvalues
+ * (brownian_system.C:527-657, :803-843).  Raw Ritz values: the caller applies
+ * eig_factor as copss.C:1607 does. */
+int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,
+                        double* lmin, double* lmax);
+
+/* ---- host utilities (no GPU needed) ---------------------------------- */
+/* The reference's Brownian noise: std::default_random_engine(seed) +
+ * std::normal_distribution(0, sqrt(dt)), each value times sqrt(2)
+ * (random_generator.C:43-67, copss.C:1565-1572); skip_vectors discarded
+ * vectors of length n first (copss.C:1544-1555). */
+int ggem_util_reference_noise(int64_t n, uint64_t seed, double dt, int64_t skip_vectors,
+                              double* out);
+/* Random sequential addition of n points uniform in the box, rejecting any
+ * pair closer than min_dist (minimum image on periodic axes);
+ * std::mt19937_64(seed), uniform_real_distribution per axis x, y, z. */
+int ggem_util_rsa_positions(const ggem_params* params, int64_t n, double min_dist, uint64_t seed,
+                            double* xyz);
+/* n i.i.d. N(0,1) values from std::mt19937_64(seed). */
+int ggem_util_normal(int64_t n, uint64_t seed, double* out);
+/* IF table of a HEX20 element of size h[3] with nqp_1d^3 Gauss points:
+ * IF[l*nqp + qp] = JxW[qp] * phi_l(qp), libMesh HEX20 node order. */
+int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*nqp */);
+
+/* ---- profiling ---------------------------------------------------------- */
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream.  ggem_profile_get sums the durations per kernel name. */
+int ggem_profile_enable(ggem_ctx* ctx, int32_t enable);
+int ggem_profile_reset(ggem_ctx* ctx);
+int ggem_profile_get(ggem_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

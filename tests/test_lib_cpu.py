@@ -1,0 +1,92 @@
+"""CPU-side checks of the product library: it loads, exports every entry
+point of include/copss_ggem.h, its host utilities behave, and with no GPU
+visible a context cannot be created (the product path has no CPU fallback)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import copss_ggem as G
+import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "copss_ggem.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ggem_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_match_header():
+    names = header_functions()
+    assert len(names) >= 25
+    L = G.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(G.EXPORTS) == names
+
+
+def test_reference_noise_deterministic_and_scaled():
+    a = G.reference_noise(30000, 123456789, 0.01)
+    b = G.reference_noise(30000, 123456789, 0.01)
+    assert np.array_equal(a, b)
+    assert abs(a.mean()) < 5 * np.sqrt(2 * 0.01) / np.sqrt(a.size)
+    assert abs(a.std() - np.sqrt(2 * 0.01)) < 0.02 * np.sqrt(2 * 0.01)
+    c = G.reference_noise(300, 123456789, 0.01, skip_vectors=1)
+    d = G.reference_noise(600, 123456789, 0.01)
+    assert not np.array_equal(c, d[:300])
+
+
+def test_rsa_min_distance_and_box():
+    cfg = W.WORKLOADS["beads100"]["cfg"]()
+    x = G.rsa_positions(cfg, 100, 2.2, 1001)
+    assert x.shape == (100, 3)
+    assert np.all(x >= 0) and np.all(x < 40)
+    d = x[:, None, :] - x[None, :, :]
+    d -= 40 * np.round(d / 40)
+    r = np.sqrt((d ** 2).sum(-1)) + np.eye(100) * 1e9
+    assert r.min() >= 2.2
+    assert np.array_equal(x, G.rsa_positions(cfg, 100, 2.2, 1001))
+
+
+@pytest.mark.parametrize("nq", [2, 3])
+def test_hex20_if_table(nq):
+    h = [2.0, 3.0, 4.0]
+    IF = G.hex20_if_table(h, nq).reshape(20, nq ** 3)
+    # partition of unity: sum_l JxW phi_l = JxW; sum over qp = volume
+    assert abs(IF.sum() - 24.0) < 1e-12
+    # 20-node serendipity brick: corner shape functions integrate to -V/8,
+    # edge functions to V/6 (both Gauss rules are exact for them)
+    vol = 24.0
+    assert np.allclose(IF[:8].sum(1), -vol / 8.0, rtol=1e-12)
+    assert np.allclose(IF[8:].sum(1), vol / 6.0, rtol=1e-12)
+
+
+def test_no_device_means_no_context():
+    """Without a HIP device the product refuses to run (no CPU fallback)."""
+    try:
+        import torch  # noqa: F401  (only to learn whether a GPU is present)
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is visible")
+    cfg = W.WORKLOADS["beads100"]["cfg"]()
+    with pytest.raises(G.GGEMError) as ei:
+        G.Context(cfg)
+    assert ei.value.code == -7
+
+
+def test_invalid_params_rejected_before_device():
+    cfg = W.WORKLOADS["beads100"]["cfg"]()
+    cfg["n_elem"] = [0, 20, 20]
+    with pytest.raises(G.GGEMError) as ei:
+        G.Context(cfg)
+    assert ei.value.code == -1
+    cfg = W.WORKLOADS["beads100"]["cfg"]()
+    cfg["search_radius_e"] = 2.0 * 4  # exactly 4 cell spacings of h = 2: a centre-distance tie
+    with pytest.raises(G.GGEMError) as ei:
+        G.Context(cfg)
+    assert ei.value.code == -6
