@@ -108,8 +108,15 @@ def main():
         k_ms, k_n = ctx.profile_get(kname)
         s_ms, s_n = ctx.profile_get(sname)
         l_ms, l_n = ctx.profile_get(lname)
+        per_step = {}
+        for nm in ("bin", "scan", "place_sort", "gather_forces", "count_lists", "scan64", "fill_ell",
+                   "elist_count", "elist_fill", lname, sname, kname, "axpby"):
+            t_ms, _ = ctx.profile_get(nm)
+            if t_ms > 0:
+                per_step[nm] = t_ms / args.steps
         res[prec] = dict(ms=ms, apply_avg_ms=k_ms / max(k_n, 1), apply_launches=k_n,
-                         spread_avg_ms=s_ms / max(s_n, 1), local_avg_ms=l_ms / max(l_n, 1))
+                         spread_avg_ms=s_ms / max(s_n, 1), local_avg_ms=l_ms / max(l_n, 1),
+                         per_step=per_step)
         # outputs of one step for the error report
         res[prec]["u"] = ctx.local_velocity_beads()
         _, Fe = ctx.spread(nqp_1d, IF, want_rho=False)
@@ -145,6 +152,7 @@ def main():
                      "avg_launch_ms": r32["apply_avg_ms"], "launches": r32["apply_launches"]},
         "kernels_ms": {"apply_fp32c": r32["apply_avg_ms"], "spread_fp32c": r32["spread_avg_ms"],
                        "pair_beads_fp32c": r32["local_avg_ms"]},
+        "step_breakdown_ms": r32["per_step"],
         "spread": {"qp_particle_pairs": sp_pairs,
                    "tflops": (F_SP * sp_pairs + 6 * 20 * nqp * ne) / (r32["spread_avg_ms"] * 1e-3) / 1e12
                    if r32["spread_avg_ms"] > 0 else None},
@@ -154,6 +162,7 @@ def main():
         out["fp64"] = {"ms_per_step": r64["ms"], "pair_interactions_per_s": pairs / (r64["apply_avg_ms"] * 1e-3),
                        "brownian_steps_per_s": 1e3 / r64["ms"], "apply_avg_ms": r64["apply_avg_ms"],
                        "spread_avg_ms": r64["spread_avg_ms"]}
+        out["fp64"]["step_breakdown_ms"] = r64["per_step"]
         out["fp32c_speedup_over_fp64"] = r64["ms"] / r32["ms"]
         out["fp32c_rel_l2_vs_fp64"] = {"u_local": rel(r32["u"], r64["u"]), "Fe": rel(r32["Fe"], r64["Fe"]),
                                        "brownian_disp": rel(r32["disp"], r64["disp"])}

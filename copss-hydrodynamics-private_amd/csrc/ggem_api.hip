@@ -1,15 +1,15 @@
-// ggem_api.hip -- the C-ABI of libcopss_ggem.so: context life cycle, the
-// host side of every entry point, the Chebyshev and Lanczos drivers and the
-// event profiler.
+// ggem_api.hip -- the C-ABI of libcopss_ggem.so: context life cycle,
+// particles, the neighbour build, local velocities, spreading, device noise
+// and the event profiler.  The Chebyshev and Lanczos drivers live in
+// ggem_cheb.hip.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <random>
 #include <string>
 #include <vector>
 
-#include "ggem_internal.h"
+#include "ggem_host.h"
 
 namespace {
 
@@ -74,14 +74,26 @@ static void prof_collect(ggem_ctx* ctx) {
   ctx->prof.pending.clear();
 }
 
-Store store_of(const ggem_ctx* ctx) {
-  Store s;
-  s.xs = ctx->xs;
-  s.cell_start = ctx->cell_start;
-  s.slot_cell = ctx->slot_cell;
-  s.perm = ctx->perm;
-  s.n = ctx->n;
-  return s;
+int ensure_tensors(ggem_ctx* ctx) {
+  const int prec = ctx->p.precision;
+  if (prec == GGEM_FP64 ? ctx->coef64_valid : ctx->coef32_valid) return GGEM_OK;
+  const size_t len = size_t(std::max<int64_t>(ctx->ell_len, 1));
+  int rc;
+  if (prec == GGEM_FP64) {
+    size_t cap = size_t(ctx->coef64_cap);
+    rc = ensure(ctx, &ctx->coef64, &cap, len);
+    ctx->coef64_cap = int64_t(cap);
+  } else {
+    size_t cap = size_t(ctx->coef32_cap);
+    rc = ensure(ctx, &ctx->coef32, &cap, len);
+    ctx->coef32_cap = int64_t(cap);
+  }
+  if (rc) return rc;
+  hipError_t e = launch_pair_tensors(ctx, prec);
+  if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
+  if (prec == GGEM_FP64) ctx->coef64_valid = true;
+  else ctx->coef32_valid = true;
+  return GGEM_OK;
 }
 
 }  // namespace ggem
@@ -162,7 +174,7 @@ int derive(const ggem_params* p, ggem_consts* c, std::string* why) {
 // reference compares centroids assembled from mesh nodes, and its rounding
 // would decide such a tie.
 int build_stencil(const ggem_params* p, const ggem_consts* c, std::vector<StencilRow>* rows,
-                  int32_t* ncells, std::string* why) {
+                  int32_t* ncells, int* reach, std::string* why) {
   const double Re = c->search_radius_e;
   int lo[3], hi[3];
   for (int d = 0; d < 3; ++d) {
@@ -177,6 +189,7 @@ int build_stencil(const ggem_params* p, const ggem_consts* c, std::vector<Stenci
     }
     lo[d] = std::max(lo[d], -w);
     hi[d] = std::min(hi[d], w);
+    reach[d] = 0;
   }
   rows->clear();
   int64_t cells = 0;
@@ -202,50 +215,14 @@ int build_stencil(const ggem_params* p, const ggem_consts* c, std::vector<Stenci
       if (dmin <= dmax) {
         rows->push_back({dj, dk, dmin, dmax});
         cells += dmax - dmin + 1;
+        reach[0] = std::max(reach[0], std::max(-dmin, dmax));
+        reach[1] = std::max(reach[1], std::abs(dj));
+        reach[2] = std::max(reach[2], std::abs(dk));
       }
     }
   *ncells = int32_t(cells);
   return GGEM_OK;
 }
-
-template <class T>
-int ensure(ggem_ctx* ctx, T** ptr, size_t* cap_elems, size_t elems) {
-  if (*ptr && *cap_elems >= elems) return GGEM_OK;
-  if (*ptr) {
-    (void)hipFree(*ptr);
-    *ptr = nullptr;
-    *cap_elems = 0;
-  }
-  if (elems == 0) return GGEM_OK;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), elems * sizeof(T));
-  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
-  *cap_elems = elems;
-  return GGEM_OK;
-}
-
-// Device scratch pools (grown on demand, freed with the context).
-struct Pools {
-  double* vec = nullptr;
-  size_t vec_cap = 0;
-  double* io = nullptr;
-  size_t io_cap = 0;
-  double* io2 = nullptr;
-  size_t io2_cap = 0;
-  int32_t* ids = nullptr;
-  size_t ids_cap = 0;
-  int64_t* offs = nullptr;
-  size_t offs_cap = 0;
-  double* red = nullptr;
-  size_t red_cap = 0;
-  double* fe = nullptr;
-  size_t fe_cap = 0;
-  double* rho = nullptr;
-  size_t rho_cap = 0;
-  double* ifd = nullptr;
-  size_t ifd_cap = 0;
-};
-
-Pools& pools(ggem_ctx* ctx) { return *reinterpret_cast<Pools*>(ctx->scratch); }
 
 void free_all(ggem_ctx* ctx) {
   auto f = [](void* p) {
@@ -259,8 +236,11 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->slot_of);
   f(ctx->slot_cell);
   f(ctx->xs);
+  f(ctx->x4);
   f(ctx->fs);
   f(ctx->fs32);
+  f(ctx->fg32);
+  f(ctx->fg64);
   f(ctx->u_loc);
   f(ctx->u_out);
   f(ctx->cell_count);
@@ -268,10 +248,21 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->scan_tmp);
   f(ctx->flag);
   f(ctx->pairs_d);
+  f(ctx->nbr_cnt);
+  f(ctx->slice_len);
+  f(ctx->slice_off);
+  f(ctx->scan64_tmp);
+  f(ctx->ell);
+  f(ctx->elist_off);
+  f(ctx->elist);
+  f(ctx->coef32);
+  f(ctx->coef64);
   f(ctx->noise);
   if (ctx->scratch) {
     Pools& P = pools(ctx);
     f(P.vec);
+    f(P.g32);
+    f(P.g64);
     f(P.io);
     f(P.io2);
     f(P.ids);
@@ -292,24 +283,33 @@ void free_all(ggem_ctx* ctx) {
   ctx->prof.pool.clear();
 }
 
-int check_ready(ggem_ctx* ctx) {
-  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
-  if (!ctx->built) return fail(ctx, GGEM_ERR_NOT_READY, "call ggem_build_neighbors after setting particles");
+int grow_scan64(ggem_ctx* ctx, int64_t len) {
+  const int64_t need = (len + 1023) / 1024 + 2;
+  if (ctx->scan64_tmp_len >= need) return GGEM_OK;
+  if (ctx->scan64_tmp) (void)hipFree(ctx->scan64_tmp);
+  ctx->scan64_tmp = nullptr;
+  ctx->scan64_tmp_len = 0;
+  GGEM_HIP(ctx, hipMalloc(&ctx->scan64_tmp, sizeof(int64_t) * need));
+  ctx->scan64_tmp_len = need;
   return GGEM_OK;
 }
 
-int sync(ggem_ctx* ctx) {
-  GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return GGEM_OK;
-}
-
-int d2h(ggem_ctx* ctx, void* host, const void* dev, size_t bytes) {
-  GGEM_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  return sync(ctx);
-}
-
-int h2d(ggem_ctx* ctx, void* dev, const void* host, size_t bytes) {
-  GGEM_HIP(ctx, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+// Element particle lists for the spreading, built on first use after a
+// neighbour build.
+int ensure_elist(ggem_ctx* ctx) {
+  if (ctx->elist_valid) return GGEM_OK;
+  const int64_t ne = ctx->grid.n_elem;
+  hipError_t e = launch_elist_count(ctx);
+  if (e == hipSuccess) e = launch_scan64(ctx, ctx->elist_off, ne, ctx->elist_off);
+  if (e != hipSuccess) return hip_fail(ctx, e, "element lists");
+  int64_t total = 0;
+  int rc = d2h(ctx, &total, ctx->elist_off + ne, sizeof(int64_t));
+  if (rc) return rc;
+  size_t cap = size_t(ctx->elist_cap);
+  if ((rc = ensure(ctx, &ctx->elist, &cap, size_t(std::max<int64_t>(total, 1))))) return rc;
+  ctx->elist_cap = int64_t(cap);
+  if ((e = launch_elist_fill(ctx)) != hipSuccess) return hip_fail(ctx, e, "element list fill");
+  ctx->elist_valid = true;
   return GGEM_OK;
 }
 
@@ -332,7 +332,8 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
   if (rc) return fail(nullptr, rc, why);
   std::vector<StencilRow> rows;
   int32_t ncells = 0;
-  rc = build_stencil(params, &c, &rows, &ncells, &why);
+  int reach[3];
+  rc = build_stencil(params, &c, &rows, &ncells, reach, &why);
   if (rc) return fail(nullptr, rc, why);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -354,6 +355,9 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
     ctx->grid.half_L[d] = c.L[d] / 2.0;
     ctx->grid.n[d] = params->n_elem[d];
     ctx->grid.periodic[d] = params->periodic[d] ? 1 : 0;
+    // a neighbour entry's single shift equals the minimum image while every
+    // pair stays inside half a box: (reach + 1) cells <= n/2
+    if (ctx->grid.periodic[d] && 2 * (reach[d] + 1) > params->n_elem[d]) ctx->exact_minimg = 1;
   }
   ctx->grid.n_elem = int64_t(params->n_elem[0]) * params->n_elem[1] * params->n_elem[2];
   ctx->gc.eps = c.one_eight_pi_mu;
@@ -367,6 +371,7 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
   ctx->gc.two_ksi_sqrt_pi = c.two_ksi_sqrt_pi;
   ctx->gc.coeff_2 = c.coeff_2;
   ctx->gc.point_type = params->point_type;
+  ctx->bsign = c.ksi >= c.alpha ? 1 : -1;  // sign of (c1 - c2): that of xi - alpha
   ctx->rows_h = rows;
   ctx->n_rows = int32_t(rows.size());
 
@@ -388,12 +393,14 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
       (e = hipMalloc(&ctx->cell_start, sizeof(int32_t) * (ne + 1))) != hipSuccess ||
       (e = hipMalloc(&ctx->scan_tmp, sizeof(int32_t) * (ntiles + 1))) != hipSuccess ||
       (e = hipMalloc(&ctx->flag, sizeof(int32_t))) != hipSuccess ||
-      (e = hipMalloc(&ctx->pairs_d, sizeof(int64_t))) != hipSuccess)
+      (e = hipMalloc(&ctx->pairs_d, sizeof(int64_t))) != hipSuccess ||
+      (e = hipMalloc(&ctx->elist_off, sizeof(int64_t) * (ne + 1))) != hipSuccess)
     return bail(e, "hipMalloc");
   if (!rows.empty()) {
     e = hipMemcpy(ctx->rows_d, rows.data(), sizeof(StencilRow) * rows.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(e, "hipMemcpy");
   }
+  if (grow_scan64(ctx, ne + 1) != GGEM_OK) return bail(hipErrorOutOfMemory, "hipMalloc");
   *out = ctx;
   return GGEM_OK;
 }
@@ -430,7 +437,7 @@ int ggem_sync(ggem_ctx* ctx) {
 int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double* f) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   if (n < 0 || (n > 0 && !xyz)) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid particle arrays");
-  if (n >= INT32_MAX) return fail(ctx, GGEM_ERR_CAPACITY, "particle count exceeds int32");
+  if (n >= kMaxParticles) return fail(ctx, GGEM_ERR_CAPACITY, "particle count exceeds 2^26");
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   if (n != ctx->n) ctx->elem_valid = false;
   if (n > ctx->cap) {
@@ -444,15 +451,26 @@ int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     f_(ctx->slot_of);
     f_(ctx->slot_cell);
     f_(ctx->xs);
+    f_(ctx->x4);
     f_(ctx->fs);
     f_(ctx->fs32);
+    f_(ctx->fg32);
+    f_(ctx->fg64);
     f_(ctx->u_loc);
     f_(ctx->u_out);
+    f_(ctx->nbr_cnt);
+    f_(ctx->slice_len);
+    f_(ctx->slice_off);
     ctx->x_in = ctx->f_in = ctx->xs = ctx->fs = ctx->u_loc = ctx->u_out = nullptr;
-    ctx->elem = ctx->perm = ctx->slot_of = ctx->slot_cell = nullptr;
+    ctx->elem = ctx->perm = ctx->slot_of = ctx->slot_cell = ctx->nbr_cnt = nullptr;
+    ctx->x4 = nullptr;
     ctx->fs32 = nullptr;
+    ctx->fg32 = nullptr;
+    ctx->fg64 = nullptr;
+    ctx->slice_len = ctx->slice_off = nullptr;
     ctx->cap = 0;
     const size_t n3 = size_t(3) * n;
+    const size_t nsl = size_t(n + kSlice - 1) / kSlice;
     GGEM_HIP(ctx, hipMalloc(&ctx->x_in, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->f_in, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->elem, sizeof(int32_t) * n));
@@ -460,15 +478,25 @@ int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     GGEM_HIP(ctx, hipMalloc(&ctx->slot_of, sizeof(int32_t) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->slot_cell, sizeof(int32_t) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->xs, sizeof(double) * n3));
+    GGEM_HIP(ctx, hipMalloc(&ctx->x4, sizeof(double4) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs32, sizeof(float) * n3));
+    GGEM_HIP(ctx, hipMalloc(&ctx->fg32, sizeof(float4) * n));
+    GGEM_HIP(ctx, hipMalloc(&ctx->fg64, sizeof(double4) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->u_loc, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->u_out, sizeof(double) * n3));
+    GGEM_HIP(ctx, hipMalloc(&ctx->nbr_cnt, sizeof(int32_t) * n));
+    GGEM_HIP(ctx, hipMalloc(&ctx->slice_len, sizeof(int64_t) * (nsl + 1)));
+    GGEM_HIP(ctx, hipMalloc(&ctx->slice_off, sizeof(int64_t) * (nsl + 2)));
+    int rc = grow_scan64(ctx, int64_t(nsl) + 1);
+    if (rc) return rc;
     ctx->cap = n;
     ctx->elem_valid = false;
   }
   ctx->n = n;
   ctx->built = false;
+  ctx->elist_valid = false;
+  ctx->coef32_valid = ctx->coef64_valid = false;
   ctx->pairs = -1;
   if (n > 0) {
     int rc = h2d(ctx, ctx->x_in, xyz, sizeof(double) * 3 * n);
@@ -487,7 +515,7 @@ int ggem_set_forces(ggem_ctx* ctx, const double* f) {
   int rc = h2d(ctx, ctx->f_in, f, sizeof(double) * 3 * ctx->n);
   if (rc) return rc;
   if (ctx->built) {
-    hipError_t e = launch_gather_forces(ctx, ctx->f_in, ctx->fs, ctx->fs32, 1, 0, 0);
+    hipError_t e = launch_gather_forces(ctx);
     if (e != hipSuccess) return hip_fail(ctx, e, "gather_forces");
   }
   return GGEM_OK;
@@ -497,26 +525,38 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   const bool hyst = (ctx->p.compat_flags & GGEM_COMPAT_ELEM_HYSTERESIS) && ctx->elem_valid;
+  ctx->built = false;
+  ctx->elist_valid = false;
+  ctx->coef32_valid = ctx->coef64_valid = false;
   hipError_t e = launch_bin(ctx, hyst);
   if (e != hipSuccess) return hip_fail(ctx, e, "bin");
+  if ((e = launch_scan(ctx)) != hipSuccess) return hip_fail(ctx, e, "scan");
+  if ((e = launch_place(ctx)) != hipSuccess) return hip_fail(ctx, e, "place");
+  if ((e = launch_gather_forces(ctx)) != hipSuccess) return hip_fail(ctx, e, "gather_forces");
+  const int64_t nsl = (ctx->n + kSlice - 1) / kSlice;
+  if ((e = launch_count_lists(ctx)) != hipSuccess) return hip_fail(ctx, e, "count");
+  if ((e = launch_scan64(ctx, ctx->slice_len, nsl, ctx->slice_off)) != hipSuccess)
+    return hip_fail(ctx, e, "scan64");
+  // one synchronisation: the out-of-domain flag, the pair count and the list size
   int32_t flag = 0;
-  int rc = d2h(ctx, &flag, ctx->flag, sizeof(int32_t));
+  int64_t tot[2] = {0, 0};
+  GGEM_HIP(ctx, hipMemcpyAsync(&flag, ctx->flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&tot[0], ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&tot[1], ctx->slice_off + nsl, sizeof(int64_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  int rc = sync(ctx);
   if (rc) return rc;
   if (flag) {
     ctx->elem_valid = false;
-    ctx->built = false;
     // point_mesh.C:602-612: a particle no element contains is fatal
     return fail(ctx, GGEM_ERR_OUT_OF_DOMAIN, "a particle lies outside the mesh");
   }
-  if ((e = launch_scan(ctx)) != hipSuccess) return hip_fail(ctx, e, "scan");
-  if ((e = launch_place(ctx)) != hipSuccess) return hip_fail(ctx, e, "place");
-  if ((e = launch_gather_forces(ctx, ctx->f_in, ctx->fs, ctx->fs32, 1, 0, 0)) != hipSuccess)
-    return hip_fail(ctx, e, "gather_forces");
-  if ((e = launch_count_pairs(ctx, ctx->pairs_d, nullptr)) != hipSuccess) return hip_fail(ctx, e, "count");
-  int64_t pairs = 0;
-  rc = d2h(ctx, &pairs, ctx->pairs_d, sizeof(int64_t));
-  if (rc) return rc;
-  ctx->pairs = pairs;
+  ctx->pairs = tot[0];
+  ctx->ell_len = tot[1];
+  size_t cap = size_t(ctx->ell_cap);
+  if ((rc = ensure(ctx, &ctx->ell, &cap, size_t(std::max<int64_t>(tot[1], 1))))) return rc;
+  ctx->ell_cap = int64_t(cap);
+  if ((e = launch_fill_ell(ctx)) != hipSuccess) return hip_fail(ctx, e, "fill_ell");
   ctx->elem_valid = true;
   ctx->built = true;
   return GGEM_OK;
@@ -544,7 +584,7 @@ int ggem_get_neighbor_csr(ggem_ctx* ctx, int64_t* offsets, int32_t* ids) {
   Pools& P = pools(ctx);
   const int64_t n = ctx->n;
   if ((rc = ensure(ctx, &P.offs, &P.offs_cap, size_t(n + 1)))) return rc;
-  hipError_t e = launch_count_pairs(ctx, ctx->pairs_d, P.offs + 1);
+  hipError_t e = launch_count_pairs(ctx, P.offs + 1);
   if (e != hipSuccess) return hip_fail(ctx, e, "count");
   std::vector<int64_t> cnt(n + 1, 0);
   if (n > 0 && (rc = d2h(ctx, cnt.data() + 1, P.offs + 1, sizeof(int64_t) * n))) return rc;
@@ -585,6 +625,7 @@ int ggem_get_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offs
 int ggem_local_velocity_beads(ggem_ctx* ctx, double* u) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = ensure_tensors(ctx))) return rc;
   hipError_t e = launch_pair_beads(ctx, ctx->p.precision, EPI_LOCAL, nullptr, ctx->u_loc);
   if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
   if (u && ctx->n > 0) return d2h(ctx, u, ctx->u_loc, sizeof(double) * 3 * ctx->n);
@@ -606,6 +647,7 @@ int ggem_combine_velocity(ggem_ctx* ctx, const double* u_glob, double* U) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   const int64_t len = 3 * ctx->n;
+  if ((rc = ensure_tensors(ctx))) return rc;
   Pools& P = pools(ctx);
   const double* ug = nullptr;
   if (u_glob && len > 0) {
@@ -658,6 +700,7 @@ int ggem_spread(ggem_ctx* ctx, int32_t nqp_1d, int32_t n_u, const double* IF, do
   if (rc) return rc;
   if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
   if (n_u < 1 || n_u > 27 || !IF) return fail(ctx, GGEM_ERR_INVALID_ARG, "n_u must be in [1, 27] with an IF table");
+  if ((rc = ensure_elist(ctx))) return rc;
   const int nqp = nqp_1d * nqp_1d * nqp_1d;
   const int64_t ne = ctx->grid.n_elem;
   Pools& P = pools(ctx);
@@ -695,315 +738,6 @@ int ggem_mobility_apply(ggem_ctx* ctx, const double* f, double* u) {
   if (f && (rc = ggem_set_forces(ctx, f))) return rc;
   return ggem_combine_velocity(ctx, nullptr, u);
 }
-
-}  // extern "C"
-
-// ---------------------------------------------------------------------------
-// Chebyshev driver
-namespace {
-
-// Chebyshev::chebyshev_coefficients, "Gauss" nodes (chebyshev.C:44-151)
-void cheb_coefficients(int N, double da, double db, std::vector<double>* coef) {
-  const double RN = double(N);
-  std::vector<double> x(N + 1), w(N + 1);
-  for (int i = 0; i <= N; ++i) {
-    const double j = double(i);
-    x[i] = std::cos((2. * j + 1.) * PI / (2. * RN + 2.));
-    w[i] = PI / (RN + 1.);
-  }
-  coef->assign(N + 1, 0.0);
-  for (int k = 0; k <= N; ++k) {
-    double s = 0.0;
-    for (int j = 0; j <= N; ++j) {
-      const double tmpt = PI * double(2. * j + 1.) * double(k) / (2. * RN + 2.);
-      const double fj = 1.0 / std::sqrt((x[j] - db) / da);
-      s += w[j] * std::cos(tmpt) * fj;
-    }
-    (*coef)[k] = s * (2. / (PI * (k == 0 ? 2.0 : 1.0)));
-  }
-}
-
-// Mobility application on slot-ordered vectors: device operator or a host
-// operator reached through the particle-order layout.
-struct Applier {
-  ggem_ctx* ctx;
-  const ggem_op* op;
-  int precision;
-  double* io_dev;  // 3n particle-order staging (host op)
-  std::vector<double> hf, hu;
-  int operator()(int nrhs, const double* x, double* v, const ChebEpi& epi) {
-    if (!op) {
-      hipError_t e = launch_apply_slots(ctx, precision, nrhs, x, v, epi);
-      return e == hipSuccess ? GGEM_OK : hip_fail(ctx, e, "apply");
-    }
-    const int64_t n = ctx->n, len = 3 * n;
-    hf.resize(len);
-    hu.resize(len);
-    for (int r = 0; r < nrhs; ++r) {
-      hipError_t e = launch_permute_out(ctx, 1, x + r * len, io_dev);
-      if (e != hipSuccess) return hip_fail(ctx, e, "permute");
-      int rc = d2h(ctx, hf.data(), io_dev, sizeof(double) * len);
-      if (rc) return rc;
-      if (op->apply(op->user, hf.data(), hu.data()) != 0)
-        return fail(ctx, GGEM_ERR_INVALID_ARG, "operator callback failed");
-      if ((rc = h2d(ctx, io_dev, hu.data(), sizeof(double) * len))) return rc;
-      double* vr = v + r * len;
-      const double* xr = x + r * len;
-      // v_r = M x_r, then the epilogue on the device vectors
-      if (epi.kind == 0) {
-        e = launch_permute_in(ctx, 1, io_dev, vr);
-      } else {
-        // M x lands in the second half of the staging buffer; v may alias
-        // x_{k-1}, so it is written only after x_{k-1} has been consumed.
-        double* mx = io_dev + len;
-        e = launch_permute_in(ctx, 1, io_dev, mx);
-        if (e == hipSuccess) {
-          if (epi.kind == 1) {
-            e = launch_axpby(ctx, len, epi.db, xr, epi.da, mx);  // db x + da M x
-          } else {
-            e = launch_axpby(ctx, len, 2. * epi.db, xr, 2. * epi.da, mx);  // 2db x + 2da M x
-            if (e == hipSuccess) e = launch_axpby(ctx, len, -1.0, epi.xm1 + r * len, 1.0, mx);
-            if (e == hipSuccess) e = launch_axpby(ctx, len, epi.c_host, mx, 1.0, epi.y + r * len);
-          }
-          if (e == hipSuccess)
-            e = hipMemcpyAsync(vr, mx, sizeof(double) * len, hipMemcpyDeviceToDevice, ctx->stream);
-        }
-      }
-      if (e != hipSuccess) return hip_fail(ctx, e, "host operator epilogue");
-    }
-    return GGEM_OK;
-  }
-};
-
-int dot_host(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out) {
-  Pools& P = pools(ctx);
-  int rc = ensure(ctx, &P.red, &P.red_cap, size_t(nrhs) * 513);
-  if (rc) return rc;
-  hipError_t e = launch_dot(ctx, nrhs, a, b, P.red + size_t(nrhs) * 512, P.red);
-  if (e != hipSuccess) return hip_fail(ctx, e, "dot");
-  return d2h(ctx, out, P.red + size_t(nrhs) * 512, sizeof(double) * nrhs);
-}
-
-}  // namespace
-
-extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, double eig_min,
-                                  double eig_max, double tol, int32_t mode, int32_t nrhs,
-                                  const double* dw, double* y, double* disp, ggem_cheb_info* info) {
-  int rc = check_ready(ctx);
-  if (rc) return rc;
-  if (N < 1 || nrhs < 1 || !(eig_max > eig_min) || (mode != GGEM_CHEB_REFERENCE && mode != GGEM_CHEB_FIXED))
-    return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Chebyshev arguments");
-  if (!dw && (ctx->noise_nrhs != nrhs || !ctx->noise))
-    return fail(ctx, GGEM_ERR_NOT_READY, "no device noise of this nrhs: call ggem_set_noise");
-  if (op && !op->apply) return fail(ctx, GGEM_ERR_INVALID_ARG, "operator without apply");
-  const int64_t n = ctx->n, len = 3 * n;
-  if (n == 0) return GGEM_OK;
-  const double d_eig = eig_max - eig_min;
-  const double da = 2.0 / d_eig;
-  const double db = -(eig_max + eig_min) / d_eig;
-  std::vector<double> X;
-  cheb_coefficients(N, da, db, &X);
-
-  // device vectors [nrhs][3][n] by slot: x0, x1, y, dwm, plus staging
-  Pools& P = pools(ctx);
-  const size_t blk = size_t(nrhs) * len;
-  if ((rc = ensure(ctx, &P.vec, &P.vec_cap, 4 * blk))) return rc;
-  if ((rc = ensure(ctx, &P.io, &P.io_cap, std::max(blk, size_t(2 * len))))) return rc;
-  double* x0 = P.vec;
-  double* x1 = P.vec + blk;
-  double* yv = P.vec + 2 * blk;
-  double* dwm = P.vec + 3 * blk;
-  const double* dw_dev = ctx->noise;
-  if (dw) {
-    if ((rc = h2d(ctx, P.io, dw, sizeof(double) * blk))) return rc;
-    dw_dev = P.io;
-  }
-  hipError_t e = launch_permute_in(ctx, nrhs, dw_dev, x0);
-  if (e != hipSuccess) return hip_fail(ctx, e, "permute");
-  Applier apply{ctx, op, ctx->p.precision, P.io, {}, {}};
-
-  std::vector<ggem_cheb_info> inf(nrhs);
-  for (auto& i : inf) i = ggem_cheb_info{0, 1, 0, 0, 0.0, 0.0};
-  ChebEpi plain{0, 0.0, 0.0, nullptr, 0.0, nullptr, nullptr};
-
-  if (mode == GGEM_CHEB_FIXED) {
-    ChebEpi init{1, da, db, nullptr, 0.0, nullptr, nullptr};
-    if ((rc = apply(nrhs, x0, x1, init))) return rc;  // x1 = db x0 + da M x0
-    if ((e = launch_fill(ctx, blk, 0.0, yv)) != hipSuccess) return hip_fail(ctx, e, "fill");
-    if ((e = launch_axpby(ctx, blk, X[0], x0, 1.0, yv)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-    if ((e = launch_axpby(ctx, blk, X[1], x1, 1.0, yv)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-    for (int i = 1; i < N; ++i) {
-      // x_{k+1} = 2da M x_k + 2db x_k - x_{k-1} into x0's storage; y += c x_{k+1}
-      ChebEpi st{2, da, db, nullptr, X[i + 1], x0, yv};
-      if ((rc = apply(nrhs, x1, x0, st))) return rc;
-      std::swap(x0, x1);
-    }
-    for (auto& i : inf) {
-      i.converged = 1;
-      i.terms = N;
-      i.applies = N + (disp ? 1 : 0);
-    }
-  } else {
-    // reference-compatible: one right-hand side at a time
-    std::vector<double> h(1);
-    for (int r = 0; r < nrhs; ++r) {
-      double* a0 = x0 + r * len;
-      double* a1 = x1 + r * len;
-      double* ay = yv + r * len;
-      double* am = dwm + r * len;
-      int applies = 0;
-      if ((rc = apply(1, a0, a1, plain))) return rc;  // x1 = M x0
-      ++applies;
-      double aux2, aux1, aux3, aux4;
-      if ((rc = dot_host(ctx, 1, a1, a0, &aux2))) return rc;
-      if ((e = launch_axpby(ctx, len, db, a0, da, a1)) != hipSuccess) return hip_fail(ctx, e, "axpby");
-      if ((e = launch_fill(ctx, len, 0.0, ay)) != hipSuccess) return hip_fail(ctx, e, "fill");
-      if ((e = launch_axpby(ctx, len, X[0], a0, 1.0, ay)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-      if ((rc = dot_host(ctx, 1, ay, ay, &aux4))) return rc;
-      aux4 = std::sqrt(aux4);
-      if ((e = launch_axpby(ctx, len, X[1], a1, 1.0, ay)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-      if ((rc = dot_host(ctx, 1, a1, a1, &aux3))) return rc;
-      aux3 = std::sqrt(aux3) * std::fabs(X[1]);
-      if ((rc = apply(1, ay, am, plain))) return rc;
-      ++applies;
-      if ((rc = dot_host(ctx, 1, am, am, &aux1))) return rc;
-      double err1 = 0, err2 = 0;
-      if (aux2 != 0.0) err1 = std::sqrt(std::fabs(aux1 - aux2) / aux2);
-      if (aux4 != 0.0) err2 = aux3 / aux4;
-      bool conv = false;
-      int terms = 1;
-      for (int i = 1; i < N; ++i) {
-        if ((rc = dot_host(ctx, 1, ay, ay, &aux4))) return rc;
-        aux4 = std::sqrt(aux4);
-        ChebEpi st{2, da, db, nullptr, X[i + 1], a0, ay};
-        if ((rc = apply(1, a1, a0, st))) return rc;  // a0 <- x_{k+1}; y += c x_{k+1}
-        ++applies;
-        terms = i + 1;
-        if ((rc = dot_host(ctx, 1, a0, a0, &aux3))) return rc;
-        aux3 = std::sqrt(aux3) * std::fabs(X[i + 1]);
-        if ((rc = apply(1, ay, am, plain))) return rc;
-        ++applies;
-        if ((rc = dot_host(ctx, 1, am, am, &aux1))) return rc;
-        if (aux2 != 0.0) err1 = std::sqrt(std::fabs(aux1 - aux2) / aux2);
-        if (aux4 != 0.0) err2 = aux3 / aux4;
-        if (err1 <= tol) {
-          conv = true;
-          break;
-        }
-        if (err2 <= 1E-9) break;
-        std::swap(a0, a1);
-      }
-      inf[r] = ggem_cheb_info{conv ? 1 : 0, terms, applies + (disp ? 1 : 0), 0, err1, err2};
-    }
-  }
-  if (y) {
-    if ((e = launch_permute_out(ctx, nrhs, yv, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
-    if ((rc = d2h(ctx, y, P.io, sizeof(double) * blk))) return rc;
-  }
-  if (disp) {  // hi_ewald: disp = M y (copss.C:1686)
-    if ((rc = apply(nrhs, yv, dwm, plain))) return rc;
-    if ((e = launch_permute_out(ctx, nrhs, dwm, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
-    if ((rc = d2h(ctx, disp, P.io, sizeof(double) * blk))) return rc;
-  }
-  if (info)
-    for (int r = 0; r < nrhs; ++r) info[r] = inf[r];
-  return sync(ctx);
-}
-
-// ---------------------------------------------------------------------------
-// Lanczos extreme eigenvalues
-namespace {
-
-// number of eigenvalues of the symmetric tridiagonal (a, b) below x (Sturm)
-int sturm_count(const std::vector<double>& a, const std::vector<double>& b, double x) {
-  int cnt = 0;
-  double q = 1.0;
-  for (size_t i = 0; i < a.size(); ++i) {
-    const double bb = i ? b[i - 1] * b[i - 1] : 0.0;
-    q = a[i] - x - (i ? bb / q : 0.0);
-    if (q == 0.0) q = 1e-300;
-    if (q < 0) ++cnt;
-  }
-  return cnt;
-}
-
-double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int k) {
-  // k-th smallest eigenvalue (0-based) by bisection on the Gershgorin interval
-  double lo = 1e300, hi = -1e300;
-  for (size_t i = 0; i < a.size(); ++i) {
-    const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < a.size() ? std::fabs(b[i]) : 0.0);
-    lo = std::min(lo, a[i] - r);
-    hi = std::max(hi, a[i] + r);
-  }
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid == lo || mid == hi) break;
-    if (sturm_count(a, b, mid) > k) hi = mid;
-    else lo = mid;
-  }
-  return 0.5 * (lo + hi);
-}
-
-}  // namespace
-
-extern "C" int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,
-                                   double* lmin, double* lmax) {
-  int rc = check_ready(ctx);
-  if (rc) return rc;
-  if (iters < 1 || !lmin || !lmax) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Lanczos arguments");
-  if (op && !op->apply) return fail(ctx, GGEM_ERR_INVALID_ARG, "operator without apply");
-  const int64_t len = 3 * ctx->n;
-  if (len == 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no particles");
-  Pools& P = pools(ctx);
-  if ((rc = ensure(ctx, &P.vec, &P.vec_cap, 4 * size_t(len)))) return rc;
-  if ((rc = ensure(ctx, &P.io, &P.io_cap, size_t(2 * len)))) return rc;
-  double* vprev = P.vec;
-  double* vcur = P.vec + len;
-  double* w = P.vec + 2 * len;
-  std::vector<double> h(len);
-  std::mt19937_64 gen(seed);
-  std::normal_distribution<double> nd(0.0, 1.0);
-  double s = 0;
-  for (auto& v : h) {
-    v = nd(gen);
-    s += v * v;
-  }
-  s = 1.0 / std::sqrt(s);
-  for (auto& v : h) v *= s;
-  if ((rc = h2d(ctx, vcur, h.data(), sizeof(double) * len))) return rc;
-  hipError_t e = launch_fill(ctx, len, 0.0, vprev);
-  if (e != hipSuccess) return hip_fail(ctx, e, "fill");
-  Applier apply{ctx, op, ctx->p.precision, P.io, {}, {}};
-  ChebEpi plain{0, 0.0, 0.0, nullptr, 0.0, nullptr, nullptr};
-  std::vector<double> alpha, beta;
-  double bprev = 0.0;
-  const int kmax = int(std::min<int64_t>(iters, len));
-  for (int j = 0; j < kmax; ++j) {
-    if ((rc = apply(1, vcur, w, plain))) return rc;
-    double a;
-    if ((rc = dot_host(ctx, 1, w, vcur, &a))) return rc;
-    alpha.push_back(a);
-    if ((e = launch_axpby(ctx, len, -a, vcur, 1.0, w)) != hipSuccess) return hip_fail(ctx, e, "axpby");
-    if ((e = launch_axpby(ctx, len, -bprev, vprev, 1.0, w)) != hipSuccess) return hip_fail(ctx, e, "axpby");
-    double b2;
-    if ((rc = dot_host(ctx, 1, w, w, &b2))) return rc;
-    const double b = std::sqrt(b2);
-    if (j + 1 == kmax || b <= 1e-12 * std::fabs(a)) break;
-    beta.push_back(b);
-    // vprev <- vcur; vcur <- w / b
-    std::swap(vprev, vcur);
-    if ((e = launch_fill(ctx, len, 0.0, vcur)) != hipSuccess) return hip_fail(ctx, e, "fill");
-    if ((e = launch_axpby(ctx, len, 1.0 / b, w, 0.0, vcur)) != hipSuccess) return hip_fail(ctx, e, "axpby");
-    bprev = b;
-  }
-  beta.resize(alpha.size() > 0 ? alpha.size() - 1 : 0);
-  *lmin = tridiag_eig(alpha, beta, 0);
-  *lmax = tridiag_eig(alpha, beta, int(alpha.size()) - 1);
-  return GGEM_OK;
-}
-
-// ---------------------------------------------------------------------------
-extern "C" {
 
 int ggem_profile_enable(ggem_ctx* ctx, int32_t enable) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");

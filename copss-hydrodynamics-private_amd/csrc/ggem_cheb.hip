@@ -1,0 +1,340 @@
+// ggem_cheb.hip -- the Brownian-step drivers of the C-ABI: the Chebyshev
+// approximation of M^{-1/2} dw with the reference's estimators and stopping
+// rule or at fixed order, and the Lanczos extreme-eigenvalue estimate.
+//
+// Vectors live on the device in slot order (FP64, [nrhs][3][n]) with a
+// per-slot gather record for the neighbour reads of the next application;
+// the mobility application and the three-term update are one kernel
+// (kernels_pair.hip).
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "ggem_host.h"
+
+using namespace ggem;
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+// Chebyshev::chebyshev_coefficients, "Gauss" nodes (chebyshev.C:44-151)
+void cheb_coefficients(int N, double da, double db, std::vector<double>* coef) {
+  const double RN = double(N);
+  std::vector<double> x(N + 1), w(N + 1);
+  for (int i = 0; i <= N; ++i) {
+    const double j = double(i);
+    x[i] = std::cos((2. * j + 1.) * PI / (2. * RN + 2.));
+    w[i] = PI / (RN + 1.);
+  }
+  coef->assign(N + 1, 0.0);
+  for (int k = 0; k <= N; ++k) {
+    double s = 0.0;
+    for (int j = 0; j <= N; ++j) {
+      const double tmpt = PI * double(2. * j + 1.) * double(k) / (2. * RN + 2.);
+      const double fj = 1.0 / std::sqrt((x[j] - db) / da);
+      s += w[j] * std::cos(tmpt) * fj;
+    }
+    (*coef)[k] = s * (2. / (PI * (k == 0 ? 2.0 : 1.0)));
+  }
+}
+
+// A set of nrhs slot vectors carved out of the pools.
+SlotVec slot_vec(Pools& P, int which, int nrhs, int64_t n, int precision) {
+  const size_t blk = size_t(nrhs) * 3 * n, gblk = size_t(nrhs) * n;
+  SlotVec v;
+  v.soa = P.vec + which * blk;
+  v.g32 = precision == GGEM_FP32C ? P.g32 + which * gblk : nullptr;
+  v.g64 = precision == GGEM_FP64 ? P.g64 + which * gblk : nullptr;
+  return v;
+}
+
+SlotVec offset(const SlotVec& v, int r, int64_t n) {
+  return SlotVec{v.soa + int64_t(r) * 3 * n, v.g32 ? v.g32 + int64_t(r) * n : nullptr,
+                 v.g64 ? v.g64 + int64_t(r) * n : nullptr};
+}
+
+int reserve(ggem_ctx* ctx, int nvec, int nrhs) {
+  Pools& P = pools(ctx);
+  const int64_t n = ctx->n;
+  int rc = ensure(ctx, &P.vec, &P.vec_cap, size_t(nvec) * nrhs * 3 * n);
+  if (!rc) rc = ensure(ctx, &P.g32, &P.g32_cap, size_t(nvec) * nrhs * n);
+  if (!rc) rc = ensure(ctx, &P.g64, &P.g64_cap, size_t(nvec) * nrhs * n);
+  if (!rc) rc = ensure(ctx, &P.io, &P.io_cap, std::max(size_t(nrhs) * 3 * n, size_t(6 * n)));
+  return rc;
+}
+
+int refresh(ggem_ctx* ctx, int nrhs, const SlotVec& v) {
+  hipError_t e = launch_make_gather(ctx, nrhs, v.soa, v.g32, v.g64);
+  return e == hipSuccess ? GGEM_OK : hip_fail(ctx, e, "gather records");
+}
+
+// v = M x (+ epilogue), on the device or through a host operator.
+struct Applier {
+  ggem_ctx* ctx;
+  const ggem_op* op;
+  int precision;
+  std::vector<double> hf, hu;
+  int operator()(int nrhs, const SlotVec& x, const SlotVec& v, const ChebEpi& epi) {
+    if (!op) {
+      hipError_t e = launch_apply(ctx, precision, nrhs, x, v, epi);
+      return e == hipSuccess ? GGEM_OK : hip_fail(ctx, e, "apply");
+    }
+    const int64_t n = ctx->n, len = 3 * n;
+    double* io = pools(ctx).io;  // 6n: particle-order staging and M x by slot
+    hf.resize(len);
+    hu.resize(len);
+    for (int r = 0; r < nrhs; ++r) {
+      const SlotVec xr = offset(x, r, n), vr = offset(v, r, n);
+      hipError_t e = launch_permute_out(ctx, 1, xr.soa, io);
+      if (e != hipSuccess) return hip_fail(ctx, e, "permute");
+      int rc = d2h(ctx, hf.data(), io, sizeof(double) * len);
+      if (rc) return rc;
+      if (op->apply(op->user, hf.data(), hu.data()) != 0)
+        return fail(ctx, GGEM_ERR_INVALID_ARG, "operator callback failed");
+      if ((rc = h2d(ctx, io, hu.data(), sizeof(double) * len))) return rc;
+      double* mx = io + len;
+      e = launch_permute_in(ctx, 1, io, mx);
+      // v may alias x_{k-1}: it is written only after x_{k-1} is consumed
+      if (e == hipSuccess && epi.kind == 1) {
+        e = launch_axpby(ctx, len, epi.db, xr.soa, epi.da, mx);  // db x + da M x
+      } else if (e == hipSuccess && epi.kind == 2) {
+        e = launch_axpby(ctx, len, 2. * epi.db, xr.soa, 2. * epi.da, mx);  // 2db x + 2da M x
+        if (e == hipSuccess) e = launch_axpby(ctx, len, -1.0, epi.xm1.soa + r * len, 1.0, mx);
+        if (e == hipSuccess) e = launch_axpby(ctx, len, epi.c, mx, 1.0, epi.y + r * len);
+      }
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(vr.soa, mx, sizeof(double) * len, hipMemcpyDeviceToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "host operator epilogue");
+      if ((rc = refresh(ctx, 1, vr))) return rc;
+    }
+    return GGEM_OK;
+  }
+};
+
+int dot_host(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out) {
+  Pools& P = pools(ctx);
+  int rc = ensure(ctx, &P.red, &P.red_cap, size_t(nrhs) * 513);
+  if (rc) return rc;
+  hipError_t e = launch_dot(ctx, nrhs, a, b, P.red + size_t(nrhs) * 512, P.red);
+  if (e != hipSuccess) return hip_fail(ctx, e, "dot");
+  return d2h(ctx, out, P.red + size_t(nrhs) * 512, sizeof(double) * nrhs);
+}
+
+// number of eigenvalues of the symmetric tridiagonal (a, b) below x (Sturm)
+int sturm_count(const std::vector<double>& a, const std::vector<double>& b, double x) {
+  int cnt = 0;
+  double q = 1.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const double bb = i ? b[i - 1] * b[i - 1] : 0.0;
+    q = a[i] - x - (i ? bb / q : 0.0);
+    if (q == 0.0) q = 1e-300;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+// k-th smallest eigenvalue (0-based) by bisection on the Gershgorin interval
+double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int k) {
+  double lo = 1e300, hi = -1e300;
+  for (size_t i = 0; i < a.size(); ++i) {
+    const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < a.size() ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid == lo || mid == hi) break;
+    if (sturm_count(a, b, mid) > k) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, double eig_min,
+                                  double eig_max, double tol, int32_t mode, int32_t nrhs,
+                                  const double* dw, double* y, double* disp, ggem_cheb_info* info) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (N < 1 || nrhs < 1 || !(eig_max > eig_min) || (mode != GGEM_CHEB_REFERENCE && mode != GGEM_CHEB_FIXED))
+    return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Chebyshev arguments");
+  if (!dw && (ctx->noise_nrhs != nrhs || !ctx->noise))
+    return fail(ctx, GGEM_ERR_NOT_READY, "no device noise of this nrhs: call ggem_set_noise");
+  if (op && !op->apply) return fail(ctx, GGEM_ERR_INVALID_ARG, "operator without apply");
+  const int64_t n = ctx->n, len = 3 * n;
+  if (n == 0) return GGEM_OK;
+  const double d_eig = eig_max - eig_min;
+  const double da = 2.0 / d_eig;
+  const double db = -(eig_max + eig_min) / d_eig;
+  std::vector<double> X;
+  cheb_coefficients(N, da, db, &X);
+
+  if ((rc = reserve(ctx, 4, nrhs))) return rc;
+  if (!op && (rc = ensure_tensors(ctx))) return rc;
+  Pools& P = pools(ctx);
+  const int prec = ctx->p.precision;
+  SlotVec x0 = slot_vec(P, 0, nrhs, n, prec);
+  SlotVec x1 = slot_vec(P, 1, nrhs, n, prec);
+  SlotVec yv = slot_vec(P, 2, nrhs, n, prec);
+  SlotVec dwm = slot_vec(P, 3, nrhs, n, prec);
+  const size_t blk = size_t(nrhs) * len;
+  const double* dw_dev = ctx->noise;
+  if (dw) {
+    if ((rc = h2d(ctx, P.io, dw, sizeof(double) * blk))) return rc;
+    dw_dev = P.io;
+  }
+  hipError_t e = launch_permute_in(ctx, nrhs, dw_dev, x0.soa);
+  if (e != hipSuccess) return hip_fail(ctx, e, "permute");
+  if ((rc = refresh(ctx, nrhs, x0))) return rc;
+  Applier apply{ctx, op, prec, {}, {}};
+  const SlotVec none{nullptr, nullptr, nullptr};
+  const ChebEpi plain{0, 0.0, 0.0, 0.0, none, nullptr};
+  std::vector<ggem_cheb_info> inf(nrhs);
+  for (auto& i : inf) i = ggem_cheb_info{0, 1, 0, 0, 0.0, 0.0};
+
+  if (mode == GGEM_CHEB_FIXED) {
+    const ChebEpi init{1, da, db, 0.0, none, nullptr};
+    if ((rc = apply(nrhs, x0, x1, init))) return rc;  // x1 = db x0 + da M x0
+    if ((e = launch_fill(ctx, blk, 0.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "fill");
+    if ((e = launch_axpby(ctx, blk, X[0], x0.soa, 1.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
+    if ((e = launch_axpby(ctx, blk, X[1], x1.soa, 1.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
+    for (int i = 1; i < N; ++i) {
+      // x_{k+1} = 2da M x_k + 2db x_k - x_{k-1} into x_{k-1}'s storage; y += c x_{k+1}
+      const ChebEpi st{2, da, db, X[i + 1], x0, yv.soa};
+      if ((rc = apply(nrhs, x1, x0, st))) return rc;
+      std::swap(x0, x1);
+    }
+    for (auto& i : inf) {
+      i.converged = 1;
+      i.terms = N;
+      i.applies = N + (disp ? 1 : 0);
+    }
+  } else {
+    // reference-compatible (brownian_system.C:1030-1155), one right-hand side at a time
+    for (int r = 0; r < nrhs; ++r) {
+      SlotVec a0 = offset(x0, r, n), a1 = offset(x1, r, n);
+      const SlotVec ay = offset(yv, r, n), am = offset(dwm, r, n);
+      int applies = 0;
+      if ((rc = apply(1, a0, a1, plain))) return rc;  // x1 = M x0
+      ++applies;
+      double aux1, aux2, aux3, aux4;
+      if ((rc = dot_host(ctx, 1, a1.soa, a0.soa, &aux2))) return rc;
+      if ((e = launch_axpby(ctx, len, db, a0.soa, da, a1.soa)) != hipSuccess) return hip_fail(ctx, e, "axpby");
+      if ((rc = refresh(ctx, 1, a1))) return rc;
+      if ((e = launch_fill(ctx, len, 0.0, ay.soa)) != hipSuccess) return hip_fail(ctx, e, "fill");
+      if ((e = launch_axpby(ctx, len, X[0], a0.soa, 1.0, ay.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
+      if ((rc = dot_host(ctx, 1, ay.soa, ay.soa, &aux4))) return rc;
+      aux4 = std::sqrt(aux4);
+      if ((e = launch_axpby(ctx, len, X[1], a1.soa, 1.0, ay.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
+      if ((rc = dot_host(ctx, 1, a1.soa, a1.soa, &aux3))) return rc;
+      aux3 = std::sqrt(aux3) * std::fabs(X[1]);
+      if ((rc = refresh(ctx, 1, ay))) return rc;
+      if ((rc = apply(1, ay, am, plain))) return rc;
+      ++applies;
+      if ((rc = dot_host(ctx, 1, am.soa, am.soa, &aux1))) return rc;
+      double err1 = 0, err2 = 0;
+      if (aux2 != 0.0) err1 = std::sqrt(std::fabs(aux1 - aux2) / aux2);
+      if (aux4 != 0.0) err2 = aux3 / aux4;
+      bool conv = false;
+      int terms = 1;
+      for (int i = 1; i < N; ++i) {
+        if ((rc = dot_host(ctx, 1, ay.soa, ay.soa, &aux4))) return rc;
+        aux4 = std::sqrt(aux4);
+        const ChebEpi st{2, da, db, X[i + 1], a0, ay.soa};
+        if ((rc = apply(1, a1, a0, st))) return rc;  // a0 <- x_{k+1}; y += c x_{k+1}
+        ++applies;
+        terms = i + 1;
+        if ((rc = dot_host(ctx, 1, a0.soa, a0.soa, &aux3))) return rc;
+        aux3 = std::sqrt(aux3) * std::fabs(X[i + 1]);
+        if ((rc = refresh(ctx, 1, ay))) return rc;
+        if ((rc = apply(1, ay, am, plain))) return rc;
+        ++applies;
+        if ((rc = dot_host(ctx, 1, am.soa, am.soa, &aux1))) return rc;
+        if (aux2 != 0.0) err1 = std::sqrt(std::fabs(aux1 - aux2) / aux2);
+        if (aux4 != 0.0) err2 = aux3 / aux4;
+        if (err1 <= tol) {
+          conv = true;
+          break;
+        }
+        if (err2 <= 1E-9) break;
+        std::swap(a0, a1);
+      }
+      inf[r] = ggem_cheb_info{conv ? 1 : 0, terms, applies + (disp ? 1 : 0), 0, err1, err2};
+    }
+  }
+  if (y) {
+    if ((e = launch_permute_out(ctx, nrhs, yv.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
+    if ((rc = d2h(ctx, y, P.io, sizeof(double) * blk))) return rc;
+  }
+  if (disp) {  // hi_ewald: disp = M y (copss.C:1686)
+    if ((rc = refresh(ctx, nrhs, yv))) return rc;
+    if ((rc = apply(nrhs, yv, dwm, plain))) return rc;
+    if ((e = launch_permute_out(ctx, nrhs, dwm.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
+    if ((rc = d2h(ctx, disp, P.io, sizeof(double) * blk))) return rc;
+  }
+  if (info)
+    for (int r = 0; r < nrhs; ++r) info[r] = inf[r];
+  return y || disp ? sync(ctx) : GGEM_OK;
+}
+
+extern "C" int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,
+                                   double* lmin, double* lmax) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (iters < 1 || !lmin || !lmax) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Lanczos arguments");
+  if (op && !op->apply) return fail(ctx, GGEM_ERR_INVALID_ARG, "operator without apply");
+  const int64_t n = ctx->n, len = 3 * n;
+  if (len == 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no particles");
+  if ((rc = reserve(ctx, 3, 1))) return rc;
+  if (!op && (rc = ensure_tensors(ctx))) return rc;
+  Pools& P = pools(ctx);
+  const int prec = ctx->p.precision;
+  SlotVec vprev = slot_vec(P, 0, 1, n, prec);
+  SlotVec vcur = slot_vec(P, 1, 1, n, prec);
+  const SlotVec w = slot_vec(P, 2, 1, n, prec);
+  std::vector<double> h(len);
+  std::mt19937_64 gen(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  double s = 0;
+  for (auto& v : h) {
+    v = nd(gen);
+    s += v * v;
+  }
+  s = 1.0 / std::sqrt(s);
+  for (auto& v : h) v *= s;
+  if ((rc = h2d(ctx, vcur.soa, h.data(), sizeof(double) * len))) return rc;
+  if ((rc = refresh(ctx, 1, vcur))) return rc;
+  hipError_t e = launch_fill(ctx, len, 0.0, vprev.soa);
+  if (e != hipSuccess) return hip_fail(ctx, e, "fill");
+  Applier apply{ctx, op, prec, {}, {}};
+  const SlotVec none{nullptr, nullptr, nullptr};
+  const ChebEpi plain{0, 0.0, 0.0, 0.0, none, nullptr};
+  std::vector<double> alpha, beta;
+  double bprev = 0.0;
+  const int kmax = int(std::min<int64_t>(iters, len));
+  for (int j = 0; j < kmax; ++j) {
+    if ((rc = apply(1, vcur, w, plain))) return rc;
+    double a;
+    if ((rc = dot_host(ctx, 1, w.soa, vcur.soa, &a))) return rc;
+    alpha.push_back(a);
+    if ((e = launch_axpby(ctx, len, -a, vcur.soa, 1.0, w.soa)) != hipSuccess) return hip_fail(ctx, e, "axpby");
+    if ((e = launch_axpby(ctx, len, -bprev, vprev.soa, 1.0, w.soa)) != hipSuccess) return hip_fail(ctx, e, "axpby");
+    double b2;
+    if ((rc = dot_host(ctx, 1, w.soa, w.soa, &b2))) return rc;
+    const double b = std::sqrt(b2);
+    if (j + 1 == kmax || b <= 1e-12 * std::fabs(a)) break;
+    beta.push_back(b);
+    std::swap(vprev, vcur);  // vprev <- v_j; vcur <- w / b
+    if ((e = launch_fill(ctx, len, 0.0, vcur.soa)) != hipSuccess) return hip_fail(ctx, e, "fill");
+    if ((e = launch_axpby(ctx, len, 1.0 / b, w.soa, 0.0, vcur.soa)) != hipSuccess) return hip_fail(ctx, e, "axpby");
+    if ((rc = refresh(ctx, 1, vcur))) return rc;
+    bprev = b;
+  }
+  beta.resize(alpha.size() > 0 ? alpha.size() - 1 : 0);
+  *lmin = tridiag_eig(alpha, beta, 0);
+  *lmax = tridiag_eig(alpha, beta, int(alpha.size()) - 1);
+  return GGEM_OK;
+}

@@ -1,7 +1,7 @@
 // ggem_device.h -- device-side building blocks shared by the kernels:
 // minimum-image displacement, the element-stencil walk over the cell-sorted
-// particle store, and the regularized-Stokeslet pair term in FP64 and in
-// FP32 with compensated accumulation.
+// particle store, XCD-aware block numbering, and the regularized-Stokeslet
+// pair coefficients in FP64 and FP32.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +19,12 @@ __device__ __forceinline__ double min_image(double d, double L, double half_L, i
   return d;
 }
 
+// Single periodic shift of a neighbour entry (code: 2 bits per axis).
+__device__ __forceinline__ double apply_shift(double d, uint32_t code2, double L) {
+  const int w = int(code2 & 1u) - int((code2 >> 1) & 1u);
+  return w ? __builtin_fma(double(w), L, d) : d;
+}
+
 __device__ __forceinline__ void cell_ijk(const Grid& g, int64_t e, int& i, int& j, int& k) {
   const int64_t nx = g.n[0], nxy = int64_t(g.n[0]) * g.n[1];
   k = int(e / nxy);
@@ -27,10 +33,19 @@ __device__ __forceinline__ void cell_ijk(const Grid& g, int64_t e, int& i, int& 
   i = int(r - int64_t(j) * nx);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
+// XCD works on one contiguous range of slots and neighbouring targets share
+// its L2.  Bijective on [0, nb).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, rmd = nb % 8;
+  const int64_t x = b % 8, i = b / 8;
+  return x * q + (x < rmd ? x : rmd) + i;
+}
+
 // Walk the particle slots of every element of the stencil of element
-// (ci, cj, ck): calls seg(a, b) for each contiguous slot range [a, b).  The
-// rows are wave-uniform, so their loads are scalar.  Ranges come row by row,
-// x-contiguous inside a row; on a periodic x axis a row can split in two.
+// (ci, cj, ck): calls seg(a, b, code) for each contiguous slot range [a, b)
+// with the periodic shift code of that range.  Rows are wave-uniform, so
+// their loads are scalar.
 template <class Seg>
 __device__ __forceinline__ void for_each_segment(const Grid& g, const StencilRow* __restrict__ rows,
                                                  int nrows, const int32_t* __restrict__ cell_start,
@@ -39,34 +54,67 @@ __device__ __forceinline__ void for_each_segment(const Grid& g, const StencilRow
   for (int r = 0; r < nrows; ++r) {
     const StencilRow R = rows[r];
     int jj = cj + R.dj, kk = ck + R.dk;
-    if (g.periodic[1]) {
-      jj += (jj < 0) ? ny : 0;
-      jj -= (jj >= ny) ? ny : 0;
-    } else if (jj < 0 || jj >= ny) {
-      continue;
+    uint32_t cy = 0, cz = 0;
+    if (jj < 0 || jj >= ny) {
+      if (!g.periodic[1]) continue;
+      // a neighbour cell reached across the low face sits near the top:
+      // d ~ +L, shift -L (code 2); across the high face: d ~ -L, +L (code 1)
+      if (jj < 0) {
+        jj += ny;
+        cy = 2;
+      } else {
+        jj -= ny;
+        cy = 1;
+      }
     }
-    if (g.periodic[2]) {
-      kk += (kk < 0) ? nz : 0;
-      kk -= (kk >= nz) ? nz : 0;
-    } else if (kk < 0 || kk >= nz) {
-      continue;
+    if (kk < 0 || kk >= nz) {
+      if (!g.periodic[2]) continue;
+      if (kk < 0) {
+        kk += nz;
+        cz = 2;
+      } else {
+        kk -= nz;
+        cz = 1;
+      }
     }
+    const uint32_t cyz = (cy << 2) | (cz << 4);
     const int64_t base = int64_t(nx) * (jj + int64_t(ny) * kk);
     int lo = ci + R.di_lo, hi = ci + R.di_hi;
     if (g.periodic[0]) {
       if (lo < 0) {
-        seg(cell_start[base + lo + nx], cell_start[base + nx]);
+        seg(cell_start[base + lo + nx], cell_start[base + nx], cyz | 2u);
         lo = 0;
       }
       if (hi >= nx) {
-        seg(cell_start[base], cell_start[base + hi - nx + 1]);
+        seg(cell_start[base], cell_start[base + hi - nx + 1], cyz | 1u);
         hi = nx - 1;
       }
     } else {
       lo = lo < 0 ? 0 : lo;
       hi = hi >= nx ? nx - 1 : hi;
     }
-    if (lo <= hi) seg(cell_start[base + lo], cell_start[base + hi + 1]);
+    if (lo <= hi) seg(cell_start[base + lo], cell_start[base + hi + 1], cyz);
+  }
+}
+
+// Displacement x_t - x_s of a neighbour entry: the raw difference, then
+// either the entry's periodic shift or, in exact mode, the full
+// minimum-image test.  Both are the reference's arithmetic: one subtraction,
+// at most one shift by the box length.
+template <bool EXACT>
+__device__ __forceinline__ void pair_disp(const Grid& g, const double4& s, const double4& t,
+                                          uint32_t code, double& dx, double& dy, double& dz) {
+  dx = t.x - s.x;
+  dy = t.y - s.y;
+  dz = t.z - s.z;
+  if (EXACT) {
+    dx = min_image(dx, g.L[0], g.half_L[0], g.periodic[0]);
+    dy = min_image(dy, g.L[1], g.half_L[1], g.periodic[1]);
+    dz = min_image(dz, g.L[2], g.half_L[2], g.periodic[2]);
+  } else if (code) {
+    dx = apply_shift(dx, code, g.L[0]);
+    dy = apply_shift(dy, code >> 2, g.L[1]);
+    dz = apply_shift(dz, code >> 4, g.L[2]);
   }
 }
 
@@ -75,22 +123,28 @@ __device__ __forceinline__ void for_each_segment(const Grid& g, const StencilRow
 // with c1 = (erf(xi r) - erf(alpha r))/r and
 // c2 = 2xi/sqrt(pi) e^{-xi^2 r^2} - 2alpha/sqrt(pi) e^{-alpha^2 r^2};
 // r < 1e-6 gives coeff_2 f.
-__device__ __forceinline__ void green_acc_f64(const GGEMConst& k, double dx, double dy, double dz,
-                                              double fx, double fy, double fz, double& ux,
-                                              double& uy, double& uz) {
-  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+__device__ __forceinline__ void green_coef_f64(const GGEMConst& k, double r2, double& A,
+                                               double& Bx) {
+  const double r = sqrt(r2);
   if (r < 1e-6) {
-    ux += k.coeff_2 * fx;
-    uy += k.coeff_2 * fy;
-    uz += k.coeff_2 * fz;
+    A = k.coeff_2;
+    Bx = 0.0;
     return;
   }
-  const double r2 = r * r;
+  const double rr = r * r;
   const double kr = k.ksi * r, ar = k.alpha * r;
   const double c1 = (erf(kr) - erf(ar)) / r;
   const double c2 = k.two_ksi_sqrt_pi * exp(-kr * kr) - k.two_alpha_sqrt_pi * exp(-ar * ar);
-  const double A = k.eps * (c1 + c2);
-  const double B = k.eps * (c1 - c2) * ((dx * fx + dy * fy + dz * fz) / r2);
+  A = k.eps * (c1 + c2);
+  Bx = k.eps * (c1 - c2) / rr;
+}
+
+__device__ __forceinline__ void green_acc_f64(const GGEMConst& k, double dx, double dy, double dz,
+                                              double fx, double fy, double fz, double& ux,
+                                              double& uy, double& uz) {
+  double A, Bx;
+  green_coef_f64(k, dx * dx + dy * dy + dz * dz, A, Bx);
+  const double B = Bx * (dx * fx + dy * fy + dz * fz);
   ux += A * fx + B * dx;
   uy += A * fy + B * dy;
   uz += A * fz + B * dz;
@@ -109,26 +163,69 @@ struct KahanF {
 };
 
 // FP32 constants of the pair term.
+constexpr int kErfcxDeg = 10;
 struct GGEMConst32 {
   float eps, alpha, ksi, two_alpha_sqrt_pi, two_ksi_sqrt_pi, coeff_2;
-  float erf_switch;  // alpha r below this: erf difference; above: erfc difference
+  float erf_switch;   // alpha r below this: erf difference; above: erfc difference
+  float erfcx_scale;  // 2 / t_max of the erfcx fit
+  float erfcx_c[kErfcxDeg + 1];
 };
 
-// Pair coefficients A, B of G f = A f + B (x.f) x in FP32.  The erf
-// difference is taken as erfc(alpha r) - erfc(xi r) once alpha r passes the
-// switch, which keeps its relative error at a few ulp where both erf values
-// approach 1.
-__device__ __forceinline__ void green_coef_f32(const GGEMConst32& k, float r2, float& A,
-                                               float& Bx) {
-  const float r = __builtin_sqrtf(r2);
-  const float kr = k.ksi * r, ar = k.alpha * r;
+// e^{-z^2} in FP32 to about one ulp: z^2 and z^2 log2(e) are carried with
+// their rounding errors (FMA) into a first-order correction of 2^y.
+__device__ __forceinline__ float exp_neg_sq(float z) {
+  constexpr float L2E = 1.44269502163f;        // log2(e) rounded to FP32
+  constexpr float L2E_LO = 1.925963033e-08f;   // log2(e) - L2E
+  constexpr float LN2 = 0.693147180560f;
+  const float s = z * z;
+  const float se = __builtin_fmaf(z, z, -s);
+  const float y = -s * L2E;
+  const float ye = __builtin_fmaf(-s, L2E, -y) - __builtin_fmaf(s, L2E_LO, se * L2E);
+  return __builtin_amdgcn_exp2f(y) * __builtin_fmaf(ye, LN2, 1.0f);
+}
+
+__device__ __forceinline__ float rcp_nr(float a) {
+  const float t0 = __builtin_amdgcn_rcpf(a);
+  const float e = __builtin_fmaf(-a, t0, 1.0f);
+  return __builtin_fmaf(t0, e, t0);
+}
+
+// Scaled complementary error function e^{z^2} erfc(z) for z >= 0.45:
+// t = 1/(1 + z/2), erfcx = t Q(2t/t_max - 1), Q a degree-10 least-squares
+// fit of erfcx(z)(1 + z/2) (relative error ~1e-7 in FP32).
+__device__ __forceinline__ float erfcx_f32(const GGEMConst32& k, float z) {
+  const float t = rcp_nr(__builtin_fmaf(0.5f, z, 1.0f));
+  const float x = __builtin_fmaf(t, k.erfcx_scale, -1.0f);
+  float q = k.erfcx_c[kErfcxDeg];
+#pragma unroll
+  for (int i = kErfcxDeg - 1; i >= 0; --i) q = __builtin_fmaf(q, x, k.erfcx_c[i]);
+  return t * q;
+}
+
+// Pair coefficients A, Bx of G f = A f + Bx (x.f) x in FP32.  Beyond the
+// switch the erf difference is taken as erfc(alpha r) - erfc(xi r) =
+// e^{-alpha^2 r^2} erfcx(alpha r) - e^{-xi^2 r^2} erfcx(xi r), sharing the two
+// exponentials with c2; this keeps a few-ulp relative error where both erf
+// values approach 1.
+__device__ __forceinline__ void green_coef_f32(const GGEMConst32& k, float r2, float& A, float& Bx) {
+  if (r2 < 1e-12f) {  // r < r_eps (ggem_stokes.C:312-318)
+    A = k.coeff_2;
+    Bx = 0.f;
+    return;
+  }
+  // 1/r from the hardware reciprocal square root with one Newton step
+  float ir = __builtin_amdgcn_rsqf(r2);
+  ir = ir * __builtin_fmaf(-0.5f * r2, ir * ir, 1.5f);
+  const float r = r2 * ir;
+  const float ar = k.alpha * r, kr = k.ksi * r;
+  const float ea = exp_neg_sq(ar), ek = exp_neg_sq(kr);
   float d;
-  if (ar < k.erf_switch) d = erff(kr) - erff(ar);
-  else d = erfcf(ar) - erfcf(kr);
-  const float c1 = d / r;
-  const float c2 = k.two_ksi_sqrt_pi * __expf(-kr * kr) - k.two_alpha_sqrt_pi * __expf(-ar * ar);
+  if (ar >= k.erf_switch) d = ea * erfcx_f32(k, ar) - ek * erfcx_f32(k, kr);
+  else d = erff(kr) - erff(ar);
+  const float c1 = d * ir;
+  const float c2 = k.two_ksi_sqrt_pi * ek - k.two_alpha_sqrt_pi * ea;
   A = k.eps * (c1 + c2);
-  Bx = k.eps * (c1 - c2) / r2;
+  Bx = k.eps * (c1 - c2) * (ir * ir);
 }
 
 }  // namespace ggem

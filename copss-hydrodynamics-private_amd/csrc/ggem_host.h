@@ -1,0 +1,80 @@
+// ggem_host.h -- host-side helpers shared by the C-ABI translation units:
+// device buffer growth, scratch pools, synchronous copies.
+#pragma once
+
+#include <algorithm>
+#include <string>
+
+#include "ggem_internal.h"
+
+namespace ggem {
+
+template <class T>
+int ensure(ggem_ctx* ctx, T** ptr, size_t* cap_elems, size_t elems) {
+  if (*ptr && *cap_elems >= elems) return GGEM_OK;
+  if (*ptr) {
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+    *cap_elems = 0;
+  }
+  if (elems == 0) return GGEM_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), elems * sizeof(T));
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+  *cap_elems = elems;
+  return GGEM_OK;
+}
+
+// Device scratch pools (grown on demand, freed with the context).
+struct Pools {
+  double* vec = nullptr;   // Chebyshev / Lanczos vectors (slot SoA)
+  size_t vec_cap = 0;
+  float4* g32 = nullptr;   // their FP32 gather records
+  size_t g32_cap = 0;
+  double4* g64 = nullptr;  // their FP64 gather records
+  size_t g64_cap = 0;
+  double* io = nullptr;
+  size_t io_cap = 0;
+  double* io2 = nullptr;
+  size_t io2_cap = 0;
+  int32_t* ids = nullptr;
+  size_t ids_cap = 0;
+  int64_t* offs = nullptr;
+  size_t offs_cap = 0;
+  double* red = nullptr;
+  size_t red_cap = 0;
+  double* fe = nullptr;
+  size_t fe_cap = 0;
+  double* rho = nullptr;
+  size_t rho_cap = 0;
+  double* ifd = nullptr;
+  size_t ifd_cap = 0;
+};
+
+inline Pools& pools(ggem_ctx* ctx) { return *reinterpret_cast<Pools*>(ctx->scratch); }
+
+// Pair tensors of the current lists in the context's precision, evaluated
+// on first use after a neighbour build.
+int ensure_tensors(ggem_ctx* ctx);
+
+inline int check_ready(ggem_ctx* ctx) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (!ctx->built) return fail(ctx, GGEM_ERR_NOT_READY, "call ggem_build_neighbors after setting particles");
+  return GGEM_OK;
+}
+
+inline int sync(ggem_ctx* ctx) {
+  GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GGEM_OK;
+}
+
+inline int d2h(ggem_ctx* ctx, void* host, const void* dev, size_t bytes) {
+  GGEM_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return sync(ctx);
+}
+
+inline int h2d(ggem_ctx* ctx, void* dev, const void* host, size_t bytes) {
+  GGEM_HIP(ctx, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return GGEM_OK;
+}
+
+}  // namespace ggem

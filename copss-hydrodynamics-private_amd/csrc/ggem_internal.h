@@ -33,20 +33,19 @@ struct Grid {
 };
 
 struct GGEMConst {
-  double eps;              // 1/(8 pi mu)
+  double eps;  // 1/(8 pi mu)
   double alpha, alpha2, alpha3_pi_23, two_alpha_sqrt_pi, coeff_1;
   double ksi, ksi2, two_ksi_sqrt_pi, coeff_2;
   int32_t point_type;
 };
 
-// Device pointers of the particle store in cell order ("slots").
-struct Store {
-  const double* xs;   // [3][n] SoA positions by slot
-  const int32_t* cell_start;  // [n_elem + 1]
-  const int32_t* slot_cell;   // [n] element of each slot
-  const int32_t* perm;        // slot -> particle id
-  int64_t n;
-};
+// Neighbour entries carry the particle slot in the low 26 bits and the
+// periodic image of the pair in the high 6 bits: 2 bits per axis, 1 = add
+// the box length, 2 = subtract it (the single shift of point_vector).
+constexpr uint32_t kSlotBits = 26;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
+constexpr int64_t kMaxParticles = int64_t(1) << kSlotBits;
+constexpr int kSlice = 64;  // targets per neighbour-list slice (one wave)
 
 struct Profiler {
   struct Rec {
@@ -74,6 +73,10 @@ struct ggem_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // periodic shifts are taken from the neighbour entries unless the stencil
+  // reaches half a box on some periodic axis; then every pair is reduced by
+  // the explicit minimum-image test
+  int exact_minimg = 0;
 
   // stencil
   std::vector<ggem::StencilRow> rows_h;
@@ -91,8 +94,11 @@ struct ggem_ctx {
   int32_t* slot_of = nullptr; // [n] particle -> slot
   int32_t* slot_cell = nullptr;
   double* xs = nullptr;       // [3][n] sorted SoA positions
+  double4* x4 = nullptr;      // [n] sorted positions, one 32-byte record per slot
   double* fs = nullptr;       // [3][n] sorted SoA forces (FP64)
   float* fs32 = nullptr;      // [3][n] sorted SoA forces (FP32)
+  float4* fg32 = nullptr;     // [n] forces by slot, gather copy (FP32)
+  double4* fg64 = nullptr;    // [n] forces by slot, gather copy (FP64)
   double* u_loc = nullptr;    // [3n] interleaved, particle order
   double* u_out = nullptr;    // [3n] interleaved, particle order
   int32_t* cell_count = nullptr;  // [n_elem]
@@ -104,6 +110,32 @@ struct ggem_ctx {
   int64_t pairs = -1;
   bool built = false;
 
+  // particle neighbour lists, sliced ELL: slice q holds targets 64q..64q+63,
+  // entry k of lane l at ell[slice_off[q] + 64 k + l]
+  int32_t* nbr_cnt = nullptr;     // [n] list length per slot
+  int64_t* slice_len = nullptr;   // [n/64 + 1] 64 x max length, scanned into slice_off
+  int64_t* slice_off = nullptr;   // [n/64 + 2]
+  int64_t* scan64_tmp = nullptr;
+  int64_t scan64_tmp_len = 0;
+  uint32_t* ell = nullptr;
+  int64_t ell_cap = 0;
+  int64_t ell_len = 0;
+  // pair tensors of the current lists, same layout as ell:
+  // G = A I + sgn w w^T stored as (A, w) with w = sqrt(|Bx|) x_ij
+  float4* coef32 = nullptr;
+  int64_t coef32_cap = 0;
+  bool coef32_valid = false;
+  double4* coef64 = nullptr;
+  int64_t coef64_cap = 0;
+  bool coef64_valid = false;
+  int bsign = 1;  // sign of Bx: +1 when xi > alpha
+
+  // element particle lists (spreading), CSR over elements
+  int64_t* elist_off = nullptr;   // [n_elem + 1]
+  uint32_t* elist = nullptr;
+  int64_t elist_cap = 0;
+  bool elist_valid = false;
+
   // device-resident Brownian noise (particle order, interleaved)
   double* noise = nullptr;
   int64_t noise_cap = 0;
@@ -111,9 +143,6 @@ struct ggem_ctx {
 
   // generic scratch (grown on demand)
   void* scratch = nullptr;
-  size_t scratch_bytes = 0;
-  double* qscratch = nullptr;    // query points etc.
-  size_t qscratch_bytes = 0;
 
   ggem::Profiler prof;
 };
@@ -123,9 +152,9 @@ namespace ggem {
 // error helpers
 int fail(ggem_ctx* ctx, int code, const std::string& msg);
 int hip_fail(ggem_ctx* ctx, hipError_t e, const char* what);
-#define GGEM_HIP(ctx, expr)                                   \
-  do {                                                        \
-    hipError_t _e = (expr);                                   \
+#define GGEM_HIP(ctx, expr)                                      \
+  do {                                                           \
+    hipError_t _e = (expr);                                      \
     if (_e != hipSuccess) return ggem::hip_fail(ctx, _e, #expr); \
   } while (0)
 
@@ -141,43 +170,56 @@ struct ProfScope {
   ~ProfScope() { prof_end(ctx, name, a, b); }
 };
 
-Store store_of(const ggem_ctx* ctx);
-
 // ---- launchers (kernels_build.hip) ----
 hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis);
 hipError_t launch_scan(ggem_ctx* ctx);
 hipError_t launch_place(ggem_ctx* ctx);
-hipError_t launch_gather_forces(ggem_ctx* ctx, const double* f_interleaved, double* fs, float* fs32,
-                                int nrhs, int64_t stride_in, int64_t stride_out);
-hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* out_d, int64_t* counts_d);
+hipError_t launch_gather_forces(ggem_ctx* ctx);
+// per-slot counts, per-slice lengths and the total pair count
+hipError_t launch_count_lists(ggem_ctx* ctx);
+// exclusive scan of int64 values; out[len] receives the total
+hipError_t launch_scan64(ggem_ctx* ctx, const int64_t* in, int64_t len, int64_t* out);
+hipError_t launch_fill_ell(ggem_ctx* ctx);
+hipError_t launch_elist_count(ggem_ctx* ctx);
+hipError_t launch_elist_fill(ggem_ctx* ctx);
+hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* counts_d);
 hipError_t launch_fill_csr(ggem_ctx* ctx, const int64_t* offsets_d, int32_t* ids_d);
 hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets_d,
                                  int32_t* ids_d, int pass);
 hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* elem);
 
 // ---- launchers (kernels_pair.hip) ----
-// Epilogue of the bead pair sum:
-enum Epi : int {
-  EPI_LOCAL = 0,    // out = U_loc (particle order, interleaved)
-  EPI_MOBILITY = 1, // out = U_loc - coeff_1 f + f (+ type rules)
+// A vector operand of the pair sum: FP64 SoA by slot for the element-wise
+// algebra plus a one-record-per-slot copy for the neighbour gathers.
+struct SlotVec {
+  double* soa;    // [3][n]
+  float4* g32;    // [n]
+  double4* g64;   // [n]
 };
+
+// Epilogue of the pair sum, per target slot s with sum m_s = sum_t G f_t:
+enum Epi : int {
+  EPI_LOCAL = 0,     // out[3p+c] = U_loc  (p = particle id, interleaved)
+  EPI_MOBILITY = 1,  // out[3p+c] = U_glob + U_loc - coeff_1 f + f (type rules)
+  EPI_CHEB = 2,      // Chebyshev step on slot vectors (ChebEpi)
+};
+struct ChebEpi {
+  int kind;            // 0: v = M x; 1: v = db x + da M x; 2: v = 2db x + 2da M x - x_{k-1}, y += c v
+  double da, db;
+  double c;            // coefficient of kind 2
+  SlotVec xm1;         // x_{k-1} (kind 2); may alias v
+  double* y;           // [nrhs][3][n] (kind 2)
+};
+// pair tensors of every list entry for the given precision
+hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,
                              double* out);
 hipError_t launch_pair_points(ggem_ctx* ctx, int precision, int64_t m, const double* pts,
                               const int32_t* elem, double* u);
-
-// Chebyshev-facing operator on slot-ordered SoA vectors: for r < nrhs,
-// v = M x[r] (x, v: [nrhs][3][n] by slot), then the recurrence epilogue.
-struct ChebEpi {
-  int kind;            // 0: v = M x; 1: v = da M x + db x; 2: v = 2da M x + 2db x - xm1, y += c v
-  double da, db;
-  const double* coef;  // device pointer to the coefficient used by kind 2 (may be null)
-  double c_host;
-  const double* xm1;   // [nrhs][3][n] (kind 2)
-  double* y;           // [nrhs][3][n] (kind 2)
-};
-hipError_t launch_apply_slots(ggem_ctx* ctx, int precision, int nrhs, const double* x, double* v,
-                              const ChebEpi& epi);
+// v = M x (+ epilogue) for nrhs slot vectors (x, v: nrhs consecutive SlotVecs
+// of stride 3n / n)
+hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x, const SlotVec& v,
+                        const ChebEpi& epi);
 
 // ---- launchers (kernels_spread.hip) ----
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
@@ -186,7 +228,9 @@ hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, cons
 // ---- launchers (kernels_vec.hip) ----
 hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in_interleaved, double* out_slots);
 hipError_t launch_permute_out(ggem_ctx* ctx, int nrhs, const double* in_slots, double* out_interleaved);
-// partial dot products over [nrhs][3n] blocks: out[r] = sum a_r * b_r (deterministic)
+// gather copies of nrhs slot SoA vectors
+hipError_t launch_make_gather(ggem_ctx* ctx, int nrhs, const double* soa, float4* g32, double4* g64);
+// out[r] = sum a_r * b_r over [nrhs][3n] blocks (deterministic)
 hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
                       double* partials);
 hipError_t launch_axpby(ggem_ctx* ctx, int64_t len, double a, const double* x, double b, double* y);

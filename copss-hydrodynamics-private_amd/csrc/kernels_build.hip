@@ -148,7 +148,7 @@ __global__ void k_place(int64_t n, const int32_t* __restrict__ elem,
 __global__ void k_sort_cells(int64_t n_elem, int64_t n, const int32_t* __restrict__ cell_start,
                              int32_t* __restrict__ perm, int32_t* __restrict__ slot_of,
                              int32_t* __restrict__ slot_cell, const double* __restrict__ x,
-                             double* __restrict__ xs) {
+                             double* __restrict__ xs, double4* __restrict__ x4) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
   const int32_t a = cell_start[e], b = cell_start[e + 1];
@@ -165,28 +165,174 @@ __global__ void k_sort_cells(int64_t n_elem, int64_t n, const int32_t* __restric
     const int32_t p = perm[s];
     slot_of[p] = s;
     slot_cell[s] = int32_t(e);
-    xs[s] = x[3 * int64_t(p)];
-    xs[n + s] = x[3 * int64_t(p) + 1];
-    xs[2 * n + s] = x[3 * int64_t(p) + 2];
+    const double px = x[3 * int64_t(p)], py = x[3 * int64_t(p) + 1], pz = x[3 * int64_t(p) + 2];
+    xs[s] = px;
+    xs[n + s] = py;
+    xs[2 * n + s] = pz;
+    x4[s] = make_double4(px, py, pz, 0.0);
   }
 }
 
-// Forces (nrhs blocks of interleaved particle-order vectors) -> slot SoA.
-__global__ void k_gather_forces(int64_t n, int nrhs, const int32_t* __restrict__ perm,
-                                const double* __restrict__ f, int64_t stride_in,
-                                double* __restrict__ fs, float* __restrict__ fs32,
-                                int64_t stride_out) {
+// Forces (interleaved, particle order) -> slot SoA (FP64, FP32) and the
+// per-slot gather copies.
+__global__ void k_gather_forces(int64_t n, const int32_t* __restrict__ perm,
+                                const double* __restrict__ f, double* __restrict__ fs,
+                                float* __restrict__ fs32, float4* __restrict__ fg32,
+                                double4* __restrict__ fg64) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int64_t p = perm[s];
-  for (int r = 0; r < nrhs; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double v = f[r * stride_in + 3 * p + c];
-      if (fs) fs[r * stride_out + c * n + s] = v;
-      if (fs32) fs32[r * stride_out + c * n + s] = float(v);
-    }
+  const double fx = f[3 * p], fy = f[3 * p + 1], fz = f[3 * p + 2];
+  fs[s] = fx;
+  fs[n + s] = fy;
+  fs[2 * n + s] = fz;
+  fs32[s] = float(fx);
+  fs32[n + s] = float(fy);
+  fs32[2 * n + s] = float(fz);
+  fg32[s] = make_float4(float(fx), float(fy), float(fz), 0.f);
+  fg64[s] = make_double4(fx, fy, fz, 0.0);
+}
+
+// Neighbour-list lengths by slot (own element + stencil, minus itself), the
+// slice lengths (64 x longest list of each wave of 64 slots) and the total.
+__global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                              const int32_t* __restrict__ cell_start,
+                              const int32_t* __restrict__ slot_cell, int64_t n,
+                              int32_t* __restrict__ cnt, int64_t* __restrict__ slice_len,
+                              unsigned long long* __restrict__ total) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t c = 0;
+  if (s < n) {
+    int ci, cj, ck;
+    cell_ijk(g, slot_cell[s], ci, cj, ck);
+    for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
+                     [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
+    c -= 1;
+    cnt[s] = int32_t(c);
   }
+  int64_t mx = c, sum = c;
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = max(mx, (int64_t)__shfl_down(mx, off, 64));
+    sum += __shfl_down(sum, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const int64_t q = s >> 6;
+    if (q * 64 < n) slice_len[q] = mx * 64;
+    if (sum) atomicAdd(total, (unsigned long long)sum);
+  }
+}
+
+__global__ void k_fill_ell(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                           const int32_t* __restrict__ cell_start,
+                           const int32_t* __restrict__ slot_cell, int64_t n,
+                           const int64_t* __restrict__ slice_off, uint32_t* __restrict__ ell) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int ci, cj, ck;
+  cell_ijk(g, slot_cell[s], ci, cj, ck);
+  uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
+  int64_t k = 0;
+  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
+    const uint32_t hi = code << kSlotBits;
+    for (int32_t t = a; t < b; ++t) {
+      if (t == s) continue;
+      out[k * kSlice] = uint32_t(t) | hi;
+      ++k;
+    }
+  });
+}
+
+// Element particle lists for the spreading: per element, the slots of its
+// stencil cells (own element included) with their periodic shift codes.
+__global__ void k_elist_count(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                              const int32_t* __restrict__ cell_start, int64_t* __restrict__ cnt) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= g.n_elem) return;
+  int ci, cj, ck;
+  cell_ijk(g, e, ci, cj, ck);
+  int64_t c = 0;
+  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
+                   [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
+  cnt[e] = c;
+}
+
+__global__ void k_elist_fill(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                             const int32_t* __restrict__ cell_start, const int64_t* __restrict__ off,
+                             uint32_t* __restrict__ elist) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= g.n_elem) return;
+  int ci, cj, ck;
+  cell_ijk(g, e, ci, cj, ck);
+  uint32_t* out = elist + off[e];
+  int64_t k = 0;
+  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
+    const uint32_t hi = code << kSlotBits;
+    for (int32_t t = a; t < b; ++t) out[k++] = uint32_t(t) | hi;
+  });
+}
+
+// Exclusive scan of int64 values (three phases, as the int32 scan).
+constexpr int kScan64Threads = 256;
+constexpr int kScan64Items = 4;
+constexpr int kScan64Tile = kScan64Threads * kScan64Items;
+
+__device__ __forceinline__ int64_t block_exclusive_scan64(int64_t v, int64_t* sh, int64_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScan64Threads; off <<= 1) {
+    const int64_t add = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  total = sh[kScan64Threads - 1];
+  const int64_t incl = sh[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan64_tiles(const int64_t* __restrict__ in, int64_t len, int64_t* __restrict__ out,
+                               int64_t* __restrict__ tile_sums) {
+  __shared__ int64_t sh[kScan64Threads];
+  const int64_t base = int64_t(blockIdx.x) * kScan64Tile + int64_t(threadIdx.x) * kScan64Items;
+  int64_t v[kScan64Items];
+  int64_t s = 0;
+#pragma unroll
+  for (int q = 0; q < kScan64Items; ++q) {
+    v[q] = (base + q < len) ? in[base + q] : 0;
+    s += v[q];
+  }
+  int64_t total;
+  int64_t run = block_exclusive_scan64(s, sh, total);
+#pragma unroll
+  for (int q = 0; q < kScan64Items; ++q) {
+    if (base + q < len) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan64_sums(int64_t* __restrict__ sums, int64_t ntiles) {
+  __shared__ int64_t sh[kScan64Threads];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < ntiles; c0 += kScan64Threads) {
+    const int64_t i = c0 + threadIdx.x;
+    const int64_t v = (i < ntiles) ? sums[i] : 0;
+    int64_t total;
+    const int64_t ex = block_exclusive_scan64(v, sh, total);
+    if (i < ntiles) sums[i] = ex + carry;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[ntiles] = carry;
+}
+
+__global__ void k_scan64_add(int64_t* __restrict__ out, int64_t len, const int64_t* __restrict__ sums,
+                             int64_t ntiles) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < len) out[i] += sums[i / kScan64Tile];
+  if (i == 0) out[len] = sums[ntiles];
 }
 
 // Neighbour count of every particle (own element + stencil, minus itself).
@@ -201,7 +347,7 @@ __global__ void k_count_pairs(Grid g, const StencilRow* __restrict__ rows, int n
     int ci, cj, ck;
     cell_ijk(g, slot_cell[s], ci, cj, ck);
     for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
-                     [&](int32_t a, int32_t b) { cnt += b - a; });
+                     [&](int32_t a, int32_t b, uint32_t) { cnt += b - a; });
     cnt -= 1;
     if (counts) counts[perm[s]] = cnt;
   }
@@ -224,7 +370,7 @@ __device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows,
   for (int32_t t = oa; t < ob; ++t)
     if (t != self_slot) out[w++] = t;
   const int64_t head = w;
-  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b) {
+  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t) {
     for (int32_t t = a; t < b; ++t)
       if (t < oa || t >= ob) out[w++] = t;
   });
@@ -262,7 +408,7 @@ __global__ void k_elem_point(Grid g, const StencilRow* __restrict__ rows, int nr
     cell_ijk(g, e, ci, cj, ck);
     int64_t cnt = 0;
     for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
-                     [&](int32_t a, int32_t b) { cnt += b - a; });
+                     [&](int32_t a, int32_t b, uint32_t) { cnt += b - a; });
     offsets[q + 1] = cnt;
   } else {
     emit_list(g, rows, nrows, cell_start, e, -1, perm, ids + offsets[q]);
@@ -310,27 +456,71 @@ hipError_t launch_place(ggem_ctx* ctx) {
                        ctx->elem, ctx->cell_start, ctx->cell_count, ctx->perm);
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
                      ctx->grid.n_elem, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
-                     ctx->slot_cell, ctx->x_in, ctx->xs);
+                     ctx->slot_cell, ctx->x_in, ctx->xs, ctx->x4);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_forces(ggem_ctx* ctx, const double* f, double* fs, float* fs32, int nrhs,
-                                int64_t stride_in, int64_t stride_out) {
+hipError_t launch_gather_forces(ggem_ctx* ctx) {
   if (ctx->n == 0) return hipSuccess;
   ProfScope ps(ctx, "gather_forces");
   hipLaunchKernelGGL(k_gather_forces, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
-                     nrhs, ctx->perm, f, stride_in, fs, fs32, stride_out);
+                     ctx->perm, ctx->f_in, ctx->fs, ctx->fs32, ctx->fg32, ctx->fg64);
   return hipGetLastError();
 }
 
-hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* total_d, int64_t* counts_d) {
-  hipError_t e = hipMemsetAsync(total_d, 0, sizeof(int64_t), ctx->stream);
-  if (e != hipSuccess) return e;
+hipError_t launch_count_lists(ggem_ctx* ctx) {
+  hipError_t e = hipMemsetAsync(ctx->pairs_d, 0, sizeof(int64_t), ctx->stream);
+  if (e != hipSuccess || ctx->n == 0) return e;
+  ProfScope ps(ctx, "count_lists");
+  hipLaunchKernelGGL(k_count_lists, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->n, ctx->nbr_cnt,
+                     ctx->slice_len, reinterpret_cast<unsigned long long*>(ctx->pairs_d));
+  return hipGetLastError();
+}
+
+hipError_t launch_scan64(ggem_ctx* ctx, const int64_t* in, int64_t len, int64_t* out) {
+  const int64_t ntiles = (len + kScan64Tile - 1) / kScan64Tile;
+  if (ctx->scan64_tmp_len < ntiles + 1) return hipErrorInvalidValue;
+  ProfScope ps(ctx, "scan64");
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_scan64_tiles, dim3(unsigned(ntiles)), dim3(kScan64Threads), 0, ctx->stream, in,
+                       len, out, ctx->scan64_tmp);
+  hipLaunchKernelGGL(k_scan64_sums, dim3(1), dim3(kScan64Threads), 0, ctx->stream, ctx->scan64_tmp,
+                     ntiles);
+  hipLaunchKernelGGL(k_scan64_add, dim3(blocks_for(len + 1)), dim3(kBlock), 0, ctx->stream, out, len,
+                     ctx->scan64_tmp, ntiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ell(ggem_ctx* ctx) {
   if (ctx->n == 0) return hipSuccess;
-  ProfScope ps(ctx, "count_pairs");
+  ProfScope ps(ctx, "fill_ell");
+  hipLaunchKernelGGL(k_fill_ell, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->n, ctx->slice_off,
+                     ctx->ell);
+  return hipGetLastError();
+}
+
+hipError_t launch_elist_count(ggem_ctx* ctx) {
+  ProfScope ps(ctx, "elist_count");
+  hipLaunchKernelGGL(k_elist_count, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
+                     ctx->grid, ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->elist_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_elist_fill(ggem_ctx* ctx) {
+  ProfScope ps(ctx, "elist_fill");
+  hipLaunchKernelGGL(k_elist_fill, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
+                     ctx->grid, ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->elist_off, ctx->elist);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* counts_d) {
+  hipError_t e = hipMemsetAsync(ctx->pairs_d, 0, sizeof(int64_t), ctx->stream);
+  if (e != hipSuccess || ctx->n == 0) return e;
   hipLaunchKernelGGL(k_count_pairs, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
                      ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->n,
-                     counts_d, reinterpret_cast<unsigned long long*>(total_d));
+                     counts_d, reinterpret_cast<unsigned long long*>(ctx->pairs_d));
   return hipGetLastError();
 }
 

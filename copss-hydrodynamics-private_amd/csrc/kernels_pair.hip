@@ -1,17 +1,29 @@
 // kernels_pair.hip -- (b) the local regularized-Stokeslet pair sum
-// u_i = sum_{j in nbr(i)} G_reg(x_j - x_i) f_j over the element stencil.
+// u_i = sum_{j in nbr(i)} G_reg(x_j - x_i) f_j, and the mobility and
+// Chebyshev applications built on it.
 //
 // Reference: GGEMStokes::local_solution_bead (ggem_stokes.C:427-503),
 // local_solution_field (:360-424), green_tensor_local_regularized
-// (:304-357), global_self_exclusion (:506-523) and the velocity combination
-// of PMSystemStokes::compute_point_velocity (pm_system_stokes.C:441-457).
+// (:304-357), global_self_exclusion (:506-523), the velocity combination of
+// PMSystemStokes::compute_point_velocity (pm_system_stokes.C:441-457) and the
+// Chebyshev vector updates (brownian_system.C:1037-1041, :1093-1101).
 //
-// One thread per target in cell order.  A wave holds 64 consecutive slots,
-// i.e. neighbouring cells of one x-row, so the lanes walk the same stencil
-// rows (wave-uniform row loads) over overlapping slot ranges (cache reuse of
-// the neighbours' positions and forces).  Displacements are formed in FP64
-// (minimum image) in both precisions; the FP32 form evaluates the tensor in
-// FP32 and sums with Kahan compensation, and the combination is FP64.
+// The targets' neighbour lists are stored sliced by wave (entry k of lane l
+// of slice q at slice_off[q] + 64k + l): every list access of a wave is one
+// coalesced transaction.  Positions do not move between neighbour builds,
+// so the pair tensor of every list entry is evaluated once per build
+// (k_tensors: the transcendental work, compute bound) and kept in the same
+// layout as G = A I + sgn w w^T, w = sqrt(|Bx|) x_ij: 16 bytes per entry in
+// FP32, 32 in FP64.  Every later application (the local velocity, each
+// Chebyshev term, M y) streams the tensors and gathers one operand record
+// per neighbour (k_apply: a block-sparse product, HBM bound).  Blocks are
+// renumbered per XCD so a chip partition keeps a contiguous stretch of cells
+// and their neighbours in its L2.
+//
+// FP64: the reference's formula with libm-class erf/exp.  FP32C:
+// displacement in FP64, tensor in FP32 (erfc form, shared exponentials),
+// partial sums in FP32 merged every 8 neighbours into a Kahan accumulator;
+// the epilogue (combination, recurrence) runs in FP64.
 #include "ggem_device.h"
 
 namespace ggem {
@@ -19,6 +31,7 @@ namespace ggem {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kMergeEvery = 8;
 
 inline unsigned blocks_for(int64_t n, int b = kBlock) { return unsigned((n + b - 1) / b); }
 
@@ -26,163 +39,221 @@ struct PairArgs {
   Grid g;
   GGEMConst k;
   GGEMConst32 k32;
-  const StencilRow* rows;
-  int nrows;
-  const int32_t* cell_start;
-  const int32_t* slot_cell;
+  const double4* x4;
+  const int32_t* cnt;
+  const int64_t* slice_off;
+  const uint32_t* ell;
   const int32_t* perm;
-  const double* xs;
+  const double* fs;  // own forces (FP64 SoA) for the mobility epilogue
   int64_t n;
+  double bsign;
 };
 
-// Sum of G_reg(x_t - x_s) f_t over the stencil of slot s's element, t != s.
-template <int P>
-__device__ __forceinline__ void pair_sum_slot(const PairArgs& a, int64_t s,
-                                              const double* __restrict__ fs,
-                                              const float* __restrict__ fs32, double& ux,
-                                              double& uy, double& uz) {
-  const int64_t n = a.n;
-  int ci, cj, ck;
-  cell_ijk(a.g, a.slot_cell[s], ci, cj, ck);
-  const double x0 = a.xs[s], y0 = a.xs[n + s], z0 = a.xs[2 * n + s];
-  if (P == GGEM_FP64) {
-    double sx = 0, sy = 0, sz = 0;
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        if (t == s) continue;
-        const double dx = min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]);
-        const double dy = min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]);
-        const double dz = min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]);
-        green_acc_f64(a.k, dx, dy, dz, fs[t], fs[n + t], fs[2 * n + t], sx, sy, sz);
-      }
-    });
-    ux = sx;
-    uy = sy;
-    uz = sz;
-  } else {
-    KahanF kx, ky, kz;
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        if (t == s) continue;
-        const float dx = float(min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]));
-        const float dy = float(min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]));
-        const float dz =
-            float(min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]));
-        const float fx = fs32[t], fy = fs32[n + t], fz = fs32[2 * n + t];
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < 1e-12f) {
-          kx.add(a.k32.coeff_2 * fx);
-          ky.add(a.k32.coeff_2 * fy);
-          kz.add(a.k32.coeff_2 * fz);
-          continue;
-        }
-        float A, Bx;
-        green_coef_f32(a.k32, r2, A, Bx);
-        const float B = Bx * (dx * fx + dy * fy + dz * fz);
-        kx.add(A * fx + B * dx);
-        ky.add(A * fy + B * dy);
-        kz.add(A * fz + B * dz);
-      }
-    });
-    ux = kx.value();
-    uy = ky.value();
-    uz = kz.value();
+template <class T>
+struct V4;
+template <>
+struct V4<float> {
+  using type = float4;
+};
+template <>
+struct V4<double> {
+  using type = double4;
+};
+
+// Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
+template <int P, bool EXACT>
+__global__ void __launch_bounds__(kBlock) k_tensors(PairArgs a, float4* __restrict__ c32,
+                                                   double4* __restrict__ c64) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t s = blk * kBlock + threadIdx.x;
+  if (s >= a.n) return;
+  const int cnt = a.cnt[s];
+  const int64_t base = a.slice_off[s >> 6] + (s & 63);
+  const uint32_t* lst = a.ell + base;
+  const double4 xs = a.x4[s];
+  const int last = cnt > 0 ? cnt - 1 : 0;
+  uint32_t e1 = cnt > 0 ? lst[0] : 0u;
+  uint32_t e2 = cnt > 1 ? lst[kSlice] : e1;
+  double4 xa = a.x4[e1 & kSlotMask];
+  for (int k = 0; k < cnt; ++k) {
+    const uint32_t e = e1;
+    const double4 xt = xa;
+    e1 = e2;
+    xa = a.x4[e1 & kSlotMask];
+    e2 = lst[uint32_t(min(k + 2, last)) * uint32_t(kSlice)];
+    double dx, dy, dz;
+    pair_disp<EXACT>(a.g, xs, xt, e >> kSlotBits, dx, dy, dz);
+    const int64_t o = base + int64_t(k) * kSlice;
+    if (P == GGEM_FP64) {
+      double A, Bx;
+      green_coef_f64(a.k, dx * dx + dy * dy + dz * dz, A, Bx);
+      const double sb = sqrt(fabs(Bx));
+      c64[o] = make_double4(A, sb * dx, sb * dy, sb * dz);
+    } else {
+      const float fx = float(dx), fy = float(dy), fz = float(dz);
+      float A, Bx;
+      green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
+      const float sb = __builtin_sqrtf(fabsf(Bx));
+      c32[o] = make_float4(A, sb * fx, sb * fy, sb * fz);
+    }
   }
 }
 
-// Velocity of every bead, written in particle order (interleaved).
-//   EPI_LOCAL:    U_loc (+ coeff_2 f for Lagrangian points, ggem_stokes.C:485-499)
-//   EPI_MOBILITY: compute_point_velocity("disturbed") with U_glob from u_glob
-template <int P>
-__global__ void __launch_bounds__(kBlock) k_pair_beads(PairArgs a, const double* __restrict__ fs,
-                                                      const float* __restrict__ fs32, int epi,
-                                                      const double* __restrict__ u_glob,
-                                                      double* __restrict__ out) {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Output of one target: m = sum over its list of G_ij x_j for NR operand
+// vectors, then the epilogue.
+template <int P, int NR, int EPI>
+__global__ void __launch_bounds__(kBlock) k_apply(PairArgs a, const typename V4<typename std::conditional<P == GGEM_FP64, double, float>::type>::type* __restrict__ coef,
+                                                 const double4* __restrict__ xg64,
+                                                 const float4* __restrict__ xg32,
+                                                 const double* __restrict__ xsoa, SlotVec v,
+                                                 ChebEpi epi, const double* __restrict__ u_glob,
+                                                 double* __restrict__ out) {
+  using T = typename std::conditional<P == GGEM_FP64, double, float>::type;
+  using T4 = typename V4<T>::type;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t s = blk * kBlock + threadIdx.x;
   if (s >= a.n) return;
-  double ux, uy, uz;
-  pair_sum_slot<P>(a, s, fs, fs32, ux, uy, uz);
   const int64_t n = a.n;
-  const double fx = fs[s], fy = fs[n + s], fz = fs[2 * n + s];
-  if (a.k.point_type == GGEM_LAGRANGIAN_POINT) {
-    ux += a.k.coeff_2 * fx;
-    uy += a.k.coeff_2 * fy;
-    uz += a.k.coeff_2 * fz;
-  }
-  const int64_t p = a.perm[s];
-  if (epi == EPI_MOBILITY) {
-    const double gx = u_glob ? u_glob[3 * p] : 0.0;
-    const double gy = u_glob ? u_glob[3 * p + 1] : 0.0;
-    const double gz = u_glob ? u_glob[3 * p + 2] : 0.0;
-    if (a.k.point_type == GGEM_POLYMER_BEAD) {
-      ux = gx + ux - a.k.coeff_1 * fx + fx;
-      uy = gy + uy - a.k.coeff_1 * fy + fy;
-      uz = gz + uz - a.k.coeff_1 * fz + fz;
-    } else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) {
-      ux = gx + ux;
-      uy = gy + uy;
-      uz = gz + uz;
-    } else {
-      ux = uy = uz = 0.0;
+  const int cnt = a.cnt[s];
+  const int64_t base = a.slice_off[s >> 6] + (s & 63);
+  const uint32_t* lst = a.ell + base;
+  const T4* cf = coef + base;
+  const T4* xg;
+  if constexpr (P == GGEM_FP64) xg = xg64;
+  else xg = xg32;
+  const T sg = T(a.bsign);
+  double acc[NR][3];
+  T ps[NR][3];
+  KahanF kh[P == GGEM_FP64 ? 1 : NR][3];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) ps[r][0] = ps[r][1] = ps[r][2] = T(0);
+  // entry one ahead so the operand gather of the next pair is in flight
+  const int last = cnt > 0 ? cnt - 1 : 0;
+  uint32_t e1 = cnt > 0 ? lst[0] : 0u;
+  for (int k = 0; k < cnt; ++k) {
+    const uint32_t t = e1 & kSlotMask;
+    e1 = lst[uint32_t(min(k + 1, last)) * uint32_t(kSlice)];
+    const T4 c = cf[int64_t(k) * kSlice];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const T4 f = xg[r * n + t];
+      const T wf = sg * (c.y * f.x + c.z * f.y + c.w * f.z);
+      ps[r][0] += c.x * f.x + wf * c.y;
+      ps[r][1] += c.x * f.y + wf * c.z;
+      ps[r][2] += c.x * f.z + wf * c.w;
+    }
+    if constexpr (P != GGEM_FP64) {
+      if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            kh[r][q].add(ps[r][q]);
+            ps[r][q] = 0.f;
+          }
+      }
     }
   }
-  out[3 * p] = ux;
-  out[3 * p + 1] = uy;
-  out[3 * p + 2] = uz;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if constexpr (P == GGEM_FP64) {
+        acc[r][q] = ps[r][q];
+      } else {
+        kh[r][q].add(ps[r][q]);
+        acc[r][q] = kh[r][q].value();
+      }
+    }
+
+  if (EPI == EPI_LOCAL || EPI == EPI_MOBILITY) {
+    // forces are the operand: own force from the FP64 store
+    const double fo[3] = {a.fs[s], a.fs[n + s], a.fs[2 * n + s]};
+    double u[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      u[q] = acc[0][q];
+      if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] += a.k.coeff_2 * fo[q];  // self term
+    }
+    const int64_t p = a.perm[s];
+    if (EPI == EPI_MOBILITY) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double ug = u_glob ? u_glob[3 * p + q] : 0.0;
+        if (a.k.point_type == GGEM_POLYMER_BEAD) u[q] = ug + u[q] - a.k.coeff_1 * fo[q] + fo[q];
+        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] = ug + u[q];
+        else u[q] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[3 * p + q] = u[q];
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double nv[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int64_t idx = int64_t(r) * 3 * n + int64_t(q) * n + s;
+        const double xv = xsoa[idx];
+        double m = acc[r][q];
+        if (a.k.point_type == GGEM_POLYMER_BEAD) m = m - a.k.coeff_1 * xv + xv;
+        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) m = m + a.k.coeff_2 * xv;
+        else m = 0.0;
+        if (epi.kind == 0) {
+          nv[q] = m;
+        } else if (epi.kind == 1) {
+          nv[q] = epi.db * xv + epi.da * m;
+        } else {
+          double tv = 2. * epi.db * xv + 2. * epi.da * m;
+          tv += -1.0 * epi.xm1.soa[idx];
+          nv[q] = tv;
+          epi.y[idx] += epi.c * tv;
+        }
+        v.soa[idx] = nv[q];
+      }
+      if (v.g32) v.g32[r * n + s] = make_float4(float(nv[0]), float(nv[1]), float(nv[2]), 0.f);
+      if (v.g64) v.g64[r * n + s] = make_double4(nv[0], nv[1], nv[2], 0.0);
+    }
+  }
 }
 
 // Field points: sum over the element's particle list, own element
-// included, no exclusion (ggem_stokes.C:396-420).
+// included, no exclusion (ggem_stokes.C:396-420), full minimum image.
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_pair_points(PairArgs a, const double* __restrict__ fs,
-                                                       const float* __restrict__ fs32, int64_t m,
+__global__ void __launch_bounds__(kBlock) k_pair_points(PairArgs a, const StencilRow* __restrict__ rows,
+                                                       int nrows, const int32_t* __restrict__ cell_start,
+                                                       const float4* __restrict__ fg32,
+                                                       const double4* __restrict__ fg64, int64_t m,
                                                        const double* __restrict__ pts,
                                                        const int32_t* __restrict__ elem,
                                                        double* __restrict__ u) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= m) return;
-  const int64_t n = a.n;
   int ci, cj, ck;
   cell_ijk(a.g, elem[q], ci, cj, ck);
-  const double x0 = pts[3 * q], y0 = pts[3 * q + 1], z0 = pts[3 * q + 2];
-  double ux, uy, uz;
-  if (P == GGEM_FP64) {
-    double sx = 0, sy = 0, sz = 0;
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        const double dx = min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]);
-        const double dy = min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]);
-        const double dz = min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]);
-        green_acc_f64(a.k, dx, dy, dz, fs[t], fs[n + t], fs[2 * n + t], sx, sy, sz);
-      }
-    });
-    ux = sx;
-    uy = sy;
-    uz = sz;
-  } else {
-    KahanF kx, ky, kz;
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        const double ddx = min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]);
-        const double ddy = min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]);
-        const double ddz = min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]);
-        const float fx = fs32[t], fy = fs32[n + t], fz = fs32[2 * n + t];
-        if (ddx * ddx + ddy * ddy + ddz * ddz < 1e-12) {
-          kx.add(a.k32.coeff_2 * fx);
-          ky.add(a.k32.coeff_2 * fy);
-          kz.add(a.k32.coeff_2 * fz);
-          continue;
-        }
-        const float dx = float(ddx), dy = float(ddy), dz = float(ddz);
-        const float r2 = dx * dx + dy * dy + dz * dz;
+  const double4 xq = make_double4(pts[3 * q], pts[3 * q + 1], pts[3 * q + 2], 0.0);
+  double ux = 0, uy = 0, uz = 0;
+  KahanF kx, ky, kz;
+  for_each_segment(a.g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi, uint32_t) {
+    for (int32_t t = lo; t < hi; ++t) {
+      double dx, dy, dz;
+      pair_disp<true>(a.g, xq, a.x4[t], 0u, dx, dy, dz);
+      if (P == GGEM_FP64) {
+        const double4 f = fg64[t];
+        green_acc_f64(a.k, dx, dy, dz, f.x, f.y, f.z, ux, uy, uz);
+      } else {
+        const float4 f = fg32[t];
+        const float fx = float(dx), fy = float(dy), fz = float(dz);
         float A, Bx;
-        green_coef_f32(a.k32, r2, A, Bx);
-        const float B = Bx * (dx * fx + dy * fy + dz * fz);
-        kx.add(A * fx + B * dx);
-        ky.add(A * fy + B * dy);
-        kz.add(A * fz + B * dz);
+        green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
+        const float B = Bx * (fx * f.x + fy * f.y + fz * f.z);
+        kx.add(A * f.x + B * fx);
+        ky.add(A * f.y + B * fy);
+        kz.add(A * f.z + B * fz);
       }
-    });
+    }
+  });
+  if (P != GGEM_FP64) {
     ux = kx.value();
     uy = ky.value();
     uz = kz.value();
@@ -192,110 +263,14 @@ __global__ void __launch_bounds__(kBlock) k_pair_points(PairArgs a, const double
   u[3 * q + 2] = uz;
 }
 
-// Mobility on slot-ordered SoA vectors for nrhs right-hand sides sharing
-// every tensor evaluation, with the Chebyshev recurrence fused into the
-// epilogue (brownian_system.C:1037-1041, :1093-1101).
-template <int P, int NR>
-__global__ void __launch_bounds__(kBlock) k_apply_slots(PairArgs a, const double* __restrict__ x,
-                                                       double* v, ChebEpi epi, int64_t vstride) {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
-  const int64_t n = a.n;
-  int ci, cj, ck;
-  cell_ijk(a.g, a.slot_cell[s], ci, cj, ck);
-  const double x0 = a.xs[s], y0 = a.xs[n + s], z0 = a.xs[2 * n + s];
-  double acc[NR][3];
-  if (P == GGEM_FP64) {
-#pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        if (t == s) continue;
-        const double dx = min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]);
-        const double dy = min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]);
-        const double dz = min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]);
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        double A, Bx;
-        if (r < 1e-6) {
-          A = a.k.coeff_2;
-          Bx = 0.0;
-        } else {
-          const double r2 = r * r;
-          const double kr = a.k.ksi * r, ar = a.k.alpha * r;
-          const double c1 = (erf(kr) - erf(ar)) / r;
-          const double c2 =
-              a.k.two_ksi_sqrt_pi * exp(-kr * kr) - a.k.two_alpha_sqrt_pi * exp(-ar * ar);
-          A = a.k.eps * (c1 + c2);
-          Bx = a.k.eps * (c1 - c2) / r2;
-        }
-#pragma unroll
-        for (int r_ = 0; r_ < NR; ++r_) {
-          const double* xr = x + r_ * vstride;
-          const double fx = xr[t], fy = xr[n + t], fz = xr[2 * n + t];
-          const double B = Bx * (dx * fx + dy * fy + dz * fz);
-          acc[r_][0] += A * fx + B * dx;
-          acc[r_][1] += A * fy + B * dy;
-          acc[r_][2] += A * fz + B * dz;
-        }
-      }
-    });
-  } else {
-    KahanF kk[NR][3];
-    for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-      for (int32_t t = lo; t < hi; ++t) {
-        if (t == s) continue;
-        const float dx = float(min_image(a.xs[t] - x0, a.g.L[0], a.g.half_L[0], a.g.periodic[0]));
-        const float dy = float(min_image(a.xs[n + t] - y0, a.g.L[1], a.g.half_L[1], a.g.periodic[1]));
-        const float dz =
-            float(min_image(a.xs[2 * n + t] - z0, a.g.L[2], a.g.half_L[2], a.g.periodic[2]));
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        float A, Bx;
-        if (r2 < 1e-12f) {
-          A = a.k32.coeff_2;
-          Bx = 0.f;
-        } else {
-          green_coef_f32(a.k32, r2, A, Bx);
-        }
-#pragma unroll
-        for (int r_ = 0; r_ < NR; ++r_) {
-          const double* xr = x + r_ * vstride;
-          const float fx = float(xr[t]), fy = float(xr[n + t]), fz = float(xr[2 * n + t]);
-          const float B = Bx * (dx * fx + dy * fy + dz * fz);
-          kk[r_][0].add(A * fx + B * dx);
-          kk[r_][1].add(A * fy + B * dy);
-          kk[r_][2].add(A * fz + B * dz);
-        }
-      }
-    });
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) acc[r][c] = kk[r][c].value();
-  }
-  const double c_k = epi.coef ? *epi.coef : epi.c_host;
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t idx = r * vstride + c * n + s;
-      const double xv = x[idx];
-      double m = acc[r][c];
-      if (a.k.point_type == GGEM_POLYMER_BEAD) m = m - a.k.coeff_1 * xv + xv;
-      else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) m = m + a.k.coeff_2 * xv;
-      else m = 0.0;
-      if (epi.kind == 0) {
-        v[idx] = m;
-      } else if (epi.kind == 1) {
-        v[idx] = epi.db * xv + epi.da * m;
-      } else {
-        double t = 2. * epi.db * xv + 2. * epi.da * m;
-        t += -1.0 * epi.xm1[idx];
-        v[idx] = t;
-        epi.y[idx] += c_k * t;
-      }
-    }
-  }
-}
+// erfcx fit (see erfcx_f32): coefficients of Q in x = 2t/t_max - 1,
+// t_max = 1/(1 + 0.45/2), least squares with relative weights on
+// z in [0.45, 1e4].
+constexpr float kErfcxTmax = 0.816326530612f;
+constexpr float kErfcxC[kErfcxDeg + 1] = {
+    4.52274475e-01f,  2.40276895e-01f, 8.51269019e-02f,  1.35228369e-02f,
+    -3.15813629e-03f, -1.64298252e-03f, 2.09066292e-04f, 1.94463725e-04f,
+    -2.89952557e-05f, -1.94542224e-05f, 3.23959705e-06f};
 
 PairArgs make_args(const ggem_ctx* ctx) {
   PairArgs a;
@@ -308,81 +283,116 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.k32.two_ksi_sqrt_pi = float(ctx->gc.two_ksi_sqrt_pi);
   a.k32.coeff_2 = float(ctx->gc.coeff_2);
   a.k32.erf_switch = 0.5f;
-  a.rows = ctx->rows_d;
-  a.nrows = ctx->n_rows;
-  a.cell_start = ctx->cell_start;
-  a.slot_cell = ctx->slot_cell;
+  a.k32.erfcx_scale = 2.0f / kErfcxTmax;
+  for (int i = 0; i <= kErfcxDeg; ++i) a.k32.erfcx_c[i] = kErfcxC[i];
+  a.x4 = ctx->x4;
+  a.cnt = ctx->nbr_cnt;
+  a.slice_off = ctx->slice_off;
+  a.ell = ctx->ell;
   a.perm = ctx->perm;
-  a.xs = ctx->xs;
+  a.fs = ctx->fs;
   a.n = ctx->n;
+  a.bsign = double(ctx->bsign);
   return a;
 }
 
-template <int P, int NR>
-void launch_apply_nr(ggem_ctx* ctx, const PairArgs& a, const double* x, double* v, ChebEpi epi) {
-  hipLaunchKernelGGL((k_apply_slots<P, NR>), dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a, x,
-                     v, epi, int64_t(3) * a.n);
+template <int P, int NR, int EPI>
+void launch_k(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float4* g32, const double* soa,
+              const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
+  if constexpr (P == GGEM_FP64)
+    hipLaunchKernelGGL((k_apply<P, NR, EPI>), dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a,
+                       ctx->coef64, g64, g32, soa, v, epi, u_glob, out);
+  else
+    hipLaunchKernelGGL((k_apply<P, NR, EPI>), dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a,
+                       ctx->coef32, g64, g32, soa, v, epi, u_glob, out);
 }
 
 template <int P>
-void launch_apply_chunk(ggem_ctx* ctx, const PairArgs& a, int nr, const double* x, double* v,
-                        const ChebEpi& epi) {
+void launch_cheb_nr(ggem_ctx* ctx, const PairArgs& a, int nr, const double4* g64, const float4* g32,
+                    const double* soa, const SlotVec& v, const ChebEpi& epi) {
   switch (nr) {
-    case 1: launch_apply_nr<P, 1>(ctx, a, x, v, epi); break;
-    case 2: launch_apply_nr<P, 2>(ctx, a, x, v, epi); break;
-    case 4: launch_apply_nr<P, 4>(ctx, a, x, v, epi); break;
-    case 8: launch_apply_nr<P, 8>(ctx, a, x, v, epi); break;
-    default: launch_apply_nr<P, 16>(ctx, a, x, v, epi); break;
+    case 1: launch_k<P, 1, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
+    case 2: launch_k<P, 2, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
+    case 4: launch_k<P, 4, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
+    default: launch_k<P, 8, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
   }
 }
 
 }  // namespace
 
-hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,
-                             double* out) {
+hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
   if (ctx->n == 0) return hipSuccess;
-  PairArgs a = make_args(ctx);
+  const PairArgs a = make_args(ctx);
+  ProfScope ps(ctx, precision == GGEM_FP64 ? "tensors_fp64" : "tensors_fp32c");
+  const dim3 grid(blocks_for(ctx->n)), block(kBlock);
+  if (precision == GGEM_FP64) {
+    if (ctx->exact_minimg)
+      hipLaunchKernelGGL((k_tensors<GGEM_FP64, true>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+    else
+      hipLaunchKernelGGL((k_tensors<GGEM_FP64, false>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+  } else {
+    if (ctx->exact_minimg)
+      hipLaunchKernelGGL((k_tensors<GGEM_FP32C, true>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+    else
+      hipLaunchKernelGGL((k_tensors<GGEM_FP32C, false>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob, double* out) {
+  if (ctx->n == 0) return hipSuccess;
+  const PairArgs a = make_args(ctx);
+  const SlotVec none{nullptr, nullptr, nullptr};
+  const ChebEpi no_epi{0, 0.0, 0.0, 0.0, none, nullptr};
   ProfScope ps(ctx, precision == GGEM_FP64 ? "pair_beads_fp64" : "pair_beads_fp32c");
-  if (precision == GGEM_FP64)
-    hipLaunchKernelGGL(k_pair_beads<GGEM_FP64>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream,
-                       a, ctx->fs, ctx->fs32, epi, u_glob, out);
-  else
-    hipLaunchKernelGGL(k_pair_beads<GGEM_FP32C>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0,
-                       ctx->stream, a, ctx->fs, ctx->fs32, epi, u_glob, out);
+  if (precision == GGEM_FP64) {
+    if (epi == EPI_LOCAL)
+      launch_k<GGEM_FP64, 1, EPI_LOCAL>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
+    else
+      launch_k<GGEM_FP64, 1, EPI_MOBILITY>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
+  } else {
+    if (epi == EPI_LOCAL)
+      launch_k<GGEM_FP32C, 1, EPI_LOCAL>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
+    else
+      launch_k<GGEM_FP32C, 1, EPI_MOBILITY>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_pair_points(ggem_ctx* ctx, int precision, int64_t m, const double* pts,
                               const int32_t* elem, double* u) {
   if (m == 0) return hipSuccess;
-  PairArgs a = make_args(ctx);
+  const PairArgs a = make_args(ctx);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "pair_points_fp64" : "pair_points_fp32c");
   if (precision == GGEM_FP64)
     hipLaunchKernelGGL(k_pair_points<GGEM_FP64>, dim3(blocks_for(m)), dim3(kBlock), 0, ctx->stream, a,
-                       ctx->fs, ctx->fs32, m, pts, elem, u);
+                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->fg32, ctx->fg64, m, pts, elem, u);
   else
     hipLaunchKernelGGL(k_pair_points<GGEM_FP32C>, dim3(blocks_for(m)), dim3(kBlock), 0, ctx->stream, a,
-                       ctx->fs, ctx->fs32, m, pts, elem, u);
+                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->fg32, ctx->fg64, m, pts, elem, u);
   return hipGetLastError();
 }
 
-hipError_t launch_apply_slots(ggem_ctx* ctx, int precision, int nrhs, const double* x, double* v,
-                              const ChebEpi& epi) {
+hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x, const SlotVec& v,
+                        const ChebEpi& epi) {
   if (ctx->n == 0) return hipSuccess;
-  PairArgs a = make_args(ctx);
+  const PairArgs a = make_args(ctx);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "apply_fp64" : "apply_fp32c");
-  const int64_t stride = 3 * ctx->n;
+  const int64_t n = ctx->n, stride = 3 * n;
   int done = 0;
   while (done < nrhs) {
     const int left = nrhs - done;
-    const int nr = left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
+    const int nr = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
     ChebEpi e = epi;
-    if (e.xm1) e.xm1 += done * stride;
+    if (e.xm1.soa) e.xm1.soa += done * stride;
     if (e.y) e.y += done * stride;
+    SlotVec vv{v.soa + done * stride, v.g32 ? v.g32 + done * n : nullptr, v.g64 ? v.g64 + done * n : nullptr};
+    const double4* g64 = x.g64 ? x.g64 + done * n : nullptr;
+    const float4* g32 = x.g32 ? x.g32 + done * n : nullptr;
     if (precision == GGEM_FP64)
-      launch_apply_chunk<GGEM_FP64>(ctx, a, nr, x + done * stride, v + done * stride, e);
+      launch_cheb_nr<GGEM_FP64>(ctx, a, nr, g64, g32, x.soa + done * stride, vv, e);
     else
-      launch_apply_chunk<GGEM_FP32C>(ctx, a, nr, x + done * stride, v + done * stride, e);
+      launch_cheb_nr<GGEM_FP32C>(ctx, a, nr, g64, g32, x.soa + done * stride, vv, e);
     done += nr;
   }
   return hipGetLastError();

@@ -9,160 +9,203 @@
 //   g(r) = alpha^3/pi^{3/2} e^{-alpha^2 r^2} (2.5 - alpha^2 r^2),
 //   Fe[j*n_u + l] = sum_qp IF[l*nqp + qp] rho_j(qp).
 //
-// Gather form, no atomics: a block owns EPB elements x nqp Gauss points,
-// one thread per (element, Gauss point).  The nqp threads of an element walk
-// the same particle list (broadcast loads); rho stays in LDS for the IF
-// contraction, which runs in FP64 in both precisions.
+// Gather form, no atomics.  The Gauss points of an element form a tensor
+// grid q = c + (g_a, g_b, g_c) h/2 and the minimum image is taken per axis,
+// so with d = x_p - c and u = d - g h/2 per axis,
+//   g = alpha^3/pi^{3/2} E_x[a] E_y[b] E_z[c] (2.5 - S_x[a] - S_y[b] - S_z[c]),
+//   S = alpha^2 u^2, E = e^{-S}.
+// One thread owns the NQ1^2 Gauss points of one x-coordinate a of one
+// element: per particle it forms the FP64 displacement once, the 1 + 2 NQ1
+// factors E (instead of one exponential per Gauss point) and updates
+// NQ1^2 x 3 accumulators.  The NQ1 threads of an element read the same
+// list entries (broadcast loads).  rho goes to LDS for the IF contraction
+// (FP64 arithmetic in both precisions).  FP32C: factors and kernel values
+// in FP32 after the FP64 displacement, partial sums merged every 8
+// particles into Kahan accumulators, rho staged in LDS as FP32.
 #include "ggem_device.h"
 
 namespace ggem {
 
 namespace {
 
+constexpr int kMaxNu = 27;
+constexpr int kMergeEvery = 8;
+constexpr int kElems = 64;  // elements per block
+
+template <class T>
 struct SpreadArgs {
   Grid g;
-  double alpha2, alpha3_pi_23;
-  float alpha2_f, alpha3_pi_23_f;
-  double gp[3];  // 1D Gauss points, ascending
-  const StencilRow* rows;
-  int nrows;
-  const int32_t* cell_start;
-  const double* xs;
-  const double* fs;
-  const float* fs32;
-  int64_t n;
+  T alpha2, alpha3_pi_23;
+  T goff[3][3];  // g_a h_d / 2 per axis d and Gauss index a
+  const int64_t* off;
+  const uint32_t* elist;
+  const double4* x4;
+  const float4* fg32;
+  const double4* fg64;
 };
 
-constexpr int kMaxNu = 27;
+// e^{-a} for a >= 0 in FP32, argument carried with its rounding error.
+__device__ __forceinline__ float exp_neg(float a) {
+  constexpr float L2E = 1.44269502163f;
+  constexpr float L2E_LO = 1.925963033e-08f;
+  constexpr float LN2 = 0.693147180560f;
+  const float y = -a * L2E;
+  const float ye = __builtin_fmaf(-a, L2E, -y) - a * L2E_LO;
+  return __builtin_amdgcn_exp2f(y) * __builtin_fmaf(ye, LN2, 1.0f);
+}
+__device__ __forceinline__ double exp_neg(double a) { return exp(-a); }
 
-template <int P, int NQ1>
-__global__ void __launch_bounds__(256) k_spread(SpreadArgs a, const double* __restrict__ IF, int n_u,
-                                               double* __restrict__ rho, double* __restrict__ Fe) {
+__device__ __forceinline__ float4 load_f(const float4* f32, const double4*, uint32_t t, float) {
+  return f32[t];
+}
+__device__ __forceinline__ double4 load_f(const float4*, const double4* f64, uint32_t t, double) {
+  return f64[t];
+}
+
+template <class T, int NQ1, bool EXACT>
+__global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const double* __restrict__ IF,
+                                                        int n_u, double* __restrict__ rho,
+                                                        double* __restrict__ Fe) {
   constexpr int NQP = NQ1 * NQ1 * NQ1;
-  constexpr int EPB = 256 / NQP;
-  __shared__ double s_rho[EPB * NQP * 3];
+  constexpr int NB = NQ1 * NQ1;  // Gauss points per thread
+  using R = typename std::conditional<sizeof(T) == 4, float, double>::type;  // LDS rho type
+  using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+  __shared__ R s_rho[kElems * NQP * 3];
   __shared__ double s_if[kMaxNu * NQP];
   for (int q = threadIdx.x; q < n_u * NQP; q += blockDim.x) s_if[q] = IF[q];
 
-  const int le = threadIdx.x / NQP;
-  const int qp = threadIdx.x - le * NQP;
-  const int64_t e = int64_t(blockIdx.x) * EPB + le;
-  const bool active = le < EPB && e < a.g.n_elem;
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int le = threadIdx.x / NQ1;
+  const int qa = threadIdx.x - le * NQ1;
+  const int64_t e = blk * kElems + le;
+  const bool active = e < a.g.n_elem;
   if (active) {
     int ci, cj, ck;
     cell_ijk(a.g, e, ci, cj, ck);
-    const int qa = qp % NQ1, qb = (qp / NQ1) % NQ1, qc = qp / (NQ1 * NQ1);
-    const double qx = (a.g.bmin[0] + (double(ci) + 0.5) * a.g.h[0]) + a.gp[qa] * (0.5 * a.g.h[0]);
-    const double qy = (a.g.bmin[1] + (double(cj) + 0.5) * a.g.h[1]) + a.gp[qb] * (0.5 * a.g.h[1]);
-    const double qz = (a.g.bmin[2] + (double(ck) + 0.5) * a.g.h[2]) + a.gp[qc] * (0.5 * a.g.h[2]);
-    const int64_t n = a.n;
-    double r0, r1, r2v;
-    if (P == GGEM_FP64) {
-      double s0 = 0, s1 = 0, s2 = 0;
-      for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-        for (int32_t t = lo; t < hi; ++t) {
-          const double dx = min_image(a.xs[t] - qx, a.g.L[0], a.g.half_L[0], a.g.periodic[0]);
-          const double dy = min_image(a.xs[n + t] - qy, a.g.L[1], a.g.half_L[1], a.g.periodic[1]);
-          const double dz = min_image(a.xs[2 * n + t] - qz, a.g.L[2], a.g.half_L[2], a.g.periodic[2]);
-          const double a2r2 = a.alpha2 * (dx * dx + dy * dy + dz * dz);
-          const double gv = a.alpha3_pi_23 * exp(-a2r2) * (2.5 - a2r2);
-          s0 += gv * a.fs[t];
-          s1 += gv * a.fs[n + t];
-          s2 += gv * a.fs[2 * n + t];
+    const double4 cen = make_double4(a.g.bmin[0] + (double(ci) + 0.5) * a.g.h[0],
+                                     a.g.bmin[1] + (double(cj) + 0.5) * a.g.h[1],
+                                     a.g.bmin[2] + (double(ck) + 0.5) * a.g.h[2], 0.0);
+    const T gx = a.goff[0][qa];
+    const int64_t b0 = a.off[e], b1 = a.off[e + 1];
+    T p[NB][3];
+    KahanF kh[sizeof(T) == 4 ? NB : 1][3];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) p[b][0] = p[b][1] = p[b][2] = T(0);
+    for (int64_t k = b0; k < b1; ++k) {
+      const uint32_t en = a.elist[k];
+      const uint32_t t = en & kSlotMask;
+      double d0, d1, d2;
+      pair_disp<EXACT>(a.g, cen, a.x4[t], en >> kSlotBits, d0, d1, d2);
+      const V4 f = load_f(a.fg32, a.fg64, t, T(0));
+      const T ux = T(d0) - gx;
+      const T sx = a.alpha2 * (ux * ux);
+      const T ex = a.alpha3_pi_23 * exp_neg(sx);
+      T ey[NQ1], sy[NQ1], ez[NQ1], sz[NQ1];
+#pragma unroll
+      for (int g = 0; g < NQ1; ++g) {
+        const T uy = T(d1) - a.goff[1][g];
+        const T uz = T(d2) - a.goff[2][g];
+        sy[g] = a.alpha2 * (uy * uy);
+        sz[g] = a.alpha2 * (uz * uz);
+        ey[g] = ex * exp_neg(sy[g]);
+        ez[g] = exp_neg(sz[g]);
+      }
+#pragma unroll
+      for (int c = 0; c < NQ1; ++c)
+#pragma unroll
+        for (int b = 0; b < NQ1; ++b) {
+          const T gv = (ey[b] * ez[c]) * ((T(2.5) - sx) - (sy[b] + sz[c]));
+          p[c * NQ1 + b][0] += gv * T(f.x);
+          p[c * NQ1 + b][1] += gv * T(f.y);
+          p[c * NQ1 + b][2] += gv * T(f.z);
         }
-      });
-      r0 = s0;
-      r1 = s1;
-      r2v = s2;
-    } else {
-      KahanF k0, k1, k2;
-      for_each_segment(a.g, a.rows, a.nrows, a.cell_start, ci, cj, ck, [&](int32_t lo, int32_t hi) {
-        for (int32_t t = lo; t < hi; ++t) {
-          const float dx = float(min_image(a.xs[t] - qx, a.g.L[0], a.g.half_L[0], a.g.periodic[0]));
-          const float dy = float(min_image(a.xs[n + t] - qy, a.g.L[1], a.g.half_L[1], a.g.periodic[1]));
-          const float dz =
-              float(min_image(a.xs[2 * n + t] - qz, a.g.L[2], a.g.half_L[2], a.g.periodic[2]));
-          const float a2r2 = a.alpha2_f * (dx * dx + dy * dy + dz * dz);
-          const float gv = a.alpha3_pi_23_f * __expf(-a2r2) * (2.5f - a2r2);
-          k0.add(gv * a.fs32[t]);
-          k1.add(gv * a.fs32[n + t]);
-          k2.add(gv * a.fs32[2 * n + t]);
-        }
-      });
-      r0 = k0.value();
-      r1 = k1.value();
-      r2v = k2.value();
+      if (sizeof(T) == 4 && ((k - b0) & (kMergeEvery - 1)) == kMergeEvery - 1) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            kh[sizeof(T) == 4 ? b : 0][j].add(float(p[b][j]));
+            p[b][j] = T(0);
+          }
+      }
     }
-    s_rho[(le * NQP + qp) * 3 + 0] = r0;
-    s_rho[(le * NQP + qp) * 3 + 1] = r1;
-    s_rho[(le * NQP + qp) * 3 + 2] = r2v;
-    if (rho) {
-      rho[(e * NQP + qp) * 3 + 0] = r0;
-      rho[(e * NQP + qp) * 3 + 1] = r1;
-      rho[(e * NQP + qp) * 3 + 2] = r2v;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int qp = (b / NQ1) * NQ1 * NQ1 + (b % NQ1) * NQ1 + qa;  // qp = a + n (b + n c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v;
+        if (sizeof(T) == 4) {
+          kh[sizeof(T) == 4 ? b : 0][j].add(float(p[b][j]));
+          v = kh[sizeof(T) == 4 ? b : 0][j].value();
+        } else {
+          v = double(p[b][j]);
+        }
+        s_rho[(le * NQP + qp) * 3 + j] = R(v);
+        if (rho) rho[(e * NQP + qp) * 3 + j] = v;
+      }
     }
   }
   __syncthreads();
   if (!Fe) return;
   const int per = 3 * n_u;
-  for (int idx = threadIdx.x; idx < EPB * per; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < kElems * per; idx += blockDim.x) {
     const int l2 = idx / per;
     const int rem = idx - l2 * per;
     const int j = rem / n_u;
     const int l = rem - j * n_u;
-    const int64_t e2 = int64_t(blockIdx.x) * EPB + l2;
+    const int64_t e2 = blk * kElems + l2;
     if (e2 >= a.g.n_elem) continue;
-    double acc = 0.0;
+    double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < NQP; ++q) acc += s_if[l * NQP + q] * s_rho[(l2 * NQP + q) * 3 + j];
-    Fe[e2 * per + rem] = acc;
+    for (int q = 0; q < NQP; ++q) s += s_if[l * NQP + q] * double(s_rho[(l2 * NQP + q) * 3 + j]);
+    Fe[e2 * per + rem] = s;
   }
+}
+
+template <class T, int NQ1>
+void launch_sp(ggem_ctx* ctx, const double* gp, const double* IF_d, int n_u, double* rho, double* Fe) {
+  SpreadArgs<T> a;
+  a.g = ctx->grid;
+  a.alpha2 = T(ctx->gc.alpha2);
+  a.alpha3_pi_23 = T(ctx->gc.alpha3_pi_23);
+  for (int d = 0; d < 3; ++d)
+    for (int g = 0; g < 3; ++g) a.goff[d][g] = T(gp[g] * (0.5 * ctx->grid.h[d]));
+  a.off = ctx->elist_off;
+  a.elist = ctx->elist;
+  a.x4 = ctx->x4;
+  a.fg32 = ctx->fg32;
+  a.fg64 = ctx->fg64;
+  const unsigned nb = unsigned((ctx->grid.n_elem + kElems - 1) / kElems);
+  if (ctx->exact_minimg)
+    hipLaunchKernelGGL((k_spread<T, NQ1, true>), dim3(nb), dim3(kElems * NQ1), 0, ctx->stream, a, IF_d, n_u,
+                       rho, Fe);
+  else
+    hipLaunchKernelGGL((k_spread<T, NQ1, false>), dim3(nb), dim3(kElems * NQ1), 0, ctx->stream, a, IF_d, n_u,
+                       rho, Fe);
 }
 
 }  // namespace
 
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
                          double* rho, double* Fe) {
-  SpreadArgs a;
-  a.g = ctx->grid;
-  a.alpha2 = ctx->gc.alpha2;
-  a.alpha3_pi_23 = ctx->gc.alpha3_pi_23;
-  a.alpha2_f = float(ctx->gc.alpha2);
-  a.alpha3_pi_23_f = float(ctx->gc.alpha3_pi_23);
+  double gp[3];
   if (nqp_1d == 2) {
-    a.gp[0] = -std::sqrt(1. / 3.);
-    a.gp[1] = std::sqrt(1. / 3.);
-    a.gp[2] = 0.0;
+    gp[0] = -std::sqrt(1. / 3.);
+    gp[1] = std::sqrt(1. / 3.);
+    gp[2] = 0.0;
   } else {
-    a.gp[0] = -std::sqrt(3. / 5.);
-    a.gp[1] = 0.0;
-    a.gp[2] = std::sqrt(3. / 5.);
+    gp[0] = -std::sqrt(3. / 5.);
+    gp[1] = 0.0;
+    gp[2] = std::sqrt(3. / 5.);
   }
-  a.rows = ctx->rows_d;
-  a.nrows = ctx->n_rows;
-  a.cell_start = ctx->cell_start;
-  a.xs = ctx->xs;
-  a.fs = ctx->fs;
-  a.fs32 = ctx->fs32;
-  a.n = ctx->n;
-  const int nqp = nqp_1d * nqp_1d * nqp_1d;
-  const int epb = 256 / nqp;
-  const unsigned nb = unsigned((ctx->grid.n_elem + epb - 1) / epb);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "spread_fp64" : "spread_fp32c");
   if (nqp_1d == 2) {
-    if (precision == GGEM_FP64)
-      hipLaunchKernelGGL((k_spread<GGEM_FP64, 2>), dim3(nb), dim3(256), 0, ctx->stream, a, IF_d, n_u,
-                         rho, Fe);
-    else
-      hipLaunchKernelGGL((k_spread<GGEM_FP32C, 2>), dim3(nb), dim3(256), 0, ctx->stream, a, IF_d, n_u,
-                         rho, Fe);
+    if (precision == GGEM_FP64) launch_sp<double, 2>(ctx, gp, IF_d, n_u, rho, Fe);
+    else launch_sp<float, 2>(ctx, gp, IF_d, n_u, rho, Fe);
   } else {
-    if (precision == GGEM_FP64)
-      hipLaunchKernelGGL((k_spread<GGEM_FP64, 3>), dim3(nb), dim3(256), 0, ctx->stream, a, IF_d, n_u,
-                         rho, Fe);
-    else
-      hipLaunchKernelGGL((k_spread<GGEM_FP32C, 3>), dim3(nb), dim3(256), 0, ctx->stream, a, IF_d, n_u,
-                         rho, Fe);
+    if (precision == GGEM_FP64) launch_sp<double, 3>(ctx, gp, IF_d, n_u, rho, Fe);
+    else launch_sp<float, 3>(ctx, gp, IF_d, n_u, rho, Fe);
   }
   return hipGetLastError();
 }

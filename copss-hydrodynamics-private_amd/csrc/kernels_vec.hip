@@ -37,6 +37,18 @@ __global__ void k_permute_out(int64_t n, int nrhs, const int32_t* __restrict__ p
     for (int c = 0; c < 3; ++c) out[r * 3 * n + 3 * p + c] = in[r * 3 * n + c * n + s];
 }
 
+// per-slot gather records of slot SoA vectors
+__global__ void k_make_gather(int64_t n, int nrhs, const double* __restrict__ soa, float4* __restrict__ g32,
+                              double4* __restrict__ g64) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  for (int r = 0; r < nrhs; ++r) {
+    const double x = soa[r * 3 * n + s], y = soa[r * 3 * n + n + s], z = soa[r * 3 * n + 2 * n + s];
+    if (g32) g32[r * n + s] = make_float4(float(x), float(y), float(z), 0.f);
+    if (g64) g64[r * n + s] = make_double4(x, y, z, 0.0);
+  }
+}
+
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -99,6 +111,13 @@ hipError_t launch_permute_out(ggem_ctx* ctx, int nrhs, const double* in, double*
   if (ctx->n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_permute_out, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
                      nrhs, ctx->perm, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_make_gather(ggem_ctx* ctx, int nrhs, const double* soa, float4* g32, double4* g64) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_make_gather, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, nrhs,
+                     soa, g32, g64);
   return hipGetLastError();
 }
 

@@ -248,7 +248,10 @@ def test_cheb_fixed_multi_rhs(c1, prec, tol):
     y, d, info = ctx.cheb_brownian(30, lo, hi, dw, mode=G.CHEB_FIXED)
     y1, d1, _ = ctx.cheb_brownian(30, lo, hi, dw[2], mode=G.CHEB_FIXED)
     ctx.set_precision(G.FP64)
-    assert np.allclose(y1[0], y[2], rtol=0, atol=1e-14 * np.abs(y[2]).max())
+    # the same right-hand side alone or inside a batch (a differently
+    # unrolled kernel instance): equal to the precision's own rounding
+    same = 1e-14 if prec == G.FP64 else 1e-7
+    assert np.allclose(y1[0], y[2], rtol=0, atol=same * np.abs(y[2]).max())
     for r in range(5):
         yo, do, io = m.cheb_brownian(30, lo, hi, 0.1, 1, dw[r])
         assert rel_l2(y[r], yo) < tol
