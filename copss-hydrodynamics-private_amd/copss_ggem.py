@@ -43,7 +43,7 @@ EXPORTS = [
     "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
-    "ggem_cheb_brownian", "ggem_lanczos_bounds", "ggem_util_reference_noise",
+    "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_util_reference_noise",
     "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
 ]
@@ -118,6 +118,7 @@ def lib():
             "ggem_mobility_apply": [P, P, P],
             "ggem_set_noise": [P, I32, P],
             "ggem_cheb_brownian": [P, P, I32, D, D, D, I32, I32, P, P, P, P],
+            "ggem_set_graph": [P, I32],
             "ggem_lanczos_bounds": [P, P, I32, U64, P, P],
             "ggem_util_reference_noise": [I64, U64, D, I64, P],
             "ggem_util_rsa_positions": [P, I64, D, U64, P],
@@ -377,6 +378,10 @@ class Context:
 
         self._cb = APPLY_FN(cb)
         return Op(None, self._cb)
+
+    def set_graph(self, enable=True):
+        """Fixed-order Chebyshev steps replay one captured HIP graph."""
+        self._c(lib().ggem_set_graph(self._h, int(enable)))
 
     # profiling
     def profile(self, enable=True):

@@ -258,6 +258,9 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->coef32);
   f(ctx->coef64);
   f(ctx->noise);
+  f(ctx->dpar);
+  if (ctx->cheb_exec) (void)hipGraphExecDestroy(ctx->cheb_exec);
+  ctx->cheb_exec = nullptr;
   if (ctx->scratch) {
     Pools& P = pools(ctx);
     f(P.vec);
@@ -737,6 +740,12 @@ int ggem_mobility_apply(ggem_ctx* ctx, const double* f, double* u) {
   if (rc) return rc;
   if (f && (rc = ggem_set_forces(ctx, f))) return rc;
   return ggem_combine_velocity(ctx, nullptr, u);
+}
+
+int ggem_set_graph(ggem_ctx* ctx, int32_t enable) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  ctx->graph_on = enable != 0;
+  return GGEM_OK;
 }
 
 int ggem_profile_enable(ggem_ctx* ctx, int32_t enable) {

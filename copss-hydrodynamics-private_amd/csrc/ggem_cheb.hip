@@ -113,6 +113,96 @@ struct Applier {
   }
 };
 
+// Operands of the fixed-order recurrence.  par = [da, db, c_0 .. c_N]; with
+// dpar set the kernels read the same values from device memory instead.
+struct FixedStep {
+  int N, nrhs;
+  const double* par;
+  const double* dpar;
+  const double* dw_dev;  // particle order, interleaved
+  SlotVec x0, x1, yv, dwm;
+  bool disp;
+};
+
+// Enqueue the whole fixed-order step: x_0 = dw, x_1 = db x_0 + da M x_0,
+// y = c_0 x_0 + c_1 x_1, x_{k+1} = 2da M x_k + 2db x_k - x_{k-1} with
+// y += c_{k+1} x_{k+1} (brownian_system.C:1047-1100 without the estimators),
+// then dwm = M y when disp is asked for.  Nothing here synchronises, so the
+// sequence can be captured into a graph.
+int enqueue_fixed(ggem_ctx* ctx, Applier& apply, const FixedStep& s) {
+  const SlotVec none{nullptr, nullptr, nullptr};
+  hipError_t e = launch_permute_in(ctx, s.nrhs, s.dw_dev, s.x0.soa);
+  if (e != hipSuccess) return hip_fail(ctx, e, "permute");
+  int rc = refresh(ctx, s.nrhs, s.x0);
+  if (rc) return rc;
+  ChebEpi init{1, s.par[0], s.par[1], 0.0, none, nullptr};
+  init.dpar = s.dpar;
+  if ((rc = apply(s.nrhs, s.x0, s.x1, init))) return rc;
+  e = launch_cheb_init_y(ctx, s.nrhs, s.x0.soa, s.x1.soa, s.yv.soa, s.par[2], s.par[3], s.dpar, s.N + 3,
+                         nullptr);
+  if (e != hipSuccess) return hip_fail(ctx, e, "init y");
+  SlotVec a = s.x0, b = s.x1;
+  for (int i = 1; i < s.N; ++i) {
+    // x_{k+1} into x_{k-1}'s storage
+    ChebEpi st{2, s.par[0], s.par[1], s.par[2 + i + 1], a, s.yv.soa};
+    st.dpar = s.dpar;
+    st.pstride = s.N + 3;
+    st.term = i + 1;
+    if ((rc = apply(s.nrhs, b, a, st))) return rc;
+    std::swap(a, b);
+  }
+  if (s.disp) {
+    if ((rc = refresh(ctx, s.nrhs, s.yv))) return rc;
+    const ChebEpi plain{0, 0.0, 0.0, 0.0, none, nullptr};
+    if ((rc = apply(s.nrhs, s.yv, s.dwm, plain))) return rc;
+  }
+  return GGEM_OK;
+}
+
+// The fixed-order step through a cached graph exec (device operator only).
+int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
+  const size_t npar = size_t(s.N) + 3;
+  int rc = ensure(ctx, &ctx->dpar, &ctx->dpar_cap, npar);
+  if (rc) return rc;
+  s.dpar = ctx->dpar;
+  const Pools& P = pools(ctx);
+  const std::vector<uintptr_t> key = {
+      uintptr_t(s.N), uintptr_t(s.nrhs), uintptr_t(ctx->p.precision), uintptr_t(s.disp), uintptr_t(ctx->n),
+      uintptr_t(s.dw_dev), uintptr_t(s.dpar), uintptr_t(P.vec), uintptr_t(P.g32), uintptr_t(P.g64),
+      uintptr_t(ctx->x4), uintptr_t(ctx->nbr_cnt), uintptr_t(ctx->slice_off), uintptr_t(ctx->ell),
+      uintptr_t(ctx->perm), uintptr_t(ctx->fs), uintptr_t(ctx->coef32), uintptr_t(ctx->coef64)};
+  if (!ctx->cheb_exec || key != ctx->cheb_key) {
+    if (ctx->cheb_exec) {
+      GGEM_HIP(ctx, hipGraphExecDestroy(ctx->cheb_exec));
+      ctx->cheb_exec = nullptr;
+    }
+    // event brackets cannot be recorded into a capture
+    const bool prof = ctx->prof.on;
+    ctx->prof.on = false;
+    GGEM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_fixed(ctx, apply, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+    ctx->prof.on = prof;
+    if (rc || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : hip_fail(ctx, ee, "hipStreamEndCapture");
+    }
+    const hipError_t ei = hipGraphInstantiate(&ctx->cheb_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      ctx->cheb_exec = nullptr;
+      return hip_fail(ctx, ei, "hipGraphInstantiate");
+    }
+    ctx->cheb_key = key;
+  }
+  hipError_t e = launch_set_params(ctx, ctx->dpar, 0, s.par, int(npar));
+  if (e != hipSuccess) return hip_fail(ctx, e, "graph parameters");
+  ProfScope ps(ctx, "cheb_graph");
+  GGEM_HIP(ctx, hipGraphLaunch(ctx->cheb_exec, ctx->stream));
+  return GGEM_OK;
+}
+
 int dot_host(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out) {
   Pools& P = pools(ctx);
   int rc = ensure(ctx, &P.red, &P.red_cap, size_t(nrhs) * 513);
@@ -186,33 +276,32 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     if ((rc = h2d(ctx, P.io, dw, sizeof(double) * blk))) return rc;
     dw_dev = P.io;
   }
-  hipError_t e = launch_permute_in(ctx, nrhs, dw_dev, x0.soa);
-  if (e != hipSuccess) return hip_fail(ctx, e, "permute");
-  if ((rc = refresh(ctx, nrhs, x0))) return rc;
   Applier apply{ctx, op, prec, {}, {}};
   const SlotVec none{nullptr, nullptr, nullptr};
   const ChebEpi plain{0, 0.0, 0.0, 0.0, none, nullptr};
   std::vector<ggem_cheb_info> inf(nrhs);
   for (auto& i : inf) i = ggem_cheb_info{0, 1, 0, 0, 0.0, 0.0};
+  hipError_t e = hipSuccess;
+  bool disp_done = false;
 
   if (mode == GGEM_CHEB_FIXED) {
-    const ChebEpi init{1, da, db, 0.0, none, nullptr};
-    if ((rc = apply(nrhs, x0, x1, init))) return rc;  // x1 = db x0 + da M x0
-    if ((e = launch_fill(ctx, blk, 0.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "fill");
-    if ((e = launch_axpby(ctx, blk, X[0], x0.soa, 1.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-    if ((e = launch_axpby(ctx, blk, X[1], x1.soa, 1.0, yv.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-    for (int i = 1; i < N; ++i) {
-      // x_{k+1} = 2da M x_k + 2db x_k - x_{k-1} into x_{k-1}'s storage; y += c x_{k+1}
-      const ChebEpi st{2, da, db, X[i + 1], x0, yv.soa};
-      if ((rc = apply(nrhs, x1, x0, st))) return rc;
-      std::swap(x0, x1);
-    }
+    std::vector<double> par(N + 3);
+    par[0] = da;
+    par[1] = db;
+    std::copy(X.begin(), X.end(), par.begin() + 2);
+    const FixedStep st{N, nrhs, par.data(), nullptr, dw_dev, x0, x1, yv, dwm, disp != nullptr};
+    rc = (!op && ctx->graph_on) ? run_fixed_graph(ctx, apply, st) : enqueue_fixed(ctx, apply, st);
+    if (rc) return rc;
+    disp_done = true;
     for (auto& i : inf) {
       i.converged = 1;
       i.terms = N;
       i.applies = N + (disp ? 1 : 0);
     }
   } else {
+    e = launch_permute_in(ctx, nrhs, dw_dev, x0.soa);
+    if (e != hipSuccess) return hip_fail(ctx, e, "permute");
+    if ((rc = refresh(ctx, nrhs, x0))) return rc;
     // reference-compatible (brownian_system.C:1030-1155), one right-hand side at a time
     for (int r = 0; r < nrhs; ++r) {
       SlotVec a0 = offset(x0, r, n), a1 = offset(x1, r, n);
@@ -270,8 +359,10 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     if ((rc = d2h(ctx, y, P.io, sizeof(double) * blk))) return rc;
   }
   if (disp) {  // hi_ewald: disp = M y (copss.C:1686)
-    if ((rc = refresh(ctx, nrhs, yv))) return rc;
-    if ((rc = apply(nrhs, yv, dwm, plain))) return rc;
+    if (!disp_done) {
+      if ((rc = refresh(ctx, nrhs, yv))) return rc;
+      if ((rc = apply(nrhs, yv, dwm, plain))) return rc;
+    }
     if ((e = launch_permute_out(ctx, nrhs, dwm.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
     if ((rc = d2h(ctx, disp, P.io, sizeof(double) * blk))) return rc;
   }

@@ -141,6 +141,14 @@ struct ggem_ctx {
   int64_t noise_cap = 0;
   int32_t noise_nrhs = 0;
 
+  // fixed-order Chebyshev step captured as a HIP graph; the recurrence
+  // reads [da, db, c_0..c_N] from dpar, so one capture serves any bounds
+  bool graph_on = false;
+  hipGraphExec_t cheb_exec = nullptr;
+  std::vector<uintptr_t> cheb_key;  // buffers and sizes of the capture
+  double* dpar = nullptr;
+  size_t dpar_cap = 0;
+
   // generic scratch (grown on demand)
   void* scratch = nullptr;
 
@@ -209,6 +217,13 @@ struct ChebEpi {
   double c;            // coefficient of kind 2
   SlotVec xm1;         // x_{k-1} (kind 2); may alias v
   double* y;           // [nrhs][3][n] (kind 2)
+  // Device-side parameters (used when dpar != nullptr, e.g. in a captured
+  // graph): per system [da, db, c_0 .. c_N] with stride pstride; term picks
+  // c; sys gives the system of each slot (nullptr: system 0).
+  const double* dpar = nullptr;
+  int pstride = 0;
+  int term = 0;
+  const int32_t* sys = nullptr;
 };
 // pair tensors of every list entry for the given precision
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
@@ -230,6 +245,11 @@ hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in_interleav
 hipError_t launch_permute_out(ggem_ctx* ctx, int nrhs, const double* in_slots, double* out_interleaved);
 // gather copies of nrhs slot SoA vectors
 hipError_t launch_make_gather(ggem_ctx* ctx, int nrhs, const double* soa, float4* g32, double4* g64);
+// dpar[off .. off+cnt) = vals (kernel arguments, so safe inside a stream or graph)
+hipError_t launch_set_params(ggem_ctx* ctx, double* dpar, int64_t off, const double* vals, int cnt);
+// y = c0 x0 + c1 x1 over nrhs slot vectors (device parameters when dpar != nullptr)
+hipError_t launch_cheb_init_y(ggem_ctx* ctx, int nrhs, const double* x0, const double* x1, double* y,
+                              double c0, double c1, const double* dpar, int pstride, const int32_t* sys);
 // out[r] = sum a_r * b_r over [nrhs][3n] blocks (deterministic)
 hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
                       double* partials);

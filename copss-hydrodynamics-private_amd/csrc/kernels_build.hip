@@ -256,19 +256,39 @@ __global__ void k_elist_count(Grid g, const StencilRow* __restrict__ rows, int n
   cnt[e] = c;
 }
 
-__global__ void k_elist_fill(Grid g, const StencilRow* __restrict__ rows, int nrows,
-                             const int32_t* __restrict__ cell_start, const int64_t* __restrict__ off,
-                             uint32_t* __restrict__ elist) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= g.n_elem) return;
-  int ci, cj, ck;
-  cell_ijk(g, e, ci, cj, ck);
-  uint32_t* out = elist + off[e];
-  int64_t k = 0;
-  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
-    const uint32_t hi = code << kSlotBits;
-    for (int32_t t = a; t < b; ++t) out[k++] = uint32_t(t) | hi;
-  });
+// The lists of a block's consecutive elements form one contiguous output
+// range: they are assembled in LDS and written out with coalesced stores
+// (direct stores when the range exceeds the LDS buffer).
+constexpr int kElistLds = 10240;
+
+__global__ void __launch_bounds__(256) k_elist_fill(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                                                   const int32_t* __restrict__ cell_start,
+                                                   const int64_t* __restrict__ off,
+                                                   uint32_t* __restrict__ elist) {
+  __shared__ uint32_t buf[kElistLds];
+  const int64_t e0 = int64_t(blockIdx.x) * blockDim.x;
+  const int64_t e_end = min(e0 + int64_t(blockDim.x), g.n_elem);
+  const int64_t base = off[e0];
+  const int64_t total = off[e_end] - base;
+  const bool staged = total <= kElistLds;
+  const int64_t e = e0 + threadIdx.x;
+  if (e < g.n_elem) {
+    int ci, cj, ck;
+    cell_ijk(g, e, ci, cj, ck);
+    const int64_t my = off[e] - base;
+    int64_t k = 0;
+    for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
+      const uint32_t hi = code << kSlotBits;
+      for (int32_t t = a; t < b; ++t) {
+        if (staged) buf[my + k] = uint32_t(t) | hi;
+        else elist[base + my + k] = uint32_t(t) | hi;
+        ++k;
+      }
+    });
+  }
+  if (!staged) return;  // block-uniform
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < total; i += blockDim.x) elist[base + i] = buf[i];
 }
 
 // Exclusive scan of int64 values (three phases, as the int32 scan).

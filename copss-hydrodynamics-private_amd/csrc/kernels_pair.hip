@@ -188,6 +188,13 @@ __global__ void __launch_bounds__(kBlock) k_apply(PairArgs a, const typename V4<
 #pragma unroll
     for (int q = 0; q < 3; ++q) out[3 * p + q] = u[q];
   } else {
+    double da = epi.da, db = epi.db, ck = epi.c;
+    if (epi.dpar) {
+      const double* pp = epi.dpar + int64_t(epi.sys ? epi.sys[s] : 0) * epi.pstride;
+      da = pp[0];
+      db = pp[1];
+      ck = pp[2 + epi.term];
+    }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       double nv[3];
@@ -202,12 +209,12 @@ __global__ void __launch_bounds__(kBlock) k_apply(PairArgs a, const typename V4<
         if (epi.kind == 0) {
           nv[q] = m;
         } else if (epi.kind == 1) {
-          nv[q] = epi.db * xv + epi.da * m;
+          nv[q] = db * xv + da * m;
         } else {
-          double tv = 2. * epi.db * xv + 2. * epi.da * m;
+          double tv = 2. * db * xv + 2. * da * m;
           tv += -1.0 * epi.xm1.soa[idx];
           nv[q] = tv;
-          epi.y[idx] += epi.c * tv;
+          epi.y[idx] += ck * tv;
         }
         v.soa[idx] = nv[q];
       }

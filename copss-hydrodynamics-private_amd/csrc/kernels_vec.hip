@@ -49,6 +49,35 @@ __global__ void k_make_gather(int64_t n, int nrhs, const double* __restrict__ so
   }
 }
 
+// y = c_0 x_0 + c_1 x_1 (brownian_system.C:1055-1058), coefficients from
+// the host or per system from device parameters [da, db, c_0 ..]
+__global__ void k_cheb_init_y(int64_t n, int nrhs, const double* __restrict__ x0,
+                              const double* __restrict__ x1, double* __restrict__ y, double c0,
+                              double c1, const double* __restrict__ dpar, int pstride,
+                              const int32_t* __restrict__ sys) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (dpar) {
+    const double* pp = dpar + int64_t(sys ? sys[s] : 0) * pstride;
+    c0 = pp[2];
+    c1 = pp[3];
+  }
+  for (int r = 0; r < nrhs; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t i = int64_t(r) * 3 * n + int64_t(c) * n + s;
+      y[i] = c0 * x0[i] + c1 * x1[i];
+    }
+}
+
+struct ParamChunk {
+  double v[64];
+};
+
+__global__ void k_set_params(double* __restrict__ dst, ParamChunk p, int cnt) {
+  if (int(threadIdx.x) < cnt) dst[threadIdx.x] = p.v[threadIdx.x];
+}
+
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -111,6 +140,24 @@ hipError_t launch_permute_out(ggem_ctx* ctx, int nrhs, const double* in, double*
   if (ctx->n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_permute_out, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
                      nrhs, ctx->perm, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_params(ggem_ctx* ctx, double* dpar, int64_t off, const double* vals, int cnt) {
+  for (int base = 0; base < cnt; base += 64) {
+    ParamChunk p;
+    const int m = cnt - base < 64 ? cnt - base : 64;
+    for (int i = 0; i < 64; ++i) p.v[i] = i < m ? vals[base + i] : 0.0;
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, ctx->stream, dpar + off + base, p, m);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cheb_init_y(ggem_ctx* ctx, int nrhs, const double* x0, const double* x1, double* y,
+                              double c0, double c1, const double* dpar, int pstride, const int32_t* sys) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cheb_init_y, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, nrhs, x0,
+                     x1, y, c0, c1, dpar, pstride, sys);
   return hipGetLastError();
 }
 

@@ -201,6 +201,16 @@ int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, double eig_m
                        double eig_max, double tol, int32_t mode, int32_t nrhs, const double* dw,
                        double* y, double* disp, ggem_cheb_info* info);
 
+/* enable != 0: ggem_cheb_brownian in GGEM_CHEB_FIXED mode with op == NULL
+ * replays one captured HIP graph of its N + 1 mobility applications and
+ * vector updates instead of launching them one by one.  The capture is
+ * redone when the particle set, the neighbour lists, N, nrhs, the precision
+ * or whether disp is asked for changes; the eigenvalue bounds and the
+ * Chebyshev coefficients are device parameters written before each replay.
+ * No counterpart in the reference, whose PETSc loop issues each operation
+ * itself (brownian_system.C:1030-1131); results equal the non-graph path. */
+int ggem_set_graph(ggem_ctx* ctx, int32_t enable);
+
 /* Extreme-eigenvalue estimates of M by Lanczos from a seeded start vector,
  * replacing the SLEPc Krylov-Schur bounds of BrownianSystem::compute_eigenvalues
  * (brownian_system.C:527-657, :803-843).  Raw Ritz values: the caller applies

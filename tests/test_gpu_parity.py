@@ -259,6 +259,53 @@ def test_cheb_fixed_multi_rhs(c1, prec, tol):
         assert info[r].terms == 30 and info[r].applies == 31
 
 
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32C])
+def test_cheb_graph_replays_match_eager(c2, prec):
+    """The captured fixed-order step equals the launch-by-launch one on every
+    replay: other work runs between replays, the bounds change (device
+    parameters), and a particle move forces a new capture."""
+    cfg, x, f, ctx, m = c2
+    ctx.set_precision(prec)
+    n = len(x)
+    dw = G.normal(2 * 3 * n, 41).reshape(2, n, 3)
+    ctx.set_noise(dw)
+    bounds = [(0.05, 3.0), (0.05, 3.0), (0.08, 2.5), (0.05, 3.0)]
+    ctx.set_graph(False)
+    eager = [ctx.cheb_brownian(12, lo, hi, mode=G.CHEB_FIXED) for lo, hi in bounds[:3]]
+    eager_host = ctx.cheb_brownian(12, 0.05, 3.0, dw[1], mode=G.CHEB_FIXED, want_disp=False)
+    ctx.set_graph(True)
+    try:
+        for k, (lo, hi) in enumerate(bounds):
+            y, d, info = ctx.cheb_brownian(12, lo, hi, mode=G.CHEB_FIXED)
+            ye, de, _ = eager[k if k < 3 else 0]
+            assert np.array_equal(y, ye) and np.array_equal(d, de), f"replay {k}"
+            assert info[0].terms == 12 and info[1].applies == 13
+            u = ctx.local_velocity_beads()  # eager work between replays
+            assert np.isfinite(u).all()
+        # host noise and no disp: a second capture
+        yh, dh, _ = ctx.cheb_brownian(12, 0.05, 3.0, dw[1], mode=G.CHEB_FIXED, want_disp=False)
+        assert dh is None and np.array_equal(yh, eager_host[0])
+        y, d, _ = ctx.cheb_brownian(12, 0.05, 3.0, mode=G.CHEB_FIXED)
+        assert np.array_equal(y, eager[0][0]) and np.array_equal(d, eager[0][1])
+        # moved particles: new lists, new capture, against the oracle
+        rng = np.random.default_rng(5)
+        x2 = x + rng.normal(scale=0.05, size=x.shape)
+        ctx.reinit(x2, f)
+        y2, d2, _ = ctx.cheb_brownian(12, 0.05, 3.0, mode=G.CHEB_FIXED)
+        ctx.set_graph(False)
+        y2e, d2e, _ = ctx.cheb_brownian(12, 0.05, 3.0, mode=G.CHEB_FIXED)
+        assert np.array_equal(y2, y2e) and np.array_equal(d2, d2e)
+        m2 = O.Mesh(ora_cfg(cfg))
+        m2.reinit(x2, f)
+        yo, do, _ = m2.cheb_brownian(12, 0.05, 3.0, 0.1, 1, dw[0])
+        tol = TOL64 if prec == G.FP64 else TOL32
+        assert rel_l2(y2[0], yo) < tol and rel_l2(d2[0], do) < tol
+    finally:
+        ctx.set_graph(False)
+        ctx.set_precision(G.FP64)
+        ctx.reinit(x, f)
+
+
 def test_cheb_host_operator(c1):
     """A host-side operator (the FEM-inclusive path's shape) drives the same recurrence."""
     cfg, x, f, ctx, m = c1
