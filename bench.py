@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--cheb-order", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=256)
     return ap.parse_args()
 
 
@@ -68,6 +70,52 @@ def run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, steps, profile):
     if profile:
         ctx.profile(False)
     return (t1 - t0) * 1e3 / max(steps, 1)
+
+
+def bench_batch(G, W, args):
+    """Batched workload: 256 independent periodic suspensions (N_k uniform in
+    [500, 8000], 5% volume fraction) with 16 noise vectors each; one step =
+    neighbour build + local pair sum + fixed-order-30 Chebyshev step of all
+    systems (FP32-compensated pair sums, FP64 recurrence, per-system Lanczos
+    bounds), the recurrence replayed from a captured graph."""
+    nrhs, order = 16, 30
+    cfgs, xs, fs, dw = W.make_batch(nsys=args.batch_systems, nrhs=nrhs)
+    for c in cfgs:
+        c["precision"] = G.FP32C
+    b = G.BatchContext(cfgs)
+    b.set_particles(xs, fs)
+    b.build_neighbors()
+    pairs = b.neighbor_pairs()
+    lmin, lmax = b.lanczos_bounds(iters=30, seed=4005)
+    lo, hi = lmin / 1.05, lmax * 1.05
+    b.set_noise(dw)
+    b.set_graph(True)
+
+    def steps(k):
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            b.build_neighbors()
+            b.local_velocity_beads(out=False)
+            b.cheb_brownian(order, lo, hi, None, want_y=False, want_disp=False)
+        b.sync()
+        return (time.perf_counter() - t0) * 1e3 / max(k, 1)
+
+    steps(max(args.warmup, 1))
+    ms = steps(args.steps)
+    b.profile_reset()
+    b.profile(True)
+    b.set_graph(False)
+    ms_eager = steps(args.steps)
+    a_ms, a_n = b.profile_get("apply_fp32c")
+    b.profile(False)
+    b.close()
+    evals = 1 + order * nrhs  # pair interactions per pair and step
+    return {"systems": len(cfgs), "beads": int(sum(len(x) for x in xs)), "noise_vectors": nrhs,
+            "cheb_order": order, "neighbor_pairs": pairs, "ms_per_step": ms,
+            "pair_interactions_per_s": evals * pairs / (ms * 1e-3),
+            "brownian_steps_per_s": 1e3 / ms, "eager_ms_per_step": ms_eager,
+            "apply_fp32c_avg_ms": a_ms / max(a_n, 1), "apply_launches_per_step": a_n / max(args.steps, 1)}
 
 
 def main():
@@ -181,6 +229,8 @@ def main():
         out["fp32c_rel_l2_vs_fp64"] = {"u_local": rel(r32["u"], r64["u"]), "Fe": rel(r32["Fe"], r64["Fe"]),
                                        "brownian_disp": rel(r32["disp"], r64["disp"])}
     ctx.close()
+    if not args.no_batch:
+        out["batch256"] = bench_batch(G, W, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -43,7 +43,8 @@ EXPORTS = [
     "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
-    "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_util_reference_noise",
+    "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
+    "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds", "ggem_util_reference_noise",
     "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
 ]
@@ -119,6 +120,10 @@ def lib():
             "ggem_set_noise": [P, I32, P],
             "ggem_cheb_brownian": [P, P, I32, D, D, D, I32, I32, P, P, P, P],
             "ggem_set_graph": [P, I32],
+            "ggem_create_batch": [P, I32, C.POINTER(C.c_void_p)],
+            "ggem_batch_set_particles": [P, P, P, P],
+            "ggem_batch_cheb_brownian": [P, I32, P, P, I32, P, P, P],
+            "ggem_batch_lanczos_bounds": [P, I32, U64, P, P],
             "ggem_lanczos_bounds": [P, P, I32, U64, P, P],
             "ggem_util_reference_noise": [I64, U64, D, I64, P],
             "ggem_util_rsa_positions": [P, I64, D, U64, P],
@@ -202,16 +207,25 @@ def hex20_if_table(h, nqp_1d):
 
 
 class Context:
-    """One GGEM context on one HIP device (PointMesh + GGEMStokes + BrownianSystem state)."""
+    """One GGEM context on one HIP device (PointMesh + GGEMStokes + BrownianSystem state).
+
+    cfg: one workload dict, or a list of them for a batch of independent
+    suspensions in one context (see BatchContext)."""
 
     def __init__(self, cfg):
-        self.cfg = dict(cfg)
-        self._params = make_params(cfg)
+        cfgs = list(cfg) if isinstance(cfg, (list, tuple)) else [cfg]
+        self.cfg = dict(cfgs[0]) if cfgs else {}
+        self.cfgs = [dict(c) for c in cfgs]
+        self.nsys = len(cfgs)
+        self._params = (Params * max(self.nsys, 1))(*[make_params(c) for c in cfgs])
         h = C.c_void_p()
-        _check(lib().ggem_create(C.byref(self._params), C.byref(h)))
+        if isinstance(cfg, (list, tuple)):
+            _check(lib().ggem_create_batch(self._params, self.nsys, C.byref(h)))
+        else:
+            _check(lib().ggem_create(self._params, C.byref(h)))
         self._h = h
         self.n = 0
-        self.n_elem = int(np.prod(cfg["n_elem"]))
+        self.n_elem = int(sum(np.prod(c["n_elem"]) for c in cfgs))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -394,3 +408,46 @@ class Context:
         ms, cnt = C.c_double(), C.c_int64()
         self._c(lib().ggem_profile_get(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+class BatchContext(Context):
+    """Many independent suspensions in one context (ggem_create_batch)."""
+
+    def __init__(self, cfgs):
+        super().__init__(list(cfgs))
+        self.counts = np.zeros(self.nsys, dtype=np.int64)
+
+    def set_particles(self, xyz_list, f_list=None):
+        """xyz_list[k]: (n_k, 3) positions of system k (f_list alike, may be None)."""
+        self.counts = np.array([len(x) for x in xyz_list], dtype=np.int64)
+        x = _f64(np.concatenate([np.asarray(v).reshape(-1, 3) for v in xyz_list]), (-1, 3))
+        self.n = x.shape[0]
+        ff = None
+        if f_list is not None:
+            ff = _f64(np.concatenate([np.asarray(v).reshape(-1, 3) for v in f_list]), (-1, 3))
+        self._c(lib().ggem_batch_set_particles(self._h, _p(self.counts), _p(x), _p(ff)))
+
+    def split(self, a):
+        """Per-system views of a (..., n, 3) array."""
+        off = np.concatenate([[0], np.cumsum(self.counts)])
+        return [a[..., off[k]:off[k + 1], :] for k in range(self.nsys)]
+
+    def cheb_brownian(self, N, eig_min, eig_max, dw=None, want_y=True, want_disp=True):
+        if dw is None:
+            nrhs = self._noise_nrhs
+        else:
+            dw = _f64(dw)
+            nrhs = dw.size // (3 * self.n)
+        lo = _f64(np.broadcast_to(np.asarray(eig_min, dtype=np.float64), (self.nsys,)).copy())
+        hi = _f64(np.broadcast_to(np.asarray(eig_max, dtype=np.float64), (self.nsys,)).copy())
+        shape = (nrhs, self.n, 3)
+        y = np.zeros(shape) if want_y else None
+        disp = np.zeros(shape) if want_disp else None
+        self._c(lib().ggem_batch_cheb_brownian(self._h, N, _p(lo), _p(hi), nrhs, _p(dw), _p(y), _p(disp)))
+        return y, disp
+
+    def lanczos_bounds(self, iters=30, seed=7):
+        lo = np.zeros(self.nsys)
+        hi = np.zeros(self.nsys)
+        self._c(lib().ggem_batch_lanczos_bounds(self._h, iters, seed, _p(lo), _p(hi)))
+        return lo, hi

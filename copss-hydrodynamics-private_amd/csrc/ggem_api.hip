@@ -259,6 +259,14 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->coef64);
   f(ctx->noise);
   f(ctx->dpar);
+  f(ctx->sys_d);
+  f(ctx->part_off_d);
+  f(ctx->sys_part);
+  f(ctx->sys_slot);
+  if (ctx->par_h) (void)hipHostFree(ctx->par_h);
+  ctx->par_h = nullptr;
+  if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
+  ctx->par_ev = nullptr;
   if (ctx->cheb_exec) (void)hipGraphExecDestroy(ctx->cheb_exec);
   ctx->cheb_exec = nullptr;
   if (ctx->scratch) {
@@ -318,26 +326,61 @@ int ensure_elist(ggem_ctx* ctx) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-const char* ggem_last_error(const ggem_ctx* ctx) {
-  if (ctx) return ctx->err.c_str();
-  return g_err.c_str();
-}
+int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double* f);
 
-int ggem_create(const ggem_params* params, ggem_ctx** out) {
+// A context over nsys suspensions (nsys == 1: the plain context).  All
+// systems share the physical constants and the precision; grids, boxes
+// and stencils are per system.
+int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
   if (!out) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null output pointer");
   *out = nullptr;
   if (!params) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null params");
-  ggem_consts c;
-  std::string why;
-  int rc = derive(params, &c, &why);
-  if (rc) return fail(nullptr, rc, why);
+  std::vector<ggem_consts> cs(nsys);
+  std::vector<SysGrid> sg(nsys);
   std::vector<StencilRow> rows;
-  int32_t ncells = 0;
-  int reach[3];
-  rc = build_stencil(params, &c, &rows, &ncells, reach, &why);
-  if (rc) return fail(nullptr, rc, why);
+  std::string why;
+  int exact = 0;
+  int64_t ne_total = 0;
+  int32_t ncells0 = 0;
+  for (int32_t k = 0; k < nsys; ++k) {
+    const ggem_params* pk = params + k;
+    int rc = derive(pk, &cs[k], &why);
+    if (rc) return fail(nullptr, rc, nsys > 1 ? "system " + std::to_string(k) + ": " + why : why);
+    if (k > 0 && (pk->mu != params->mu || pk->alpha != params->alpha || cs[k].ksi != cs[0].ksi ||
+                  pk->point_type != params->point_type || pk->precision != params->precision ||
+                  pk->device != params->device || pk->compat_flags != params->compat_flags))
+      return fail(nullptr, GGEM_ERR_INVALID_ARG,
+                  "system " + std::to_string(k) + ": a batch shares mu, alpha, xi, point type, precision, "
+                  "flags and device");
+    std::vector<StencilRow> rk;
+    int32_t ncells = 0;
+    int reach[3];
+    rc = build_stencil(pk, &cs[k], &rk, &ncells, reach, &why);
+    if (rc) return fail(nullptr, rc, nsys > 1 ? "system " + std::to_string(k) + ": " + why : why);
+    if (k == 0) ncells0 = ncells;
+    Grid& g = sg[k].g;
+    for (int d = 0; d < 3; ++d) {
+      g.bmin[d] = pk->box_min[d];
+      g.L[d] = cs[k].L[d];
+      g.h[d] = cs[k].h[d];
+      g.half_L[d] = cs[k].L[d] / 2.0;
+      g.n[d] = pk->n_elem[d];
+      g.periodic[d] = pk->periodic[d] ? 1 : 0;
+      // a neighbour entry's single shift equals the minimum image while every
+      // pair stays inside half a box: (reach + 1) cells <= n/2
+      if (g.periodic[d] && 2 * (reach[d] + 1) > pk->n_elem[d]) exact = 1;
+    }
+    g.n_elem = int64_t(pk->n_elem[0]) * pk->n_elem[1] * pk->n_elem[2];
+    sg[k].elem_off = ne_total;
+    sg[k].row_off = int32_t(rows.size());
+    sg[k].nrows = int32_t(rk.size());
+    ne_total += g.n_elem;
+    rows.insert(rows.end(), rk.begin(), rk.end());
+  }
+  if (ne_total >= INT32_MAX) return fail(nullptr, GGEM_ERR_CAPACITY, "element count exceeds int32");
+  const ggem_consts& c = cs[0];
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(nullptr, GGEM_ERR_NO_DEVICE, "no HIP device visible");
@@ -347,22 +390,17 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
   ggem_ctx* ctx = new ggem_ctx();
   ctx->p = *params;
   ctx->c = c;
-  ctx->c.stencil_rows = int32_t(rows.size());
-  ctx->c.stencil_cells = ncells;
+  ctx->c.stencil_rows = sg[0].nrows;
+  ctx->c.stencil_cells = ncells0;
   ctx->device = params->device;
   ctx->scratch = new Pools();
-  for (int d = 0; d < 3; ++d) {
-    ctx->grid.bmin[d] = params->box_min[d];
-    ctx->grid.L[d] = c.L[d];
-    ctx->grid.h[d] = c.h[d];
-    ctx->grid.half_L[d] = c.L[d] / 2.0;
-    ctx->grid.n[d] = params->n_elem[d];
-    ctx->grid.periodic[d] = params->periodic[d] ? 1 : 0;
-    // a neighbour entry's single shift equals the minimum image while every
-    // pair stays inside half a box: (reach + 1) cells <= n/2
-    if (ctx->grid.periodic[d] && 2 * (reach[d] + 1) > params->n_elem[d]) ctx->exact_minimg = 1;
-  }
-  ctx->grid.n_elem = int64_t(params->n_elem[0]) * params->n_elem[1] * params->n_elem[2];
+  ctx->exact_minimg = exact;
+  ctx->nsys = nsys;
+  ctx->sys_h = sg;
+  ctx->part_off.assign(nsys + 1, 0);
+  // the whole store is one element range; per-system views come from sys_h
+  ctx->grid = sg[0].g;
+  ctx->grid.n_elem = ne_total;
   ctx->gc.eps = c.one_eight_pi_mu;
   ctx->gc.alpha = c.alpha;
   ctx->gc.alpha2 = c.alpha2;
@@ -399,6 +437,13 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
       (e = hipMalloc(&ctx->pairs_d, sizeof(int64_t))) != hipSuccess ||
       (e = hipMalloc(&ctx->elist_off, sizeof(int64_t) * (ne + 1))) != hipSuccess)
     return bail(e, "hipMalloc");
+  if (nsys > 1) {
+    if ((e = hipMalloc(&ctx->sys_d, sizeof(SysGrid) * nsys)) != hipSuccess ||
+        (e = hipMalloc(&ctx->part_off_d, sizeof(int64_t) * (nsys + 1))) != hipSuccess)
+      return bail(e, "hipMalloc");
+    if ((e = hipMemcpy(ctx->sys_d, sg.data(), sizeof(SysGrid) * nsys, hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(e, "hipMemcpy");
+  }
   if (!rows.empty()) {
     e = hipMemcpy(ctx->rows_d, rows.data(), sizeof(StencilRow) * rows.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(e, "hipMemcpy");
@@ -406,6 +451,25 @@ int ggem_create(const ggem_params* params, ggem_ctx** out) {
   if (grow_scan64(ctx, ne + 1) != GGEM_OK) return bail(hipErrorOutOfMemory, "hipMalloc");
   *out = ctx;
   return GGEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ggem_last_error(const ggem_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_err.c_str();
+}
+
+int ggem_create(const ggem_params* params, ggem_ctx** out) { return create_impl(params, 1, out); }
+
+int ggem_create_batch(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
+  if (nsys < 1) {
+    if (out) *out = nullptr;
+    return fail(nullptr, GGEM_ERR_INVALID_ARG, "nsys must be positive");
+  }
+  return create_impl(params, nsys, out);
 }
 
 int ggem_destroy(ggem_ctx* ctx) {
@@ -439,6 +503,47 @@ int ggem_sync(ggem_ctx* ctx) {
 
 int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double* f) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (ctx->nsys > 1) return fail(ctx, GGEM_ERR_INVALID_ARG, "batched context: use ggem_batch_set_particles");
+  return set_particles_impl(ctx, n, xyz, f);
+}
+
+int ggem_batch_set_particles(ggem_ctx* ctx, const int64_t* counts, const double* xyz, const double* f) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (!counts) return fail(ctx, GGEM_ERR_INVALID_ARG, "null counts");
+  std::vector<int64_t> off(ctx->nsys + 1, 0);
+  for (int32_t k = 0; k < ctx->nsys; ++k) {
+    if (counts[k] < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "negative particle count");
+    off[k + 1] = off[k] + counts[k];
+  }
+  const int64_t n = off[ctx->nsys];
+  // element ids of the previous build belong to the previous partition
+  const bool same = off == ctx->part_off;
+  int rc = set_particles_impl(ctx, n, xyz, f);
+  if (rc) return rc;
+  if (!same) ctx->elem_valid = false;
+  ctx->part_off = off;
+  if (ctx->nsys == 1) return GGEM_OK;
+  if (n > ctx->sys_cap) {
+    if (ctx->sys_part) (void)hipFree(ctx->sys_part);
+    if (ctx->sys_slot) (void)hipFree(ctx->sys_slot);
+    ctx->sys_part = ctx->sys_slot = nullptr;
+    ctx->sys_cap = 0;
+    GGEM_HIP(ctx, hipMalloc(&ctx->sys_part, sizeof(int32_t) * n));
+    GGEM_HIP(ctx, hipMalloc(&ctx->sys_slot, sizeof(int32_t) * n));
+    ctx->sys_cap = n;
+  }
+  GGEM_HIP(ctx, hipMemcpyAsync(ctx->part_off_d, ctx->part_off.data(), sizeof(int64_t) * (ctx->nsys + 1),
+                               hipMemcpyHostToDevice, ctx->stream));
+  hipError_t e = launch_part_sys(ctx);
+  if (e != hipSuccess) return hip_fail(ctx, e, "part_sys");
+  return sync(ctx);
+}
+
+}  // extern "C"
+
+namespace {
+
+int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double* f) {
   if (n < 0 || (n > 0 && !xyz)) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid particle arrays");
   if (n >= kMaxParticles) return fail(ctx, GGEM_ERR_CAPACITY, "particle count exceeds 2^26");
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
@@ -510,6 +615,10 @@ int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double
   }
   return sync(ctx);
 }
+
+}  // namespace
+
+extern "C" {
 
 int ggem_set_forces(ggem_ctx* ctx, const double* f) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
@@ -583,6 +692,7 @@ int ggem_neighbor_pairs(ggem_ctx* ctx, int64_t* npairs) {
 int ggem_get_neighbor_csr(ggem_ctx* ctx, int64_t* offsets, int32_t* ids) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_get_neighbor_csr"))) return rc;
   if (!offsets) return fail(ctx, GGEM_ERR_INVALID_ARG, "null offsets");
   Pools& P = pools(ctx);
   const int64_t n = ctx->n;
@@ -605,6 +715,7 @@ int ggem_get_neighbor_csr(ggem_ctx* ctx, int64_t* offsets, int32_t* ids) {
 int ggem_get_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets, int32_t* ids) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_get_elem_point_csr"))) return rc;
   if (!offsets || e0 < 0 || ne < 0 || e0 + ne > ctx->grid.n_elem)
     return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid element range");
   Pools& P = pools(ctx);
@@ -668,6 +779,7 @@ int ggem_local_velocity_points(ggem_ctx* ctx, int64_t m, const double* pts, cons
                                double* u) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_local_velocity_points"))) return rc;
   if (m < 0 || (m > 0 && (!pts || !u))) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid point arrays");
   if (m == 0) return GGEM_OK;
   if (elem) {
@@ -701,6 +813,7 @@ int ggem_local_velocity_points(ggem_ctx* ctx, int64_t m, const double* pts, cons
 int ggem_spread(ggem_ctx* ctx, int32_t nqp_1d, int32_t n_u, const double* IF, double* rho, double* Fe) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_spread"))) return rc;
   if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
   if (n_u < 1 || n_u > 27 || !IF) return fail(ctx, GGEM_ERR_INVALID_ARG, "n_u must be in [1, 27] with an IF table");
   if ((rc = ensure_elist(ctx))) return rc;

@@ -122,6 +122,7 @@ struct FixedStep {
   const double* dw_dev;  // particle order, interleaved
   SlotVec x0, x1, yv, dwm;
   bool disp;
+  const int32_t* sys = nullptr;  // batched: system of each slot, dpar holds N + 3 values per system
 };
 
 // Enqueue the whole fixed-order step: x_0 = dw, x_1 = db x_0 + da M x_0,
@@ -137,9 +138,11 @@ int enqueue_fixed(ggem_ctx* ctx, Applier& apply, const FixedStep& s) {
   if (rc) return rc;
   ChebEpi init{1, s.par[0], s.par[1], 0.0, none, nullptr};
   init.dpar = s.dpar;
+  init.pstride = s.N + 3;
+  init.sys = s.sys;
   if ((rc = apply(s.nrhs, s.x0, s.x1, init))) return rc;
   e = launch_cheb_init_y(ctx, s.nrhs, s.x0.soa, s.x1.soa, s.yv.soa, s.par[2], s.par[3], s.dpar, s.N + 3,
-                         nullptr);
+                         s.sys);
   if (e != hipSuccess) return hip_fail(ctx, e, "init y");
   SlotVec a = s.x0, b = s.x1;
   for (int i = 1; i < s.N; ++i) {
@@ -148,6 +151,7 @@ int enqueue_fixed(ggem_ctx* ctx, Applier& apply, const FixedStep& s) {
     st.dpar = s.dpar;
     st.pstride = s.N + 3;
     st.term = i + 1;
+    st.sys = s.sys;
     if ((rc = apply(s.nrhs, b, a, st))) return rc;
     std::swap(a, b);
   }
@@ -159,18 +163,43 @@ int enqueue_fixed(ggem_ctx* ctx, Applier& apply, const FixedStep& s) {
   return GGEM_OK;
 }
 
-// The fixed-order step through a cached graph exec (device operator only).
-int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
-  const size_t npar = size_t(s.N) + 3;
-  int rc = ensure(ctx, &ctx->dpar, &ctx->dpar_cap, npar);
+// Write the recurrence parameters of every system into ctx->dpar, in
+// stream order ahead of the kernels that read them.
+int upload_params(ggem_ctx* ctx, const double* vals, size_t cnt) {
+  int rc = ensure(ctx, &ctx->dpar, &ctx->dpar_cap, cnt);
   if (rc) return rc;
+  if (cnt <= 256) {  // small sets ride in kernel arguments
+    hipError_t e = launch_set_params(ctx, ctx->dpar, 0, vals, int(cnt));
+    return e == hipSuccess ? GGEM_OK : hip_fail(ctx, e, "recurrence parameters");
+  }
+  // pinned staging, reused once the previous copy out of it has completed
+  if (!ctx->par_ev) GGEM_HIP(ctx, hipEventCreateWithFlags(&ctx->par_ev, hipEventDisableTiming));
+  else GGEM_HIP(ctx, hipEventSynchronize(ctx->par_ev));
+  if (ctx->par_cap < cnt) {
+    if (ctx->par_h) GGEM_HIP(ctx, hipHostFree(ctx->par_h));
+    ctx->par_h = nullptr;
+    ctx->par_cap = 0;
+    GGEM_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->par_h), sizeof(double) * cnt));
+    ctx->par_cap = cnt;
+  }
+  std::copy(vals, vals + cnt, ctx->par_h);
+  GGEM_HIP(ctx, hipMemcpyAsync(ctx->dpar, ctx->par_h, sizeof(double) * cnt, hipMemcpyHostToDevice, ctx->stream));
+  GGEM_HIP(ctx, hipEventRecord(ctx->par_ev, ctx->stream));
+  return GGEM_OK;
+}
+
+// The fixed-order step through a cached graph exec (device operator only);
+// the parameters are already in ctx->dpar.
+int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
+  int rc = GGEM_OK;
   s.dpar = ctx->dpar;
   const Pools& P = pools(ctx);
   const std::vector<uintptr_t> key = {
       uintptr_t(s.N), uintptr_t(s.nrhs), uintptr_t(ctx->p.precision), uintptr_t(s.disp), uintptr_t(ctx->n),
       uintptr_t(s.dw_dev), uintptr_t(s.dpar), uintptr_t(P.vec), uintptr_t(P.g32), uintptr_t(P.g64),
       uintptr_t(ctx->x4), uintptr_t(ctx->nbr_cnt), uintptr_t(ctx->slice_off), uintptr_t(ctx->ell),
-      uintptr_t(ctx->perm), uintptr_t(ctx->fs), uintptr_t(ctx->coef32), uintptr_t(ctx->coef64)};
+      uintptr_t(ctx->perm), uintptr_t(ctx->fs), uintptr_t(ctx->coef32), uintptr_t(ctx->coef64),
+      uintptr_t(s.sys), uintptr_t(ctx->nsys)};
   if (!ctx->cheb_exec || key != ctx->cheb_key) {
     if (ctx->cheb_exec) {
       GGEM_HIP(ctx, hipGraphExecDestroy(ctx->cheb_exec));
@@ -196,8 +225,6 @@ int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
     }
     ctx->cheb_key = key;
   }
-  hipError_t e = launch_set_params(ctx, ctx->dpar, 0, s.par, int(npar));
-  if (e != hipSuccess) return hip_fail(ctx, e, "graph parameters");
   ProfScope ps(ctx, "cheb_graph");
   GGEM_HIP(ctx, hipGraphLaunch(ctx->cheb_exec, ctx->stream));
   return GGEM_OK;
@@ -249,6 +276,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
                                   const double* dw, double* y, double* disp, ggem_cheb_info* info) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_cheb_brownian (use ggem_batch_cheb_brownian)"))) return rc;
   if (N < 1 || nrhs < 1 || !(eig_max > eig_min) || (mode != GGEM_CHEB_REFERENCE && mode != GGEM_CHEB_FIXED))
     return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Chebyshev arguments");
   if (!dw && (ctx->noise_nrhs != nrhs || !ctx->noise))
@@ -290,7 +318,12 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     par[1] = db;
     std::copy(X.begin(), X.end(), par.begin() + 2);
     const FixedStep st{N, nrhs, par.data(), nullptr, dw_dev, x0, x1, yv, dwm, disp != nullptr};
-    rc = (!op && ctx->graph_on) ? run_fixed_graph(ctx, apply, st) : enqueue_fixed(ctx, apply, st);
+    if (!op && ctx->graph_on) {
+      if ((rc = upload_params(ctx, par.data(), par.size()))) return rc;
+      rc = run_fixed_graph(ctx, apply, st);
+    } else {
+      rc = enqueue_fixed(ctx, apply, st);
+    }
     if (rc) return rc;
     disp_done = true;
     for (auto& i : inf) {
@@ -375,6 +408,7 @@ extern "C" int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t ite
                                    double* lmin, double* lmax) {
   int rc = check_ready(ctx);
   if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_lanczos_bounds (use ggem_batch_lanczos_bounds)"))) return rc;
   if (iters < 1 || !lmin || !lmax) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Lanczos arguments");
   if (op && !op->apply) return fail(ctx, GGEM_ERR_INVALID_ARG, "operator without apply");
   const int64_t n = ctx->n, len = 3 * n;
@@ -427,5 +461,164 @@ extern "C" int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t ite
   beta.resize(alpha.size() > 0 ? alpha.size() - 1 : 0);
   *lmin = tridiag_eig(alpha, beta, 0);
   *lmax = tridiag_eig(alpha, beta, int(alpha.size()) - 1);
+  return GGEM_OK;
+}
+
+extern "C" int ggem_batch_cheb_brownian(ggem_ctx* ctx, int32_t N, const double* eig_min, const double* eig_max,
+                                        int32_t nrhs, const double* dw, double* y, double* disp) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  const int nsys = ctx->nsys;
+  if (N < 1 || nrhs < 1 || !eig_min || !eig_max) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Chebyshev arguments");
+  for (int k = 0; k < nsys; ++k)
+    if (!(eig_max[k] > eig_min[k]))
+      return fail(ctx, GGEM_ERR_INVALID_ARG, "system " + std::to_string(k) + ": eig_max must exceed eig_min");
+  if (!dw && (ctx->noise_nrhs != nrhs || !ctx->noise))
+    return fail(ctx, GGEM_ERR_NOT_READY, "no device noise of this nrhs: call ggem_set_noise");
+  const int64_t n = ctx->n, len = 3 * n;
+  if (n == 0) return GGEM_OK;
+  // per system [da, db, c_0 .. c_N] (brownian_system.C:1001-1003, chebyshev.C:57-151)
+  const size_t stride = size_t(N) + 3;
+  std::vector<double> par(stride * nsys), X;
+  for (int k = 0; k < nsys; ++k) {
+    const double d_eig = eig_max[k] - eig_min[k];
+    const double da = 2.0 / d_eig, db = -(eig_max[k] + eig_min[k]) / d_eig;
+    cheb_coefficients(N, da, db, &X);
+    double* pk = par.data() + stride * k;
+    pk[0] = da;
+    pk[1] = db;
+    std::copy(X.begin(), X.end(), pk + 2);
+  }
+  if ((rc = reserve(ctx, 4, nrhs))) return rc;
+  if ((rc = ensure_tensors(ctx))) return rc;
+  Pools& P = pools(ctx);
+  const int prec = ctx->p.precision;
+  const SlotVec x0 = slot_vec(P, 0, nrhs, n, prec), x1 = slot_vec(P, 1, nrhs, n, prec);
+  const SlotVec yv = slot_vec(P, 2, nrhs, n, prec), dwm = slot_vec(P, 3, nrhs, n, prec);
+  const size_t blk = size_t(nrhs) * len;
+  const double* dw_dev = ctx->noise;
+  if (dw) {
+    if ((rc = h2d(ctx, P.io, dw, sizeof(double) * blk))) return rc;
+    dw_dev = P.io;
+  }
+  if ((rc = upload_params(ctx, par.data(), par.size()))) return rc;
+  Applier apply{ctx, nullptr, prec, {}, {}};
+  const FixedStep st{N, nrhs, par.data(), ctx->dpar, dw_dev, x0, x1, yv, dwm, disp != nullptr,
+                     nsys > 1 ? ctx->sys_slot : nullptr};
+  rc = ctx->graph_on ? run_fixed_graph(ctx, apply, st) : enqueue_fixed(ctx, apply, st);
+  if (rc) return rc;
+  hipError_t e;
+  if (y) {
+    if ((e = launch_permute_out(ctx, nrhs, yv.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
+    if ((rc = d2h(ctx, y, P.io, sizeof(double) * blk))) return rc;
+  }
+  if (disp) {
+    if ((e = launch_permute_out(ctx, nrhs, dwm.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
+    if ((rc = d2h(ctx, disp, P.io, sizeof(double) * blk))) return rc;
+  }
+  return GGEM_OK;
+}
+
+// Lanczos on every system at once: one operator application per step for
+// the whole batch, per-system dot products, and per-system breakdown.
+extern "C" int ggem_batch_lanczos_bounds(ggem_ctx* ctx, int32_t iters, uint64_t seed, double* lmin,
+                                         double* lmax) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if (iters < 1 || !lmin || !lmax) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Lanczos arguments");
+  const int nsys = ctx->nsys;
+  const int64_t n = ctx->n, len = 3 * n;
+  if (len == 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no particles");
+  if (nsys == 1) return ggem_lanczos_bounds(ctx, nullptr, iters, seed, lmin, lmax);
+  if ((rc = reserve(ctx, 3, 1))) return rc;
+  if ((rc = ensure_tensors(ctx))) return rc;
+  Pools& P = pools(ctx);
+  const int prec = ctx->p.precision;
+  SlotVec vprev = slot_vec(P, 0, 1, n, prec);
+  SlotVec vcur = slot_vec(P, 1, 1, n, prec);
+  const SlotVec w = slot_vec(P, 2, 1, n, prec);
+  // start vectors: system k draws 3 n_k normals from mt19937_64(seed + k) in
+  // the [3][n_k] slot layout its own context would use; the slots of system
+  // k are its particle range
+  std::vector<double> h(len, 0.0);
+  std::vector<int> kmax(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    const int64_t s0 = ctx->part_off[k], nk = ctx->part_off[k + 1] - s0;
+    kmax[k] = int(std::min<int64_t>(iters, 3 * nk));
+    std::mt19937_64 gen(seed + uint64_t(k));
+    std::normal_distribution<double> nd(0.0, 1.0);
+    double ss = 0;
+    for (int64_t j = 0; j < 3 * nk; ++j) {
+      const double v = nd(gen);
+      h[(j / nk) * n + s0 + (j % nk)] = v;
+      ss += v * v;
+    }
+    const double sc = ss > 0 ? 1.0 / std::sqrt(ss) : 0.0;
+    for (int c = 0; c < 3; ++c)
+      for (int64_t i = 0; i < nk; ++i) h[c * n + s0 + i] *= sc;
+  }
+  if ((rc = h2d(ctx, vcur.soa, h.data(), sizeof(double) * len))) return rc;
+  if ((rc = refresh(ctx, 1, vcur))) return rc;
+  hipError_t e = launch_fill(ctx, len, 0.0, vprev.soa);
+  if (e != hipSuccess) return hip_fail(ctx, e, "fill");
+  if ((rc = ensure(ctx, &P.red, &P.red_cap, size_t(4) * nsys))) return rc;
+  double* d_a = P.red;             // a_k, then the scale 1/b_k
+  double* d_b = P.red + nsys;      // b_{k-1}
+  double* d_dot = P.red + 2 * nsys;
+  Applier apply{ctx, nullptr, prec, {}, {}};
+  const SlotVec none{nullptr, nullptr, nullptr};
+  const ChebEpi plain{0, 0.0, 0.0, 0.0, none, nullptr};
+  std::vector<std::vector<double>> alpha(nsys), beta(nsys);
+  std::vector<double> a(nsys), b2(nsys), bprev(nsys, 0.0), sc(nsys);
+  std::vector<char> active(nsys);
+  int kall = 0;
+  for (int k = 0; k < nsys; ++k) {
+    active[k] = kmax[k] > 0;
+    kall = std::max(kall, kmax[k]);
+  }
+  for (int j = 0; j < kall; ++j) {
+    if ((rc = apply(1, vcur, w, plain))) return rc;
+    if ((e = launch_seg_dot(ctx, w.soa, vcur.soa, d_dot)) != hipSuccess) return hip_fail(ctx, e, "dot");
+    if ((rc = d2h(ctx, a.data(), d_dot, sizeof(double) * nsys))) return rc;
+    if ((rc = h2d(ctx, d_a, a.data(), sizeof(double) * nsys))) return rc;
+    if ((rc = h2d(ctx, d_b, bprev.data(), sizeof(double) * nsys))) return rc;
+    if ((e = launch_seg_axpy2(ctx, d_a, vcur.soa, d_b, vprev.soa, w.soa)) != hipSuccess)
+      return hip_fail(ctx, e, "update");
+    if ((e = launch_seg_dot(ctx, w.soa, w.soa, d_dot)) != hipSuccess) return hip_fail(ctx, e, "dot");
+    if ((rc = d2h(ctx, b2.data(), d_dot, sizeof(double) * nsys))) return rc;
+    bool any = false;
+    for (int k = 0; k < nsys; ++k) {
+      sc[k] = 0.0;
+      if (!active[k]) {
+        bprev[k] = 0.0;
+        continue;
+      }
+      alpha[k].push_back(a[k]);
+      const double bk = std::sqrt(b2[k]);
+      if (j + 1 == kmax[k] || bk <= 1e-12 * std::fabs(a[k])) {
+        active[k] = 0;
+        bprev[k] = 0.0;
+        continue;
+      }
+      beta[k].push_back(bk);
+      sc[k] = 1.0 / bk;
+      bprev[k] = bk;
+      any = true;
+    }
+    if (!any) break;
+    std::swap(vprev, vcur);  // vprev <- v_j; vcur <- w / b
+    if ((rc = h2d(ctx, d_a, sc.data(), sizeof(double) * nsys))) return rc;
+    if ((e = launch_seg_scale(ctx, d_a, w.soa, vcur.soa)) != hipSuccess) return hip_fail(ctx, e, "scale");
+    if ((rc = refresh(ctx, 1, vcur))) return rc;
+  }
+  for (int k = 0; k < nsys; ++k) {
+    if (alpha[k].empty()) {
+      lmin[k] = lmax[k] = 0.0;
+      continue;
+    }
+    beta[k].resize(alpha[k].size() - 1);
+    lmin[k] = tridiag_eig(alpha[k], beta[k], 0);
+    lmax[k] = tridiag_eig(alpha[k], beta[k], int(alpha[k].size()) - 1);
+  }
   return GGEM_OK;
 }

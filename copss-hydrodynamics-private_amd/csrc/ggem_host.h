@@ -62,6 +62,13 @@ inline int check_ready(ggem_ctx* ctx) {
   return GGEM_OK;
 }
 
+// Calls that address the single mesh of a plain context.
+inline int single_only(ggem_ctx* ctx, const char* what) {
+  if (ctx->nsys > 1)
+    return fail(ctx, GGEM_ERR_INVALID_ARG, std::string(what) + " is not available for a batched context");
+  return GGEM_OK;
+}
+
 inline int sync(ggem_ctx* ctx) {
   GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GGEM_OK;

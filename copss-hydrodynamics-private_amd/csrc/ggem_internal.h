@@ -39,6 +39,15 @@ struct GGEMConst {
   int32_t point_type;
 };
 
+// One suspension of a batched context: its grid, the first global element
+// of its cells and its stencil rows.  Elements are numbered system by
+// system, so the slots of a system form one contiguous range.
+struct SysGrid {
+  Grid g;
+  int64_t elem_off;
+  int32_t row_off, nrows;
+};
+
 // Neighbour entries carry the particle slot in the low 26 bits and the
 // periodic image of the pair in the high 6 bits: 2 bits per axis, 1 = add
 // the box length, 2 = subtract it (the single shift of point_vector).
@@ -149,6 +158,21 @@ struct ggem_ctx {
   double* dpar = nullptr;
   size_t dpar_cap = 0;
 
+  // batched suspensions (nsys > 1): particles, elements and slots of all
+  // systems in one store; the stencil rows of system k are
+  // rows_d[row_off .. row_off + nrows)
+  int32_t nsys = 1;
+  std::vector<ggem::SysGrid> sys_h;
+  ggem::SysGrid* sys_d = nullptr;
+  std::vector<int64_t> part_off;     // [nsys + 1] particle ranges
+  int64_t* part_off_d = nullptr;
+  int32_t* sys_part = nullptr;       // [n] system of each particle
+  int32_t* sys_slot = nullptr;       // [n] system of each slot
+  int64_t sys_cap = 0;
+  double* par_h = nullptr;           // pinned staging of per-system recurrence parameters
+  size_t par_cap = 0;
+  hipEvent_t par_ev = nullptr;
+
   // generic scratch (grown on demand)
   void* scratch = nullptr;
 
@@ -188,6 +212,8 @@ hipError_t launch_count_lists(ggem_ctx* ctx);
 // exclusive scan of int64 values; out[len] receives the total
 hipError_t launch_scan64(ggem_ctx* ctx, const int64_t* in, int64_t len, int64_t* out);
 hipError_t launch_fill_ell(ggem_ctx* ctx);
+// batched contexts: system of each particle from part_off_d
+hipError_t launch_part_sys(ggem_ctx* ctx);
 hipError_t launch_elist_count(ggem_ctx* ctx);
 hipError_t launch_elist_fill(ggem_ctx* ctx);
 hipError_t launch_count_pairs(ggem_ctx* ctx, int64_t* counts_d);
@@ -254,6 +280,12 @@ hipError_t launch_cheb_init_y(ggem_ctx* ctx, int nrhs, const double* x0, const d
 hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
                       double* partials);
 hipError_t launch_axpby(ggem_ctx* ctx, int64_t len, double a, const double* x, double b, double* y);
+// batched contexts: out_d[k] = a . b over system k's slots ([3][n] vectors)
+hipError_t launch_seg_dot(ggem_ctx* ctx, const double* a, const double* b, double* out_d);
+// w -= ca[k] x + cb[k] y, out = sc[k] x on the slots of system k
+hipError_t launch_seg_axpy2(ggem_ctx* ctx, const double* ca, const double* x, const double* cb, const double* y,
+                            double* w);
+hipError_t launch_seg_scale(ggem_ctx* ctx, const double* sc, const double* x, double* out);
 hipError_t launch_fill(ggem_ctx* ctx, int64_t len, double v, double* y);
 
 }  // namespace ggem

@@ -50,23 +50,71 @@ __device__ __forceinline__ bool contains_dev(const Grid& g, int64_t e, double x,
   return true;
 }
 
-__global__ void k_bin(Grid g, int64_t n, const double* __restrict__ x, int32_t* __restrict__ elem,
-                      int use_prev, int32_t* __restrict__ count, int32_t* __restrict__ flag) {
+// Grid, stencil rows and cell starts seen by one particle or slot: the
+// context's own, or those of its system in a batched context (cell starts
+// shifted to the system's first element, so element ids stay local).
+struct SysView {
+  Grid g;
+  const StencilRow* rows;
+  int nrows;
+  const int32_t* cs;
+  int64_t eoff;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ SysView sys_view(const Grid& g, const StencilRow* rows, int nrows,
+                                            const int32_t* cs, const SysGrid* sys, const int32_t* sys_of,
+                                            int64_t i) {
+  if constexpr (BATCH) {
+    const SysGrid& S = sys[sys_of[i]];
+    return SysView{S.g, rows + S.row_off, S.nrows, cs + S.elem_off, S.elem_off};
+  } else {
+    return SysView{g, rows, nrows, cs, 0};
+  }
+}
+
+template <bool BATCH>
+__global__ void k_bin(Grid g, const SysGrid* __restrict__ sys, const int32_t* __restrict__ sys_part,
+                      int64_t n, const double* __restrict__ x, int32_t* __restrict__ elem, int use_prev,
+                      int32_t* __restrict__ count, int32_t* __restrict__ flag) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const SysView v = sys_view<BATCH>(g, nullptr, 0, nullptr, sys, sys_part, i);
   const double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
   int64_t e = -1;
   if (use_prev) {
-    const int32_t old = elem[i];
-    if (old >= 0 && old < g.n_elem && contains_dev(g, old, px, py, pz)) e = old;
+    const int64_t old = int64_t(elem[i]) - v.eoff;
+    if (old >= 0 && old < v.g.n_elem && contains_dev(v.g, old, px, py, pz)) e = old;
   }
-  if (e < 0) e = locate_dev(g, px, py, pz);
+  if (e < 0) e = locate_dev(v.g, px, py, pz);
   if (e < 0) {
     atomicOr(flag, 1);
     e = 0;  // keep later kernels in bounds; the host reports the error
   }
+  e += v.eoff;
   elem[i] = int32_t(e);
   atomicAdd(&count[e], 1);
+}
+
+// system of each slot (batched contexts)
+__global__ void k_slot_sys(int64_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ sys_part,
+                           int32_t* __restrict__ sys_slot) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s < n) sys_slot[s] = sys_part[perm[s]];
+}
+
+// system of each particle from the particle ranges (binary search)
+__global__ void k_part_sys(int64_t n, int nsys, const int64_t* __restrict__ part_off,
+                           int32_t* __restrict__ sys_part) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {  // last k with part_off[k] <= i
+    const int mid = (lo + hi + 1) >> 1;
+    if (part_off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  sys_part[i] = lo;
 }
 
 // Exclusive scan of int32 counts in three phases: per-block scan with block
@@ -195,17 +243,20 @@ __global__ void k_gather_forces(int64_t n, const int32_t* __restrict__ perm,
 
 // Neighbour-list lengths by slot (own element + stencil, minus itself), the
 // slice lengths (64 x longest list of each wave of 64 slots) and the total.
+template <bool BATCH>
 __global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int nrows,
                               const int32_t* __restrict__ cell_start,
-                              const int32_t* __restrict__ slot_cell, int64_t n,
+                              const int32_t* __restrict__ slot_cell, const SysGrid* __restrict__ sys,
+                              const int32_t* __restrict__ sys_slot, int64_t n,
                               int32_t* __restrict__ cnt, int64_t* __restrict__ slice_len,
                               unsigned long long* __restrict__ total) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t c = 0;
   if (s < n) {
+    const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
     int ci, cj, ck;
-    cell_ijk(g, slot_cell[s], ci, cj, ck);
-    for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
+    cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
+    for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck,
                      [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
     c -= 1;
     cnt[s] = int32_t(c);
@@ -222,17 +273,20 @@ __global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int n
   }
 }
 
+template <bool BATCH>
 __global__ void k_fill_ell(Grid g, const StencilRow* __restrict__ rows, int nrows,
                            const int32_t* __restrict__ cell_start,
-                           const int32_t* __restrict__ slot_cell, int64_t n,
+                           const int32_t* __restrict__ slot_cell, const SysGrid* __restrict__ sys,
+                           const int32_t* __restrict__ sys_slot, int64_t n,
                            const int64_t* __restrict__ slice_off, uint32_t* __restrict__ ell) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
   int ci, cj, ck;
-  cell_ijk(g, slot_cell[s], ci, cj, ck);
+  cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
   uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
   int64_t k = 0;
-  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
+  for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
     const uint32_t hi = code << kSlotBits;
     for (int32_t t = a; t < b; ++t) {
       if (t == s) continue;
@@ -450,8 +504,14 @@ hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis) {
   if (e != hipSuccess) return e;
   if (ctx->n == 0) return hipSuccess;
   ProfScope ps(ctx, "bin");
-  hipLaunchKernelGGL(k_bin, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid, ctx->n,
-                     ctx->x_in, ctx->elem, hysteresis ? 1 : 0, ctx->cell_count, ctx->flag);
+  if (ctx->nsys > 1)
+    hipLaunchKernelGGL(k_bin<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       ctx->sys_d, ctx->sys_part, ctx->n, ctx->x_in, ctx->elem, hysteresis ? 1 : 0,
+                       ctx->cell_count, ctx->flag);
+  else
+    hipLaunchKernelGGL(k_bin<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       nullptr, nullptr, ctx->n, ctx->x_in, ctx->elem, hysteresis ? 1 : 0, ctx->cell_count,
+                       ctx->flag);
   return hipGetLastError();
 }
 
@@ -477,6 +537,9 @@ hipError_t launch_place(ggem_ctx* ctx) {
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
                      ctx->grid.n_elem, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
                      ctx->slot_cell, ctx->x_in, ctx->xs, ctx->x4);
+  if (ctx->nsys > 1 && ctx->n > 0)
+    hipLaunchKernelGGL(k_slot_sys, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->perm,
+                       ctx->sys_part, ctx->sys_slot);
   return hipGetLastError();
 }
 
@@ -492,9 +555,15 @@ hipError_t launch_count_lists(ggem_ctx* ctx) {
   hipError_t e = hipMemsetAsync(ctx->pairs_d, 0, sizeof(int64_t), ctx->stream);
   if (e != hipSuccess || ctx->n == 0) return e;
   ProfScope ps(ctx, "count_lists");
-  hipLaunchKernelGGL(k_count_lists, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->n, ctx->nbr_cnt,
-                     ctx->slice_len, reinterpret_cast<unsigned long long*>(ctx->pairs_d));
+  auto* tot = reinterpret_cast<unsigned long long*>(ctx->pairs_d);
+  if (ctx->nsys > 1)
+    hipLaunchKernelGGL(k_count_lists<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->sys_d, ctx->sys_slot,
+                       ctx->n, ctx->nbr_cnt, ctx->slice_len, tot);
+  else
+    hipLaunchKernelGGL(k_count_lists<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream,
+                       ctx->grid, ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, nullptr, nullptr,
+                       ctx->n, ctx->nbr_cnt, ctx->slice_len, tot);
   return hipGetLastError();
 }
 
@@ -515,9 +584,21 @@ hipError_t launch_scan64(ggem_ctx* ctx, const int64_t* in, int64_t len, int64_t*
 hipError_t launch_fill_ell(ggem_ctx* ctx) {
   if (ctx->n == 0) return hipSuccess;
   ProfScope ps(ctx, "fill_ell");
-  hipLaunchKernelGGL(k_fill_ell, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->n, ctx->slice_off,
-                     ctx->ell);
+  if (ctx->nsys > 1)
+    hipLaunchKernelGGL(k_fill_ell<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->sys_d, ctx->sys_slot,
+                       ctx->n, ctx->slice_off, ctx->ell);
+  else
+    hipLaunchKernelGGL(k_fill_ell<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, nullptr, nullptr, ctx->n,
+                       ctx->slice_off, ctx->ell);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_sys(ggem_ctx* ctx) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_part_sys, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->nsys,
+                     ctx->part_off_d, ctx->sys_part);
   return hipGetLastError();
 }
 

@@ -47,7 +47,16 @@ struct PairArgs {
   const double* fs;  // own forces (FP64 SoA) for the mobility epilogue
   int64_t n;
   double bsign;
+  const SysGrid* sys;       // batched contexts: grid of each system
+  const int32_t* sys_slot;  // and system of each slot
 };
+
+// Grid of a slot's system.
+template <bool BATCH>
+__device__ __forceinline__ const Grid& slot_grid(const PairArgs& a, int64_t s) {
+  if constexpr (BATCH) return a.sys[a.sys_slot[s]].g;
+  else return a.g;
+}
 
 template <class T>
 struct V4;
@@ -61,12 +70,13 @@ struct V4<double> {
 };
 
 // Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
-template <int P, bool EXACT>
+template <int P, bool EXACT, bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_tensors(PairArgs a, float4* __restrict__ c32,
                                                    double4* __restrict__ c64) {
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t s = blk * kBlock + threadIdx.x;
   if (s >= a.n) return;
+  const Grid& g = slot_grid<BATCH>(a, s);
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
   const uint32_t* lst = a.ell + base;
@@ -82,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_tensors(PairArgs a, float4* __restri
     xa = a.x4[e1 & kSlotMask];
     e2 = lst[uint32_t(min(k + 2, last)) * uint32_t(kSlice)];
     double dx, dy, dz;
-    pair_disp<EXACT>(a.g, xs, xt, e >> kSlotBits, dx, dy, dz);
+    pair_disp<EXACT>(g, xs, xt, e >> kSlotBits, dx, dy, dz);
     const int64_t o = base + int64_t(k) * kSlice;
     if (P == GGEM_FP64) {
       double A, Bx;
@@ -300,6 +310,8 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.fs = ctx->fs;
   a.n = ctx->n;
   a.bsign = double(ctx->bsign);
+  a.sys = ctx->sys_d;
+  a.sys_slot = ctx->sys_slot;
   return a;
 }
 
@@ -332,17 +344,27 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
   const PairArgs a = make_args(ctx);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "tensors_fp64" : "tensors_fp32c");
   const dim3 grid(blocks_for(ctx->n)), block(kBlock);
+  const bool ex = ctx->exact_minimg != 0, bt = ctx->nsys > 1;
+#define GGEM_TENSORS(P, E, B) \
+  hipLaunchKernelGGL((k_tensors<P, E, B>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64)
   if (precision == GGEM_FP64) {
-    if (ctx->exact_minimg)
-      hipLaunchKernelGGL((k_tensors<GGEM_FP64, true>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
-    else
-      hipLaunchKernelGGL((k_tensors<GGEM_FP64, false>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+    if (bt) {
+      if (ex) GGEM_TENSORS(GGEM_FP64, true, true);
+      else GGEM_TENSORS(GGEM_FP64, false, true);
+    } else {
+      if (ex) GGEM_TENSORS(GGEM_FP64, true, false);
+      else GGEM_TENSORS(GGEM_FP64, false, false);
+    }
   } else {
-    if (ctx->exact_minimg)
-      hipLaunchKernelGGL((k_tensors<GGEM_FP32C, true>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
-    else
-      hipLaunchKernelGGL((k_tensors<GGEM_FP32C, false>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64);
+    if (bt) {
+      if (ex) GGEM_TENSORS(GGEM_FP32C, true, true);
+      else GGEM_TENSORS(GGEM_FP32C, false, true);
+    } else {
+      if (ex) GGEM_TENSORS(GGEM_FP32C, true, false);
+      else GGEM_TENSORS(GGEM_FP32C, false, false);
+    }
   }
+#undef GGEM_TENSORS
   return hipGetLastError();
 }
 

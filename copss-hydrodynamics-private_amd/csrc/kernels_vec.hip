@@ -116,6 +116,42 @@ __global__ void k_dot_final(int nparts, const double* __restrict__ partials, dou
   if (threadIdx.x == 0) out[r] = t;
 }
 
+// Per-system dot products of slot SoA vectors ([3][n]): block k sums the
+// slot range of system k (its cells' slots are contiguous).
+__global__ void k_seg_dot(const SysGrid* __restrict__ sys, const int32_t* __restrict__ cell_start, int64_t n,
+                          const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+  __shared__ double sh[kBlock / 64];
+  const SysGrid& S = sys[blockIdx.x];
+  const int64_t s0 = cell_start[S.elem_off], s1 = cell_start[S.elem_off + S.g.n_elem];
+  double acc = 0.0;
+  for (int c = 0; c < 3; ++c)
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) acc += a[c * n + s] * b[c * n + s];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// w = w - ca[k] x - cb[k] y per slot of system k
+__global__ void k_seg_axpy2(int64_t n, const int32_t* __restrict__ sys_slot, const double* __restrict__ ca,
+                            const double* __restrict__ x, const double* __restrict__ cb,
+                            const double* __restrict__ y, double* __restrict__ w) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int k = sys_slot[s];
+  const double a = ca[k], b = cb[k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) w[c * n + s] = w[c * n + s] - a * x[c * n + s] - b * y[c * n + s];
+}
+
+// out = sc[k] x per slot of system k
+__global__ void k_seg_scale(int64_t n, const int32_t* __restrict__ sys_slot, const double* __restrict__ sc,
+                            const double* __restrict__ x, double* __restrict__ out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double a = sc[sys_slot[s]];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c * n + s] = a * x[c * n + s];
+}
+
 __global__ void k_axpby(int64_t len, double a, const double* __restrict__ x, double b,
                         double* __restrict__ y) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -176,6 +212,28 @@ hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b,
   hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, ctx->stream, len, nrhs, a, b,
                      partials);
   hipLaunchKernelGGL(k_dot_final, dim3(nrhs), dim3(kBlock), 0, ctx->stream, kDotBlocks, partials, out_d);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_dot(ggem_ctx* ctx, const double* a, const double* b, double* out_d) {
+  ProfScope ps(ctx, "seg_dot");
+  hipLaunchKernelGGL(k_seg_dot, dim3(unsigned(ctx->nsys)), dim3(kBlock), 0, ctx->stream, ctx->sys_d,
+                     ctx->cell_start, ctx->n, a, b, out_d);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_axpy2(ggem_ctx* ctx, const double* ca, const double* x, const double* cb, const double* y,
+                            double* w) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_axpy2, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->sys_slot,
+                     ca, x, cb, y, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_scale(ggem_ctx* ctx, const double* sc, const double* x, double* out) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_scale, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->sys_slot,
+                     sc, x, out);
   return hipGetLastError();
 }
 

@@ -9,6 +9,8 @@ the path is measured on:
   slit25k    400 chains x 64 beads in a 160x160x100 slit (walls at z = 0, 100),
              80x80x50 elements
   beads1m    1,000,000 beads, [0,500)^3 periodic, 128^3 elements
+  batch256   256 independent periodic suspensions, N_k uniform in [500, 8000],
+             5% volume fraction, ~2a elements, 16 noise vectors each
   ggem3      the reference's GGEM validation case: 3 beads in a 30^3
              wall-bounded box, 15^3 elements, alpha = 0.35
 
@@ -124,3 +126,32 @@ def polymer_chains(cfg, n_chains, n_beads, seed, bond=0.97, excl=0.9):
             grid.setdefault(key(p), []).append(i)
             i += 1
     return out
+
+
+def batch_cfgs(nsys=256, seed=4004, n_lo=500, n_hi=8000, phi=0.05, **kw):
+    """Parameter dicts and bead counts of a batch of periodic suspensions:
+    N_k uniform in [n_lo, n_hi] (numpy Generator(PCG64(seed))), cube side
+    L_k = (N_k (4 pi / 3) / phi)^(1/3) for volume fraction phi of unit
+    beads, floor(L_k / 2) elements per axis."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(n_lo, n_hi + 1, size=nsys)
+    cfgs = []
+    for nk in counts:
+        L = (nk * (4.0 * math.pi / 3.0) / phi) ** (1.0 / 3.0)
+        ne = int(L // 2.0)
+        cfgs.append(_cfg([0, 0, 0], [L, L, L], [1, 1, 1], [ne, ne, ne], **kw))
+    return cfgs, [int(c) for c in counts]
+
+
+def make_batch(nsys=256, nrhs=16, seed=4004, **kw):
+    """(cfgs, positions per system, forces per system, noise [nrhs, N, 3]):
+    random sequential addition at 2.2a (std::mt19937_64(10000 + k)), N(0,1)
+    forces and noise (std::mt19937_64(seed + 1000), (seed + 2000))."""
+    cfgs, counts = batch_cfgs(nsys, seed, **kw)
+    xs = [G.rsa_positions(c, nk, 2.2, 10000 + k) for k, (c, nk) in enumerate(zip(cfgs, counts))]
+    ntot = sum(counts)
+    f = G.normal(3 * ntot, seed + 1000).reshape(ntot, 3)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    fs = [f[off[k]:off[k + 1]] for k in range(nsys)]
+    dw = G.normal(nrhs * 3 * ntot, seed + 2000).reshape(nrhs, ntot, 3) if nrhs else None
+    return cfgs, xs, fs, dw

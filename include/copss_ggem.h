@@ -218,6 +218,37 @@ int ggem_set_graph(ggem_ctx* ctx, int32_t enable);
 int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,
                         double* lmin, double* lmax);
 
+/* ---- batched suspensions ---------------------------------------------- */
+/* Many independent suspensions in one context: the particles, cells and
+ * neighbour lists of all systems share one device store, so every neighbour
+ * build, pair sum and operator application covers the whole batch in one
+ * launch.  No counterpart in the reference, which runs one suspension per
+ * process (copss.C:1565-1745); each system follows the same arithmetic as a
+ * plain context built from its own params.
+ * params[nsys]: per-system box, grid, periodicity and search radius; mu,
+ * alpha, xi, point type, precision, compat flags and device must agree.
+ * On a batched context the whole-store calls work unchanged (build,
+ * local_velocity_beads, combine_velocity, global_self_exclusion,
+ * mobility_apply, set_forces, set_noise, get_elem_ids with element ids
+ * numbered system after system); the single-mesh calls (spread,
+ * local_velocity_points, the CSR exports, cheb_brownian, lanczos_bounds)
+ * return GGEM_ERR_INVALID_ARG. */
+int ggem_create_batch(const ggem_params* params, int32_t nsys, ggem_ctx** out);
+/* counts[nsys] particles per system; xyz, f hold the systems one after the
+ * other (3 * sum(counts) values, interleaved). */
+int ggem_batch_set_particles(ggem_ctx* ctx, const int64_t* counts, const double* xyz, const double* f);
+/* Fixed-order Chebyshev step of every system with its own bounds
+ * eig_min[k], eig_max[k] (GGEM_CHEB_FIXED semantics of ggem_cheb_brownian,
+ * device operator).  dw, y, disp: nrhs blocks of 3 * sum(counts); dw NULL
+ * uses ggem_set_noise's vectors.  Replayed from a captured graph when
+ * ggem_set_graph is on. */
+int ggem_batch_cheb_brownian(ggem_ctx* ctx, int32_t N, const double* eig_min, const double* eig_max,
+                             int32_t nrhs, const double* dw, double* y, double* disp);
+/* Lanczos bounds of every system's mobility (ggem_lanczos_bounds per
+ * system, start vector of system k from mt19937_64(seed + k)).  lmin, lmax:
+ * nsys values each. */
+int ggem_batch_lanczos_bounds(ggem_ctx* ctx, int32_t iters, uint64_t seed, double* lmin, double* lmax);
+
 /* ---- host utilities (no GPU needed) ---------------------------------- */
 /* The reference's Brownian noise: std::default_random_engine(seed) +
  * std::normal_distribution(0, sqrt(dt)), each value times sqrt(2)
