@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-slit", action="store_true")
+    ap.add_argument("--slit-steps", type=int, default=100)
     ap.add_argument("--batch-systems", type=int, default=256)
     return ap.parse_args()
 
@@ -116,6 +118,30 @@ def bench_batch(G, W, args):
             "pair_interactions_per_s": evals * pairs / (ms * 1e-3),
             "brownian_steps_per_s": 1e3 / ms, "eager_ms_per_step": ms_eager,
             "apply_fp32c_avg_ms": a_ms / max(a_n, 1), "apply_launches_per_step": a_n / max(args.steps, 1)}
+
+
+def bench_slit(G, W, args):
+    """Polymer-slit workload: 400 chains x 64 beads (FENE k=30, b_max=1.5a,
+    LJ slit walls at z = 0, 100), 160x160x100 box periodic in x, y, 80x80x50
+    elements; Fixman mid-point steps (copss.C:1565-1745) with the device
+    mobility, Chebyshev order 30 in the reference's convergence mode, FP32-
+    compensated pair sums.  Forces and position updates on the host."""
+    import fixman as F
+    cfg, x, _ = W.make_inputs("slit25k")
+    cfg["precision"] = G.FP32C
+    forces = F.SlitPolymerForces(cfg, 400, 64)
+    ctx = G.Context(cfg)
+    integ = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), forces, dt=1e-3, cheb_order=30)
+    x = integ.step(x)  # warm-up, eigenvalue bounds
+    t0 = time.perf_counter()
+    for _ in range(args.slit_steps):
+        x = integ.step(x)
+    dt_s = time.perf_counter() - t0
+    ctx.close()
+    return {"beads": int(len(x)), "steps": args.slit_steps, "ms_per_step": dt_s * 1e3 / args.slit_steps,
+            "brownian_steps_per_s": args.slit_steps / dt_s, "cheb_retries": integ.stats["cheb_retries"],
+            "cheb_failures": integ.stats["cheb_failures"],
+            "note": "host forces and position update; includes host<->device copies per step"}
 
 
 def main():
@@ -231,6 +257,8 @@ def main():
     ctx.close()
     if not args.no_batch:
         out["batch256"] = bench_batch(G, W, args)
+    if not args.no_slit:
+        out["slit25k_fixman"] = bench_slit(G, W, args)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

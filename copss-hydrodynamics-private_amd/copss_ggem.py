@@ -44,7 +44,8 @@ EXPORTS = [
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
-    "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds", "ggem_util_reference_noise",
+    "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
+    "ggem_set_autotune", "ggem_tuning_report", "ggem_util_reference_noise",
     "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
 ]
@@ -120,6 +121,8 @@ def lib():
             "ggem_set_noise": [P, I32, P],
             "ggem_cheb_brownian": [P, P, I32, D, D, D, I32, I32, P, P, P, P],
             "ggem_set_graph": [P, I32],
+            "ggem_set_autotune": [P, I32],
+            "ggem_tuning_report": [P, P, I64, P],
             "ggem_create_batch": [P, I32, C.POINTER(C.c_void_p)],
             "ggem_batch_set_particles": [P, P, P, P],
             "ggem_batch_cheb_brownian": [P, I32, P, P, I32, P, P, P],
@@ -396,6 +399,22 @@ class Context:
     def set_graph(self, enable=True):
         """Fixed-order Chebyshev steps replay one captured HIP graph."""
         self._c(lib().ggem_set_graph(self._h, int(enable)))
+
+    def set_autotune(self, enable=True):
+        """Pick launch shapes by timing the candidates on first use."""
+        self._c(lib().ggem_set_autotune(self._h, int(enable)))
+
+    def tuning(self):
+        """{signature: (pick, [ms per candidate])} of the autotuner."""
+        n = C.c_int64()
+        self._c(lib().ggem_tuning_report(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        self._c(lib().ggem_tuning_report(self._h, buf, n.value + 1, C.byref(n)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out[f[0]] = (int(f[1]), [float(v) for v in f[2:]])
+        return out
 
     # profiling
     def profile(self, enable=True):

@@ -74,6 +74,56 @@ static void prof_collect(ggem_ctx* ctx) {
   ctx->prof.pending.clear();
 }
 
+int tune_lookup(const ggem_ctx* ctx, const std::string& key) {
+  if (!ctx->tune.on) return 0;
+  auto it = ctx->tune.pick.find(key);
+  return it == ctx->tune.pick.end() ? 0 : it->second;
+}
+
+std::string tune_key(const char* kernel, int precision, int extra, int64_t n) {
+  int lg = 0;
+  while ((int64_t(1) << (lg + 1)) <= n) ++lg;
+  return std::string(kernel) + (precision == GGEM_FP64 ? "/fp64/" : "/fp32c/") + std::to_string(extra) + "/" +
+         std::to_string(lg);
+}
+
+int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch) {
+  Tuner& T = ctx->tune;
+  if (!T.on) return 0;
+  auto it = T.pick.find(key);
+  if (it != T.pick.end()) return it->second;
+  if (ncand <= 1) return 0;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess) return 0;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return 0;
+  }
+  const bool prof = ctx->prof.on;
+  ctx->prof.on = false;
+  std::vector<float> ms(ncand, 1e30f);
+  for (int c = 0; c < ncand; ++c) {
+    if (launch(c) != hipSuccess) continue;  // warm-up; a shape that cannot launch is skipped
+    bool ok = hipEventRecord(a, ctx->stream) == hipSuccess;
+    for (int r = 0; r < 3 && ok; ++r) ok = launch(c) == hipSuccess;
+    ok = ok && hipEventRecord(b, ctx->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
+    float t = 0.f;
+    if (ok && hipEventElapsedTime(&t, a, b) == hipSuccess) ms[c] = t / 3.f;
+  }
+  (void)hipGetLastError();
+  ctx->prof.on = prof;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  int best = 0;
+  for (int c = 1; c < ncand; ++c)
+    if (ms[c] < ms[best]) best = c;
+  T.pick[key] = best;
+  T.ms[key] = ms;
+  return best;
+}
+
 int ensure_tensors(ggem_ctx* ctx) {
   const int prec = ctx->p.precision;
   if (prec == GGEM_FP64 ? ctx->coef64_valid : ctx->coef32_valid) return GGEM_OK;
@@ -858,6 +908,35 @@ int ggem_mobility_apply(ggem_ctx* ctx, const double* f, double* u) {
 int ggem_set_graph(ggem_ctx* ctx, int32_t enable) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   ctx->graph_on = enable != 0;
+  return GGEM_OK;
+}
+
+int ggem_set_autotune(ggem_ctx* ctx, int32_t enable) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  ctx->tune.on = enable != 0;
+  return GGEM_OK;
+}
+
+int ggem_tuning_report(ggem_ctx* ctx, char* buf, int64_t cap, int64_t* len) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  std::string out;
+  for (const auto& kv : ctx->tune.pick) {
+    out += kv.first + " " + std::to_string(kv.second);
+    auto m = ctx->tune.ms.find(kv.first);
+    if (m != ctx->tune.ms.end())
+      for (float t : m->second) {
+        char b[32];
+        std::snprintf(b, sizeof(b), " %.5f", double(t));
+        out += b;
+      }
+    out += "\n";
+  }
+  if (len) *len = int64_t(out.size());
+  if (buf && cap > 0) {
+    const size_t k = std::min<size_t>(out.size(), size_t(cap - 1));
+    std::memcpy(buf, out.data(), k);
+    buf[k] = 0;
+  }
   return GGEM_OK;
 }
 

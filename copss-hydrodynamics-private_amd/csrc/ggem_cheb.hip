@@ -205,6 +205,9 @@ int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
       GGEM_HIP(ctx, hipGraphExecDestroy(ctx->cheb_exec));
       ctx->cheb_exec = nullptr;
     }
+    // launch shapes cannot be timed inside a capture: one eager pass first
+    // lets the autotuner pick them (its results are overwritten by the replay)
+    if (ctx->tune.on && (rc = enqueue_fixed(ctx, apply, s))) return rc;
     // event brackets cannot be recorded into a capture
     const bool prof = ctx->prof.on;
     ctx->prof.on = false;

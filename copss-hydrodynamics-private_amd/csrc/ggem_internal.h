@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -70,6 +72,15 @@ struct Profiler {
     int64_t launches;
   };
   std::vector<Acc> acc;
+};
+
+// Launch-shape autotuner: for each kernel signature (kernel, precision,
+// operand count, log2 size), the fastest of a fixed candidate list of
+// launch shapes, all of which compute bit-identical results.
+struct Tuner {
+  bool on = false;
+  std::map<std::string, int> pick;
+  std::map<std::string, std::vector<float>> ms;  // measured ms per candidate
 };
 
 }  // namespace ggem
@@ -177,6 +188,7 @@ struct ggem_ctx {
   void* scratch = nullptr;
 
   ggem::Profiler prof;
+  ggem::Tuner tune;
 };
 
 namespace ggem {
@@ -201,6 +213,16 @@ struct ProfScope {
   ProfScope(ggem_ctx* c, const char* n) : ctx(c), name(n) { prof_begin(ctx, name, &a, &b); }
   ~ProfScope() { prof_end(ctx, name, a, b); }
 };
+
+// Candidate index for `key`: the recorded pick, else (autotuning on, stream
+// not capturing) each candidate is launched once to warm up and three times
+// under events and the fastest is kept; 0 (the default shape) otherwise.
+// launch(c) must be idempotent.
+int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch);
+// The recorded pick without tuning (0 when none or autotuning is off).
+int tune_lookup(const ggem_ctx* ctx, const std::string& key);
+// "name/precision/.../log2 n" signature of a launch
+std::string tune_key(const char* kernel, int precision, int extra, int64_t n);
 
 // ---- launchers (kernels_build.hip) ----
 hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis);

@@ -24,6 +24,8 @@
 // displacement in FP64, tensor in FP32 (erfc form, shared exponentials),
 // partial sums in FP32 merged every 8 neighbours into a Kahan accumulator;
 // the epilogue (combination, recurrence) runs in FP64.
+#include <vector>
+
 #include "ggem_device.h"
 
 namespace ggem {
@@ -49,7 +51,23 @@ struct PairArgs {
   double bsign;
   const SysGrid* sys;       // batched contexts: grid of each system
   const int32_t* sys_slot;  // and system of each slot
+  int32_t xcd;              // renumber blocks per XCD (xcd_block)
 };
+
+// Launch shapes the autotuner chooses from (index 0 is the default); every
+// shape computes the same per-slot arithmetic, so results do not depend on
+// the choice.
+struct Shape {
+  int block, xcd;
+};
+constexpr Shape kApplyShapes[] = {{256, 1}, {128, 1}, {512, 1}, {64, 1}, {256, 0}};
+constexpr int kNApply = sizeof(kApplyShapes) / sizeof(kApplyShapes[0]);
+constexpr Shape kTensorShapes[] = {{256, 1}, {128, 1}, {512, 1}, {64, 1}};
+constexpr int kNTensor = sizeof(kTensorShapes) / sizeof(kTensorShapes[0]);
+
+__device__ __forceinline__ int64_t block_index(int xcd) {
+  return xcd ? xcd_block(blockIdx.x, gridDim.x) : int64_t(blockIdx.x);
+}
 
 // Grid of a slot's system.
 template <bool BATCH>
@@ -70,11 +88,10 @@ struct V4<double> {
 };
 
 // Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
-template <int P, bool EXACT, bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_tensors(PairArgs a, float4* __restrict__ c32,
-                                                   double4* __restrict__ c64) {
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t s = blk * kBlock + threadIdx.x;
+template <int P, bool EXACT, bool BATCH, int BLK>
+__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float4* __restrict__ c32,
+                                                double4* __restrict__ c64) {
+  const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
   const Grid& g = slot_grid<BATCH>(a, s);
   const int cnt = a.cnt[s];
@@ -111,8 +128,8 @@ __global__ void __launch_bounds__(kBlock) k_tensors(PairArgs a, float4* __restri
 
 // Output of one target: m = sum over its list of G_ij x_j for NR operand
 // vectors, then the epilogue.
-template <int P, int NR, int EPI>
-__global__ void __launch_bounds__(kBlock) k_apply(PairArgs a, const typename V4<typename std::conditional<P == GGEM_FP64, double, float>::type>::type* __restrict__ coef,
+template <int P, int NR, int EPI, int BLK>
+__global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restrict__ coef_raw,
                                                  const double4* __restrict__ xg64,
                                                  const float4* __restrict__ xg32,
                                                  const double* __restrict__ xsoa, SlotVec v,
@@ -120,8 +137,11 @@ __global__ void __launch_bounds__(kBlock) k_apply(PairArgs a, const typename V4<
                                                  double* __restrict__ out) {
   using T = typename std::conditional<P == GGEM_FP64, double, float>::type;
   using T4 = typename V4<T>::type;
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t s = blk * kBlock + threadIdx.x;
+  // (the tensor pointer is untyped in the signature: a type computed from
+  // the header's unnamed precision enum would be mangled differently by the
+  // host and device compilations)
+  const T4* __restrict__ coef = static_cast<const T4*>(coef_raw);
+  const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
   const int64_t n = a.n;
   const int cnt = a.cnt[s];
@@ -312,29 +332,71 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.bsign = double(ctx->bsign);
   a.sys = ctx->sys_d;
   a.sys_slot = ctx->sys_slot;
+  a.xcd = 1;
   return a;
 }
 
-template <int P, int NR, int EPI>
-void launch_k(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float4* g32, const double* soa,
-              const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
+template <int P, int NR, int EPI, int BLK>
+void launch_kb(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float4* g32, const double* soa,
+               const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
   if constexpr (P == GGEM_FP64)
-    hipLaunchKernelGGL((k_apply<P, NR, EPI>), dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a,
+    hipLaunchKernelGGL((k_apply<P, NR, EPI, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
                        ctx->coef64, g64, g32, soa, v, epi, u_glob, out);
   else
-    hipLaunchKernelGGL((k_apply<P, NR, EPI>), dim3(blocks_for(a.n)), dim3(kBlock), 0, ctx->stream, a,
+    hipLaunchKernelGGL((k_apply<P, NR, EPI, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
                        ctx->coef32, g64, g32, soa, v, epi, u_glob, out);
 }
 
-template <int P>
-void launch_cheb_nr(ggem_ctx* ctx, const PairArgs& a, int nr, const double4* g64, const float4* g32,
-                    const double* soa, const SlotVec& v, const ChebEpi& epi) {
-  switch (nr) {
-    case 1: launch_k<P, 1, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
-    case 2: launch_k<P, 2, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
-    case 4: launch_k<P, 4, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
-    default: launch_k<P, 8, EPI_CHEB>(ctx, a, g64, g32, soa, v, epi, nullptr, nullptr); break;
+// one apply launch in candidate shape c
+template <int P, int NR, int EPI>
+hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const float4* g32, const double* soa,
+                    const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
+  const Shape sh = kApplyShapes[c];
+  a.xcd = sh.xcd;
+  switch (sh.block) {
+    case 64: launch_kb<P, NR, EPI, 64>(ctx, a, g64, g32, soa, v, epi, u_glob, out); break;
+    case 128: launch_kb<P, NR, EPI, 128>(ctx, a, g64, g32, soa, v, epi, u_glob, out); break;
+    case 512: launch_kb<P, NR, EPI, 512>(ctx, a, g64, g32, soa, v, epi, u_glob, out); break;
+    default: launch_kb<P, NR, EPI, 256>(ctx, a, g64, g32, soa, v, epi, u_glob, out); break;
   }
+  return hipGetLastError();
+}
+
+template <int P>
+hipError_t launch_cheb_nr(ggem_ctx* ctx, const PairArgs& a, int c, int nr, const double4* g64, const float4* g32,
+                          const double* soa, const SlotVec& v, const ChebEpi& epi) {
+  switch (nr) {
+    case 1: return launch_k<P, 1, EPI_CHEB>(ctx, a, c, g64, g32, soa, v, epi, nullptr, nullptr);
+    case 2: return launch_k<P, 2, EPI_CHEB>(ctx, a, c, g64, g32, soa, v, epi, nullptr, nullptr);
+    case 4: return launch_k<P, 4, EPI_CHEB>(ctx, a, c, g64, g32, soa, v, epi, nullptr, nullptr);
+    default: return launch_k<P, 8, EPI_CHEB>(ctx, a, c, g64, g32, soa, v, epi, nullptr, nullptr);
+  }
+}
+
+template <int P, bool E, bool B, int BLK>
+void launch_tb(ggem_ctx* ctx, const PairArgs& a) {
+  hipLaunchKernelGGL((k_tensors<P, E, B, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
+                     ctx->coef32, ctx->coef64);
+}
+
+template <int P, bool E, bool B>
+hipError_t launch_t(ggem_ctx* ctx, PairArgs a, int c) {
+  const Shape sh = kTensorShapes[c];
+  a.xcd = sh.xcd;
+  switch (sh.block) {
+    case 64: launch_tb<P, E, B, 64>(ctx, a); break;
+    case 128: launch_tb<P, E, B, 128>(ctx, a); break;
+    case 512: launch_tb<P, E, B, 512>(ctx, a); break;
+    default: launch_tb<P, E, B, 256>(ctx, a); break;
+  }
+  return hipGetLastError();
+}
+
+template <int P>
+hipError_t launch_t_any(ggem_ctx* ctx, const PairArgs& a, int c) {
+  const bool ex = ctx->exact_minimg != 0, bt = ctx->nsys > 1;
+  if (bt) return ex ? launch_t<P, true, true>(ctx, a, c) : launch_t<P, false, true>(ctx, a, c);
+  return ex ? launch_t<P, true, false>(ctx, a, c) : launch_t<P, false, false>(ctx, a, c);
 }
 
 }  // namespace
@@ -342,30 +404,12 @@ void launch_cheb_nr(ggem_ctx* ctx, const PairArgs& a, int nr, const double4* g64
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
   if (ctx->n == 0) return hipSuccess;
   const PairArgs a = make_args(ctx);
+  auto run = [&](int c) {
+    return precision == GGEM_FP64 ? launch_t_any<GGEM_FP64>(ctx, a, c) : launch_t_any<GGEM_FP32C>(ctx, a, c);
+  };
+  const int c = tune_pick(ctx, tune_key("tensors", precision, ctx->nsys > 1, ctx->n), kNTensor, run);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "tensors_fp64" : "tensors_fp32c");
-  const dim3 grid(blocks_for(ctx->n)), block(kBlock);
-  const bool ex = ctx->exact_minimg != 0, bt = ctx->nsys > 1;
-#define GGEM_TENSORS(P, E, B) \
-  hipLaunchKernelGGL((k_tensors<P, E, B>), grid, block, 0, ctx->stream, a, ctx->coef32, ctx->coef64)
-  if (precision == GGEM_FP64) {
-    if (bt) {
-      if (ex) GGEM_TENSORS(GGEM_FP64, true, true);
-      else GGEM_TENSORS(GGEM_FP64, false, true);
-    } else {
-      if (ex) GGEM_TENSORS(GGEM_FP64, true, false);
-      else GGEM_TENSORS(GGEM_FP64, false, false);
-    }
-  } else {
-    if (bt) {
-      if (ex) GGEM_TENSORS(GGEM_FP32C, true, true);
-      else GGEM_TENSORS(GGEM_FP32C, false, true);
-    } else {
-      if (ex) GGEM_TENSORS(GGEM_FP32C, true, false);
-      else GGEM_TENSORS(GGEM_FP32C, false, false);
-    }
-  }
-#undef GGEM_TENSORS
-  return hipGetLastError();
+  return run(c);
 }
 
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob, double* out) {
@@ -373,19 +417,21 @@ hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double
   const PairArgs a = make_args(ctx);
   const SlotVec none{nullptr, nullptr, nullptr};
   const ChebEpi no_epi{0, 0.0, 0.0, 0.0, none, nullptr};
+  auto run = [&](int c) {
+    if (precision == GGEM_FP64)
+      return epi == EPI_LOCAL
+                 ? launch_k<GGEM_FP64, 1, EPI_LOCAL>(ctx, a, c, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out)
+                 : launch_k<GGEM_FP64, 1, EPI_MOBILITY>(ctx, a, c, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob,
+                                                        out);
+    return epi == EPI_LOCAL
+               ? launch_k<GGEM_FP32C, 1, EPI_LOCAL>(ctx, a, c, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out)
+               : launch_k<GGEM_FP32C, 1, EPI_MOBILITY>(ctx, a, c, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob,
+                                                       out);
+  };
+  // both epilogues write out[] only: idempotent, so the shape can be tuned here
+  const int c = tune_pick(ctx, tune_key("pair_beads", precision, epi, ctx->n), kNApply, run);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "pair_beads_fp64" : "pair_beads_fp32c");
-  if (precision == GGEM_FP64) {
-    if (epi == EPI_LOCAL)
-      launch_k<GGEM_FP64, 1, EPI_LOCAL>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
-    else
-      launch_k<GGEM_FP64, 1, EPI_MOBILITY>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
-  } else {
-    if (epi == EPI_LOCAL)
-      launch_k<GGEM_FP32C, 1, EPI_LOCAL>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
-    else
-      launch_k<GGEM_FP32C, 1, EPI_MOBILITY>(ctx, a, ctx->fg64, ctx->fg32, nullptr, none, no_epi, u_glob, out);
-  }
-  return hipGetLastError();
+  return run(c);
 }
 
 hipError_t launch_pair_points(ggem_ctx* ctx, int precision, int64_t m, const double* pts,
@@ -406,25 +452,48 @@ hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x
                         const ChebEpi& epi) {
   if (ctx->n == 0) return hipSuccess;
   const PairArgs a = make_args(ctx);
-  ProfScope ps(ctx, precision == GGEM_FP64 ? "apply_fp64" : "apply_fp32c");
   const int64_t n = ctx->n, stride = 3 * n;
-  int done = 0;
-  while (done < nrhs) {
+  struct Chunk {
+    int nr;
+    ChebEpi e;
+    SlotVec vv;
+    const double4* g64;
+    const float4* g32;
+    const double* soa;
+  };
+  std::vector<Chunk> chunks;
+  for (int done = 0; done < nrhs;) {
     const int left = nrhs - done;
     const int nr = left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1;
     ChebEpi e = epi;
     if (e.xm1.soa) e.xm1.soa += done * stride;
     if (e.y) e.y += done * stride;
     SlotVec vv{v.soa + done * stride, v.g32 ? v.g32 + done * n : nullptr, v.g64 ? v.g64 + done * n : nullptr};
-    const double4* g64 = x.g64 ? x.g64 + done * n : nullptr;
-    const float4* g32 = x.g32 ? x.g32 + done * n : nullptr;
-    if (precision == GGEM_FP64)
-      launch_cheb_nr<GGEM_FP64>(ctx, a, nr, g64, g32, x.soa + done * stride, vv, e);
-    else
-      launch_cheb_nr<GGEM_FP32C>(ctx, a, nr, g64, g32, x.soa + done * stride, vv, e);
+    chunks.push_back({nr, e, vv, x.g64 ? x.g64 + done * n : nullptr, x.g32 ? x.g32 + done * n : nullptr,
+                      x.soa + done * stride});
     done += nr;
   }
-  return hipGetLastError();
+  auto run = [&](const Chunk& k, int c) {
+    return precision == GGEM_FP64 ? launch_cheb_nr<GGEM_FP64>(ctx, a, c, k.nr, k.g64, k.g32, k.soa, k.vv, k.e)
+                                  : launch_cheb_nr<GGEM_FP32C>(ctx, a, c, k.nr, k.g64, k.g32, k.soa, k.vv, k.e);
+  };
+  // shapes: tuned on launches that only write v (kinds 0 and 1 with v not
+  // aliasing x); the recurrence step (kind 2 updates y and overwrites
+  // x_{k-1}) reuses the pick of its operand count
+  std::vector<int> pick(chunks.size());
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    const std::string key = tune_key("apply", precision, chunks[i].nr, n);
+    if (epi.kind != 2 && v.soa != x.soa)
+      pick[i] = tune_pick(ctx, key, kNApply, [&](int c) { return run(chunks[i], c); });
+    else
+      pick[i] = tune_lookup(ctx, key);
+  }
+  ProfScope ps(ctx, precision == GGEM_FP64 ? "apply_fp64" : "apply_fp32c");
+  for (size_t i = 0; i < chunks.size(); ++i) {
+    const hipError_t e = run(chunks[i], pick[i]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace ggem

@@ -30,7 +30,10 @@ namespace {
 
 constexpr int kMaxNu = 27;
 constexpr int kMergeEvery = 8;
-constexpr int kElems = 64;  // elements per block
+// elements per block, candidates of the autotuner (index 0 is the default);
+// the per-element arithmetic does not depend on the choice
+constexpr int kElemChoices[] = {64, 32, 16};
+constexpr int kNElem = sizeof(kElemChoices) / sizeof(kElemChoices[0]);
 
 template <class T>
 struct SpreadArgs {
@@ -62,7 +65,7 @@ __device__ __forceinline__ double4 load_f(const float4*, const double4* f64, uin
   return f64[t];
 }
 
-template <class T, int NQ1, bool EXACT>
+template <class T, int NQ1, bool EXACT, int kElems>
 __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const double* __restrict__ IF,
                                                         int n_u, double* __restrict__ rho,
                                                         double* __restrict__ Fe) {
@@ -163,8 +166,20 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
   }
 }
 
+template <class T, int NQ1, int KE>
+void launch_spk(ggem_ctx* ctx, const SpreadArgs<T>& a, const double* IF_d, int n_u, double* rho, double* Fe) {
+  const unsigned nb = unsigned((ctx->grid.n_elem + KE - 1) / KE);
+  if (ctx->exact_minimg)
+    hipLaunchKernelGGL((k_spread<T, NQ1, true, KE>), dim3(nb), dim3(KE * NQ1), 0, ctx->stream, a, IF_d, n_u, rho,
+                       Fe);
+  else
+    hipLaunchKernelGGL((k_spread<T, NQ1, false, KE>), dim3(nb), dim3(KE * NQ1), 0, ctx->stream, a, IF_d, n_u,
+                       rho, Fe);
+}
+
 template <class T, int NQ1>
-void launch_sp(ggem_ctx* ctx, const double* gp, const double* IF_d, int n_u, double* rho, double* Fe) {
+hipError_t launch_sp(ggem_ctx* ctx, int c, const double* gp, const double* IF_d, int n_u, double* rho,
+                     double* Fe) {
   SpreadArgs<T> a;
   a.g = ctx->grid;
   a.alpha2 = T(ctx->gc.alpha2);
@@ -176,13 +191,12 @@ void launch_sp(ggem_ctx* ctx, const double* gp, const double* IF_d, int n_u, dou
   a.x4 = ctx->x4;
   a.fg32 = ctx->fg32;
   a.fg64 = ctx->fg64;
-  const unsigned nb = unsigned((ctx->grid.n_elem + kElems - 1) / kElems);
-  if (ctx->exact_minimg)
-    hipLaunchKernelGGL((k_spread<T, NQ1, true>), dim3(nb), dim3(kElems * NQ1), 0, ctx->stream, a, IF_d, n_u,
-                       rho, Fe);
-  else
-    hipLaunchKernelGGL((k_spread<T, NQ1, false>), dim3(nb), dim3(kElems * NQ1), 0, ctx->stream, a, IF_d, n_u,
-                       rho, Fe);
+  switch (kElemChoices[c]) {
+    case 32: launch_spk<T, NQ1, 32>(ctx, a, IF_d, n_u, rho, Fe); break;
+    case 16: launch_spk<T, NQ1, 16>(ctx, a, IF_d, n_u, rho, Fe); break;
+    default: launch_spk<T, NQ1, 64>(ctx, a, IF_d, n_u, rho, Fe); break;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -199,15 +213,17 @@ hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, cons
     gp[1] = 0.0;
     gp[2] = std::sqrt(3. / 5.);
   }
+  auto run = [&](int c) {
+    if (nqp_1d == 2)
+      return precision == GGEM_FP64 ? launch_sp<double, 2>(ctx, c, gp, IF_d, n_u, rho, Fe)
+                                    : launch_sp<float, 2>(ctx, c, gp, IF_d, n_u, rho, Fe);
+    return precision == GGEM_FP64 ? launch_sp<double, 3>(ctx, c, gp, IF_d, n_u, rho, Fe)
+                                  : launch_sp<float, 3>(ctx, c, gp, IF_d, n_u, rho, Fe);
+  };
+  // rho and Fe are outputs only: idempotent
+  const int c = tune_pick(ctx, tune_key("spread", precision, nqp_1d, ctx->grid.n_elem), kNElem, run);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "spread_fp64" : "spread_fp32c");
-  if (nqp_1d == 2) {
-    if (precision == GGEM_FP64) launch_sp<double, 2>(ctx, gp, IF_d, n_u, rho, Fe);
-    else launch_sp<float, 2>(ctx, gp, IF_d, n_u, rho, Fe);
-  } else {
-    if (precision == GGEM_FP64) launch_sp<double, 3>(ctx, gp, IF_d, n_u, rho, Fe);
-    else launch_sp<float, 3>(ctx, gp, IF_d, n_u, rho, Fe);
-  }
-  return hipGetLastError();
+  return run(c);
 }
 
 }  // namespace ggem

@@ -68,10 +68,11 @@ def make_inputs(name, n=None, **kw):
     return cfg, x, f
 
 
-def polymer_chains(cfg, n_chains, n_beads, seed, bond=0.97, excl=0.9):
+def polymer_chains(cfg, n_chains, n_beads, seed, bond=0.97, excl=0.9, wall_margin=1.2):
     """Random-walk chains (bond length `bond`), self-avoiding at `excl`
-    against all earlier beads, confined to the slit by rejection; chain
-    starts uniform in the box.  numpy Generator(PCG64(seed))."""
+    against all earlier beads, kept `wall_margin` away from the walls
+    (outside the WCA range 2^(1/6) of the slit-wall force) by rejection;
+    chain starts uniform in the box.  numpy Generator(PCG64(seed))."""
     rng = np.random.default_rng(seed)
     lo = np.array(cfg["box_min"], float)
     hi = np.array(cfg["box_max"], float)
@@ -87,7 +88,7 @@ def polymer_chains(cfg, n_chains, n_beads, seed, bond=0.97, excl=0.9):
         return tuple(c)
 
     def ok(p):
-        if np.any(~per & ((p < lo + 0.5) | (p > hi - 0.5))):
+        if np.any(~per & ((p < lo + wall_margin) | (p > hi - wall_margin))):
             return False
         c = np.array(key(p))
         for dk in (-1, 0, 1):

@@ -267,6 +267,19 @@ int ggem_util_normal(int64_t n, uint64_t seed, double* out);
  * IF[l*nqp + qp] = JxW[qp] * phi_l(qp), libMesh HEX20 node order. */
 int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*nqp */);
 
+/* ---- launch-shape autotuning -------------------------------------------- */
+/* enable != 0: the first launch of the pair-tensor, pair-sum and mobility
+ * kernels for a new signature (precision, operand count, log2 of the
+ * particle count) times each candidate launch shape (block size, XCD block
+ * renumbering) and keeps the fastest for the context; all shapes give
+ * bit-identical results.  Off (the default): the default shape.  No
+ * counterpart in the reference (CPU loops have no launch shape). */
+int ggem_set_autotune(ggem_ctx* ctx, int32_t enable);
+/* The picks so far, one line per signature: "<signature> <pick> <ms per
+ * candidate>...".  buf (cap bytes, NUL-terminated) may be NULL; *len gets
+ * the full length. */
+int ggem_tuning_report(ggem_ctx* ctx, char* buf, int64_t cap, int64_t* len);
+
 /* ---- profiling ---------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
  * context's stream.  ggem_profile_get sums the durations per kernel name. */

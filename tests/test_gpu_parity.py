@@ -289,7 +289,8 @@ def test_cheb_graph_replays_match_eager(c2, prec):
         assert np.array_equal(y, eager[0][0]) and np.array_equal(d, eager[0][1])
         # moved particles: new lists, new capture, against the oracle
         rng = np.random.default_rng(5)
-        x2 = x + rng.normal(scale=0.05, size=x.shape)
+        L = np.array(cfg["box_max"], float)
+        x2 = np.mod(x + rng.normal(scale=0.05, size=x.shape), L)  # periodic box [0, L)
         ctx.reinit(x2, f)
         y2, d2, _ = ctx.cheb_brownian(12, 0.05, 3.0, mode=G.CHEB_FIXED)
         ctx.set_graph(False)
