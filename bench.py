@@ -49,6 +49,29 @@ def parse():
     return ap.parse_args()
 
 
+# Algorithmic HBM bytes of one FP32C Chebyshev application (k_apply, one
+# right-hand side): per ordered pair the 4-byte list entry and the 16-byte
+# pair tensor; per slot the operand record read once (16), x, x_{k-1} (24
+# each), y read and written (48), v and its record written (24 + 16), the
+# list length (4).
+B_PAIR_APPLY32 = 4 + 16
+B_SLOT_APPLY = 16 + 24 + 24 + 48 + 24 + 16 + 4
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_beads1m.txt")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH + WRITE, calibrated) from
+    the committed rocprofv3 summary of the same workload, or None."""
+    try:
+        with open(PROFILE_TRAFFIC) as fh:
+            for line in fh:
+                if line.startswith(kernel):
+                    return float(line.split()[-1]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -211,7 +234,8 @@ def main():
     # pair evaluations per step: the local velocity plus the order-N recurrence
     evals = 1 + order
     value = evals * pairs / (r32["ms"] * 1e-3)
-    achieved = F_PAIR * pairs / apply_s
+    bytes_apply = B_PAIR_APPLY32 * pairs + B_SLOT_APPLY * n
+    achieved = bytes_apply / apply_s
     out = {
         "metric": "pair-interactions/s + Brownian steps/s, FP32-comp vs FP64 @ <=1e-6 rel err",
         "value": value,
@@ -232,10 +256,13 @@ def main():
         "pair_evaluations_per_step": evals,
         "pair_kernel_interactions_per_s": pairs / apply_s,
         "graph": {"ms_per_step": r32["ms_graph"], "brownian_steps_per_s": 1e3 / r32["ms_graph"]},
-        "roofline": {"bound": "valu", "kernel": "apply_fp32c (pair sum, mobility apply)",
-                     "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None,
-                     "algorithmic": "%d FLOP x %d pairs per launch" % (F_PAIR, pairs),
+        "roofline": {"bound": "hbm", "kernel": "k_apply FP32C Chebyshev step (pair sum + recurrence update)",
+                     "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM,
+                     "traffic": profiled_traffic("k_apply<1, 1, 2, 256>") if args.workload == "beads1m" else None,
+                     "algorithmic": "%d B x %d pairs + %d B x %d slots per launch"
+                                    % (B_PAIR_APPLY32, pairs, B_SLOT_APPLY, n),
+                     "valu_tflops": F_PAIR * pairs / apply_s / 1e12,
                      "avg_launch_ms": r32["apply_avg_ms"], "launches": r32["apply_launches"]},
         "kernels_ms": {"apply_fp32c": r32["apply_avg_ms"], "spread_fp32c": r32["spread_avg_ms"],
                        "pair_beads_fp32c": r32["local_avg_ms"]},
