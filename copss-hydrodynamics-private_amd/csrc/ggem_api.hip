@@ -172,6 +172,10 @@ int derive(const ggem_params* p, ggem_consts* c, std::string* why) {
     *why = "element count exceeds int32";
     return GGEM_ERR_CAPACITY;
   }
+  if (p->n_elem[0] > (1 << kCellBitsX) || p->n_elem[1] > (1 << kCellBitsY) || p->n_elem[2] > (1 << kCellBitsZ)) {
+    *why = "more than 2048 x 2048 x 1024 elements per axis";
+    return GGEM_ERR_CAPACITY;
+  }
   if (!(p->mu > 0) || !(p->alpha > 0)) {
     *why = "mu and alpha must be positive";
     return GGEM_ERR_INVALID_ARG;
@@ -306,6 +310,7 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->elist_off);
   f(ctx->elist);
   f(ctx->coef32);
+  f(ctx->pc32);
   f(ctx->coef64);
   f(ctx->noise);
   f(ctx->dpar);
@@ -619,6 +624,8 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     f_(ctx->nbr_cnt);
     f_(ctx->slice_len);
     f_(ctx->slice_off);
+    f_(ctx->pc32);
+    ctx->pc32 = nullptr;
     ctx->x_in = ctx->f_in = ctx->xs = ctx->fs = ctx->u_loc = ctx->u_out = nullptr;
     ctx->elem = ctx->perm = ctx->slot_of = ctx->slot_cell = ctx->nbr_cnt = nullptr;
     ctx->x4 = nullptr;
@@ -637,6 +644,7 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     GGEM_HIP(ctx, hipMalloc(&ctx->slot_cell, sizeof(int32_t) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->xs, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->x4, sizeof(double4) * n));
+    GGEM_HIP(ctx, hipMalloc(&ctx->pc32, sizeof(float4) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs32, sizeof(float) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->fg32, sizeof(float4) * n));

@@ -57,6 +57,8 @@ constexpr uint32_t kSlotBits = 26;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 constexpr int64_t kMaxParticles = int64_t(1) << kSlotBits;
 constexpr int kSlice = 64;  // targets per neighbour-list slice (one wave)
+// packed cell coordinates of the FP32C position records: i | j << 11 | k << 22
+constexpr int kCellBitsX = 11, kCellBitsY = 11, kCellBitsZ = 10;
 
 struct Profiler {
   struct Rec {
@@ -140,11 +142,13 @@ struct ggem_ctx {
   uint32_t* ell = nullptr;
   int64_t ell_cap = 0;
   int64_t ell_len = 0;
-  // pair tensors of the current lists, same layout as ell:
-  // G = A I + sgn w w^T stored as (A, w) with w = sqrt(|Bx|) x_ij
-  float4* coef32 = nullptr;
+  // pair tensors of the current lists, same layout as ell, G = A I + Bx x x^T:
+  // FP32C keeps (A, Bx) and forms x_ij from the cell-relative positions pc32;
+  // FP64 keeps (A, w) with w = sqrt(|Bx|) x_ij and Bx = sgn |Bx|
+  float2* coef32 = nullptr;
   int64_t coef32_cap = 0;
   bool coef32_valid = false;
+  float4* pc32 = nullptr;     // [n] by slot: FP32 position relative to its cell centre, packed cell (i, j, k)
   double4* coef64 = nullptr;
   int64_t coef64_cap = 0;
   bool coef64_valid = false;

@@ -103,6 +103,27 @@ __global__ void k_slot_sys(int64_t n, const int32_t* __restrict__ perm, const in
   if (s < n) sys_slot[s] = sys_part[perm[s]];
 }
 
+// FP32C position records: each particle's FP64 offset from its cell centre
+// rounded to FP32, the cell's (i, j, k) packed into the fourth word.  A
+// pair displacement is then (cell difference) h + (offset difference), all
+// terms small, in FP32.
+template <bool BATCH>
+__global__ void k_cell_rel(Grid g, const SysGrid* __restrict__ sys, const int32_t* __restrict__ sys_slot,
+                           int64_t n, const int32_t* __restrict__ slot_cell, const double4* __restrict__ x4,
+                           float4* __restrict__ pc) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const SysView v = sys_view<BATCH>(g, nullptr, 0, nullptr, sys, sys_slot, s);
+  int ci, cj, ck;
+  cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
+  const double4 p = x4[s];
+  const double cx = v.g.bmin[0] + (double(ci) + 0.5) * v.g.h[0];
+  const double cy = v.g.bmin[1] + (double(cj) + 0.5) * v.g.h[1];
+  const double cz = v.g.bmin[2] + (double(ck) + 0.5) * v.g.h[2];
+  const uint32_t packed = uint32_t(ci) | (uint32_t(cj) << kCellBitsX) | (uint32_t(ck) << (kCellBitsX + kCellBitsY));
+  pc[s] = make_float4(float(p.x - cx), float(p.y - cy), float(p.z - cz), __uint_as_float(packed));
+}
+
 // system of each particle from the particle ranges (binary search)
 __global__ void k_part_sys(int64_t n, int nsys, const int64_t* __restrict__ part_off,
                            int32_t* __restrict__ sys_part) {
@@ -537,9 +558,16 @@ hipError_t launch_place(ggem_ctx* ctx) {
   hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
                      ctx->grid.n_elem, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
                      ctx->slot_cell, ctx->x_in, ctx->xs, ctx->x4);
-  if (ctx->nsys > 1 && ctx->n > 0)
+  if (ctx->n == 0) return hipGetLastError();
+  if (ctx->nsys > 1) {
     hipLaunchKernelGGL(k_slot_sys, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->perm,
                        ctx->sys_part, ctx->sys_slot);
+    hipLaunchKernelGGL(k_cell_rel<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       ctx->sys_d, ctx->sys_slot, ctx->n, ctx->slot_cell, ctx->x4, ctx->pc32);
+  } else {
+    hipLaunchKernelGGL(k_cell_rel<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
+                       nullptr, nullptr, ctx->n, ctx->slot_cell, ctx->x4, ctx->pc32);
+  }
   return hipGetLastError();
 }
 

@@ -51,7 +51,9 @@ struct PairArgs {
   double bsign;
   const SysGrid* sys;       // batched contexts: grid of each system
   const int32_t* sys_slot;  // and system of each slot
+  const float4* pc32;       // FP32C position records (cell-relative offsets, packed cell)
   int32_t xcd;              // renumber blocks per XCD (xcd_block)
+  int32_t exact;            // explicit minimum image (no list shift codes)
 };
 
 // Launch shapes the autotuner chooses from (index 0 is the default); every
@@ -89,7 +91,7 @@ struct V4<double> {
 
 // Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
 template <int P, bool EXACT, bool BATCH, int BLK>
-__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float4* __restrict__ c32,
+__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float2* __restrict__ c32,
                                                 double4* __restrict__ c64) {
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
@@ -120,8 +122,7 @@ __global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float4* __restrict_
       const float fx = float(dx), fy = float(dy), fz = float(dz);
       float A, Bx;
       green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
-      const float sb = __builtin_sqrtf(fabsf(Bx));
-      c32[o] = make_float4(A, sb * fx, sb * fy, sb * fz);
+      c32[o] = make_float2(A, Bx);
     }
   }
 }
@@ -135,44 +136,126 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
                                                  const double* __restrict__ xsoa, SlotVec v,
                                                  ChebEpi epi, const double* __restrict__ u_glob,
                                                  double* __restrict__ out) {
-  using T = typename std::conditional<P == GGEM_FP64, double, float>::type;
-  using T4 = typename V4<T>::type;
   // (the tensor pointer is untyped in the signature: a type computed from
   // the header's unnamed precision enum would be mangled differently by the
   // host and device compilations)
-  const T4* __restrict__ coef = static_cast<const T4*>(coef_raw);
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
   const int64_t n = a.n;
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
   const uint32_t* lst = a.ell + base;
-  const T4* cf = coef + base;
-  const T4* xg;
-  if constexpr (P == GGEM_FP64) xg = xg64;
-  else xg = xg32;
-  const T sg = T(a.bsign);
   double acc[NR][3];
-  T ps[NR][3];
-  KahanF kh[P == GGEM_FP64 ? 1 : NR][3];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) ps[r][0] = ps[r][1] = ps[r][2] = T(0);
-  // entry one ahead so the operand gather of the next pair is in flight
+  // Groups of U entries: the entries of the next group are loaded one group
+  // ahead, and the U tensors and operand records of a group are all
+  // requested before the first is used (memory-level parallelism for the
+  // dependent gathers).  The summation order is the list order.
+  constexpr int U = NR >= 4 ? 1 : (NR == 2 ? 2 : 4);
   const int last = cnt > 0 ? cnt - 1 : 0;
-  uint32_t e1 = cnt > 0 ? lst[0] : 0u;
-  for (int k = 0; k < cnt; ++k) {
-    const uint32_t t = e1 & kSlotMask;
-    e1 = lst[uint32_t(min(k + 1, last)) * uint32_t(kSlice)];
-    const T4 c = cf[int64_t(k) * kSlice];
+  uint32_t en[U];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const T4 f = xg[r * n + t];
-      const T wf = sg * (c.y * f.x + c.z * f.y + c.w * f.z);
-      ps[r][0] += c.x * f.x + wf * c.y;
-      ps[r][1] += c.x * f.y + wf * c.z;
-      ps[r][2] += c.x * f.z + wf * c.w;
+  for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? lst[uint32_t(min(i, last)) * uint32_t(kSlice)] : 0u;
+
+  if constexpr (P == GGEM_FP64) {
+    const double4* __restrict__ cf = static_cast<const double4*>(coef_raw) + base;
+    const double4* __restrict__ xg = xg64;
+    const double sg = a.bsign;
+    double ps[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) ps[r][0] = ps[r][1] = ps[r][2] = 0.0;
+    // G f = A f + sgn (w . f) w with the stored (A, w)
+    auto pair_acc = [&](const double4& c, const double4* f) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double wf = sg * (c.y * f[r].x + c.z * f[r].y + c.w * f[r].z);
+        ps[r][0] += c.x * f[r].x + wf * c.y;
+        ps[r][1] += c.x * f[r].y + wf * c.z;
+        ps[r][2] += c.x * f[r].z + wf * c.w;
+      }
+    };
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+      uint32_t e[U];
+      double4 c[U];
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        e[i] = en[i];
+        en[i] = lst[uint32_t(min(k + U + i, last)) * uint32_t(kSlice)];
+        c[i] = cf[int64_t(k + i) * kSlice];
+      }
+      double4 f[U][NR];
+#pragma unroll
+      for (int i = 0; i < U; ++i)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + (e[i] & kSlotMask)];
+#pragma unroll
+      for (int i = 0; i < U; ++i) pair_acc(c[i], f[i]);
     }
-    if constexpr (P != GGEM_FP64) {
+#pragma unroll
+    for (int i = 0; i < U - 1; ++i) {
+      if (k + i < cnt) {  // entries en[0 .. cnt - k)
+        double4 f[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) f[r] = xg[r * n + (en[i] & kSlotMask)];
+        pair_acc(cf[int64_t(k + i) * kSlice], f);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[r][q] = ps[r][q];
+  } else {
+    // FP32C: G f = A f + Bx (x . f) x with the stored (A, Bx) and x_ij from
+    // the cell-relative FP32 positions: x_ij = (cell_t - cell_s + shift) h
+    // + (offset_t - offset_s), per axis
+    const float2* __restrict__ cf = static_cast<const float2*>(coef_raw) + base;
+    const float4* __restrict__ xg = xg32;
+    const float4* __restrict__ pc = a.pc32;
+    const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
+    const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const float Lx = float(g.L[0]), Ly = float(g.L[1]), Lz = float(g.L[2]);
+    const float4 po = pc[s];
+    const uint32_t co = __float_as_uint(po.w);
+    const int io = int(co & ((1u << kCellBitsX) - 1));
+    const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+    const int ko = int(co >> (kCellBitsX + kCellBitsY));
+    float ps[NR][3];
+    KahanF kh[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) ps[r][0] = ps[r][1] = ps[r][2] = 0.f;
+    auto disp = [&](uint32_t e, const float4& pt, float& dx, float& dy, float& dz) {
+      const uint32_t ct = __float_as_uint(pt.w);
+      int di = int(ct & ((1u << kCellBitsX) - 1)) - io;
+      int dj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1)) - jo;
+      int dk = int(ct >> (kCellBitsX + kCellBitsY)) - ko;
+      const uint32_t code = e >> kSlotBits;
+      if (!a.exact) {  // the entry's single periodic shift, in cells
+        di += (int(code & 1u) - int((code >> 1) & 1u)) * nx;
+        dj += (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * ny;
+        dk += (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * nz;
+      }
+      dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+      dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+      dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+      if (a.exact) {  // minimum image (pm_periodic_boundary.C:350-374)
+        if (g.periodic[0]) dx = dx <= -0.5f * Lx ? dx + Lx : (dx > 0.5f * Lx ? dx - Lx : dx);
+        if (g.periodic[1]) dy = dy <= -0.5f * Ly ? dy + Ly : (dy > 0.5f * Ly ? dy - Ly : dy);
+        if (g.periodic[2]) dz = dz <= -0.5f * Lz ? dz + Lz : (dz > 0.5f * Lz ? dz - Lz : dz);
+      }
+    };
+    auto pair_acc = [&](uint32_t e, const float2& c, const float4& pt, const float4* f) {
+      float dx, dy, dz;
+      disp(e, pt, dx, dy, dz);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const float b = c.y * (dx * f[r].x + dy * f[r].y + dz * f[r].z);
+        ps[r][0] += c.x * f[r].x + b * dx;
+        ps[r][1] += c.x * f[r].y + b * dy;
+        ps[r][2] += c.x * f[r].z + b * dz;
+      }
+    };
+    auto merge = [&](int k) {
       if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -182,19 +265,50 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
             ps[r][q] = 0.f;
           }
       }
+    };
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+      uint32_t e[U];
+      float2 c[U];
+      float4 pt[U];
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        e[i] = en[i];
+        en[i] = lst[uint32_t(min(k + U + i, last)) * uint32_t(kSlice)];
+        c[i] = cf[int64_t(k + i) * kSlice];
+      }
+      float4 f[U][NR];
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        pt[i] = pc[e[i] & kSlotMask];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + (e[i] & kSlotMask)];
+      }
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        pair_acc(e[i], c[i], pt[i], f[i]);
+        merge(k + i);
+      }
     }
-  }
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
+    for (int i = 0; i < U - 1; ++i) {
+      if (k + i < cnt) {  // entries en[0 .. cnt - k)
+        const uint32_t t = en[i] & kSlotMask;
+        float4 f[NR];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if constexpr (P == GGEM_FP64) {
-        acc[r][q] = ps[r][q];
-      } else {
+        for (int r = 0; r < NR; ++r) f[r] = xg[r * n + t];
+        pair_acc(en[i], cf[int64_t(k + i) * kSlice], pc[t], f);
+        merge(k + i);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
         kh[r][q].add(ps[r][q]);
         acc[r][q] = kh[r][q].value();
       }
-    }
+  }
 
   if (EPI == EPI_LOCAL || EPI == EPI_MOBILITY) {
     // forces are the operand: own force from the FP64 store
@@ -332,7 +446,9 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.bsign = double(ctx->bsign);
   a.sys = ctx->sys_d;
   a.sys_slot = ctx->sys_slot;
+  a.pc32 = ctx->pc32;
   a.xcd = 1;
+  a.exact = ctx->exact_minimg;
   return a;
 }
 
