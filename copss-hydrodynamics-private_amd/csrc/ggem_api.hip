@@ -298,6 +298,7 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->u_loc);
   f(ctx->u_out);
   f(ctx->cell_count);
+  f(ctx->pcell);
   f(ctx->cell_start);
   f(ctx->scan_tmp);
   f(ctx->flag);
@@ -397,7 +398,7 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
   std::vector<StencilRow> rows;
   std::string why;
   int exact = 0;
-  int64_t ne_total = 0;
+  int64_t ne_total = 0, nc_total = 0;
   int32_t ncells0 = 0;
   for (int32_t k = 0; k < nsys; ++k) {
     const ggem_params* pk = params + k;
@@ -428,13 +429,18 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
       if (g.periodic[d] && 2 * (reach[d] + 1) > pk->n_elem[d]) exact = 1;
     }
     g.n_elem = int64_t(pk->n_elem[0]) * pk->n_elem[1] * pk->n_elem[2];
+    for (int d = 0; d < 3; ++d) g.nb[d] = (pk->n_elem[d] + (1 << kBrickBits) - 1) >> kBrickBits;
+    g.n_cells = (int64_t(g.nb[0]) * g.nb[1] * g.nb[2]) << (3 * kBrickBits);
     sg[k].elem_off = ne_total;
+    sg[k].cell_off = nc_total;
     sg[k].row_off = int32_t(rows.size());
     sg[k].nrows = int32_t(rk.size());
     ne_total += g.n_elem;
+    nc_total += g.n_cells;
     rows.insert(rows.end(), rk.begin(), rk.end());
   }
-  if (ne_total >= INT32_MAX) return fail(nullptr, GGEM_ERR_CAPACITY, "element count exceeds int32");
+  if (ne_total >= INT32_MAX || nc_total >= INT32_MAX)
+    return fail(nullptr, GGEM_ERR_CAPACITY, "element count exceeds int32");
   const ggem_consts& c = cs[0];
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -456,6 +462,7 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
   // the whole store is one element range; per-system views come from sys_h
   ctx->grid = sg[0].g;
   ctx->grid.n_elem = ne_total;
+  ctx->grid.n_cells = nc_total;
   ctx->gc.eps = c.one_eight_pi_mu;
   ctx->gc.alpha = c.alpha;
   ctx->gc.alpha2 = c.alpha2;
@@ -482,11 +489,11 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
   if (e != hipSuccess) return bail(e, "hipSetDevice");
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return bail(e, "hipStreamCreate");
-  const int64_t ne = ctx->grid.n_elem;
-  const int64_t ntiles = (ne + 2047) / 2048;
+  const int64_t ne = ctx->grid.n_elem, nc = ctx->grid.n_cells;
+  const int64_t ntiles = (nc + 2047) / 2048;
   if ((e = hipMalloc(&ctx->rows_d, sizeof(StencilRow) * std::max<size_t>(rows.size(), 1))) != hipSuccess ||
-      (e = hipMalloc(&ctx->cell_count, sizeof(int32_t) * ne)) != hipSuccess ||
-      (e = hipMalloc(&ctx->cell_start, sizeof(int32_t) * (ne + 1))) != hipSuccess ||
+      (e = hipMalloc(&ctx->cell_count, sizeof(int32_t) * nc)) != hipSuccess ||
+      (e = hipMalloc(&ctx->cell_start, sizeof(int32_t) * (nc + 1))) != hipSuccess ||
       (e = hipMalloc(&ctx->scan_tmp, sizeof(int32_t) * (ntiles + 1))) != hipSuccess ||
       (e = hipMalloc(&ctx->flag, sizeof(int32_t))) != hipSuccess ||
       (e = hipMalloc(&ctx->pairs_d, sizeof(int64_t))) != hipSuccess ||
@@ -625,7 +632,9 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     f_(ctx->slice_len);
     f_(ctx->slice_off);
     f_(ctx->pc32);
+    f_(ctx->pcell);
     ctx->pc32 = nullptr;
+    ctx->pcell = nullptr;
     ctx->x_in = ctx->f_in = ctx->xs = ctx->fs = ctx->u_loc = ctx->u_out = nullptr;
     ctx->elem = ctx->perm = ctx->slot_of = ctx->slot_cell = ctx->nbr_cnt = nullptr;
     ctx->x4 = nullptr;
@@ -645,6 +654,7 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     GGEM_HIP(ctx, hipMalloc(&ctx->xs, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->x4, sizeof(double4) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->pc32, sizeof(float4) * n));
+    GGEM_HIP(ctx, hipMalloc(&ctx->pcell, sizeof(int32_t) * n));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs, sizeof(double) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->fs32, sizeof(float) * n3));
     GGEM_HIP(ctx, hipMalloc(&ctx->fg32, sizeof(float4) * n));

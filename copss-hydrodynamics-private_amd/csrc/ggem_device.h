@@ -25,12 +25,34 @@ __device__ __forceinline__ double apply_shift(double d, uint32_t code2, double L
   return w ? __builtin_fma(double(w), L, d) : d;
 }
 
-__device__ __forceinline__ void cell_ijk(const Grid& g, int64_t e, int& i, int& j, int& k) {
+// Reference element id e = i + nx (j + ny k) -> (i, j, k).
+__device__ __forceinline__ void elem_ijk(const Grid& g, int64_t e, int& i, int& j, int& k) {
   const int64_t nx = g.n[0], nxy = int64_t(g.n[0]) * g.n[1];
   k = int(e / nxy);
   const int64_t r = e - int64_t(k) * nxy;
   j = int(r / nx);
   i = int(r - int64_t(j) * nx);
+}
+
+// Brick-order cell index of cell (i, j, k) and its inverse.
+__device__ __forceinline__ int64_t cell_index(const Grid& g, int i, int j, int k) {
+  constexpr int B = kBrickBits, M = (1 << kBrickBits) - 1;
+  const int64_t brick = (i >> B) + int64_t(g.nb[0]) * ((j >> B) + int64_t(g.nb[1]) * (k >> B));
+  return (brick << (3 * B)) | ((k & M) << (2 * B)) | ((j & M) << B) | (i & M);
+}
+
+__device__ __forceinline__ void cell_ijk(const Grid& g, int64_t c, int& i, int& j, int& k) {
+  constexpr int B = kBrickBits, M = (1 << kBrickBits) - 1;
+  const int64_t brick = c >> (3 * B);
+  const int l = int(c & ((1 << (3 * B)) - 1));
+  const int64_t bxy = int64_t(g.nb[0]) * g.nb[1];
+  const int bk = int(brick / bxy);
+  const int64_t r = brick - int64_t(bk) * bxy;
+  const int bj = int(r / g.nb[0]);
+  const int bi = int(r - int64_t(bj) * g.nb[0]);
+  i = (bi << B) | (l & M);
+  j = (bj << B) | ((l >> B) & M);
+  k = (bk << B) | (l >> (2 * B));
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
@@ -44,8 +66,8 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
 
 // Walk the particle slots of every element of the stencil of element
 // (ci, cj, ck): calls seg(a, b, code) for each contiguous slot range [a, b)
-// with the periodic shift code of that range.  Rows are wave-uniform, so
-// their loads are scalar.
+// with the periodic shift code of that range (a row's x-run is one range
+// per brick it crosses), rows in stencil order, x ascending within a row.
 template <class Seg>
 __device__ __forceinline__ void for_each_segment(const Grid& g, const StencilRow* __restrict__ rows,
                                                  int nrows, const int32_t* __restrict__ cell_start,
@@ -78,22 +100,31 @@ __device__ __forceinline__ void for_each_segment(const Grid& g, const StencilRow
       }
     }
     const uint32_t cyz = (cy << 2) | (cz << 4);
-    const int64_t base = int64_t(nx) * (jj + int64_t(ny) * kk);
+    // x-run [x0, x1] of row (jj, kk): contiguous slots within each brick
+    auto run = [&](int x0, int x1, uint32_t code) {
+      constexpr int M = (1 << kBrickBits) - 1;
+      for (int p = x0; p <= x1;) {
+        const int q = min(x1, p | M);
+        const int64_t c0 = cell_index(g, p, jj, kk);
+        seg(cell_start[c0], cell_start[c0 + (q - p) + 1], code);
+        p = q + 1;
+      }
+    };
     int lo = ci + R.di_lo, hi = ci + R.di_hi;
     if (g.periodic[0]) {
       if (lo < 0) {
-        seg(cell_start[base + lo + nx], cell_start[base + nx], cyz | 2u);
+        run(lo + nx, nx - 1, cyz | 2u);
         lo = 0;
       }
       if (hi >= nx) {
-        seg(cell_start[base], cell_start[base + hi - nx + 1], cyz | 1u);
+        run(0, hi - nx, cyz | 1u);
         hi = nx - 1;
       }
     } else {
       lo = lo < 0 ? 0 : lo;
       hi = hi >= nx ? nx - 1 : hi;
     }
-    if (lo <= hi) seg(cell_start[base + lo], cell_start[base + hi + 1], cyz);
+    if (lo <= hi) run(lo, hi, cyz);
   }
 }
 

@@ -23,7 +23,11 @@ struct StencilRow {
   int32_t dj, dk, di_lo, di_hi;
 };
 
-// Grid and GGEM constants passed by value to every kernel.
+// Grid and GGEM constants passed by value to every kernel.  Elements keep
+// the reference's id e = i + nx (j + ny k) at the interface; inside, the
+// cell-sorted particle store numbers cells brick by brick (8 x 8 x 8 cells
+// per brick, bricks and the cells of a brick in x-major order) so that the
+// particles of a compact 3D region occupy one contiguous slot range.
 struct Grid {
   double bmin[3];
   double L[3];
@@ -31,8 +35,11 @@ struct Grid {
   double half_L[3];
   int32_t n[3];
   int32_t periodic[3];
-  int64_t n_elem;
+  int32_t nb[3];    // bricks per axis, ceil(n / 8)
+  int64_t n_elem;   // n[0] n[1] n[2]
+  int64_t n_cells;  // brick-padded cell index range, 512 nb[0] nb[1] nb[2]
 };
+constexpr int kBrickBits = 3;  // 8 cells per brick edge
 
 struct GGEMConst {
   double eps;  // 1/(8 pi mu)
@@ -46,7 +53,8 @@ struct GGEMConst {
 // system, so the slots of a system form one contiguous range.
 struct SysGrid {
   Grid g;
-  int64_t elem_off;
+  int64_t elem_off;  // first element id of the system
+  int64_t cell_off;  // first brick-order cell index of the system
   int32_t row_off, nrows;
 };
 
@@ -123,8 +131,9 @@ struct ggem_ctx {
   double4* fg64 = nullptr;    // [n] forces by slot, gather copy (FP64)
   double* u_loc = nullptr;    // [3n] interleaved, particle order
   double* u_out = nullptr;    // [3n] interleaved, particle order
-  int32_t* cell_count = nullptr;  // [n_elem]
-  int32_t* cell_start = nullptr;  // [n_elem + 1]
+  int32_t* pcell = nullptr;       // [n] brick-order cell of each particle
+  int32_t* cell_count = nullptr;  // [n_cells]
+  int32_t* cell_start = nullptr;  // [n_cells + 1], by brick-order cell index
   int32_t* scan_tmp = nullptr;
   int64_t scan_tmp_len = 0;
   int32_t* flag = nullptr;        // device error flag

@@ -33,12 +33,19 @@ __device__ __forceinline__ int64_t locate_dev(const Grid& g, double x, double y,
   return c[0] + int64_t(g.n[0]) * (c[1] + int64_t(g.n[1]) * c[2]);
 }
 
+// brick-order cell index of reference element e
+__device__ __forceinline__ int64_t elem_cell(const Grid& g, int64_t e) {
+  int i, j, k;
+  elem_ijk(g, e, i, j, k);
+  return cell_index(g, i, j, k);
+}
+
 // Elem::contains_point of an axis-aligned hex with libMesh's 1e-6 tolerance
 // on the reference element.
 __device__ __forceinline__ bool contains_dev(const Grid& g, int64_t e, double x, double y,
                                              double z) {
   int ci, cj, ck;
-  cell_ijk(g, e, ci, cj, ck);
+  elem_ijk(g, e, ci, cj, ck);
   const double v[3] = {x, y, z};
   const int ijk[3] = {ci, cj, ck};
 #pragma unroll
@@ -57,8 +64,9 @@ struct SysView {
   Grid g;
   const StencilRow* rows;
   int nrows;
-  const int32_t* cs;
-  int64_t eoff;
+  const int32_t* cs;  // cell starts from the system's first cell
+  int64_t eoff;       // first element id of the system
+  int64_t coff;       // first cell index of the system
 };
 
 template <bool BATCH>
@@ -67,16 +75,16 @@ __device__ __forceinline__ SysView sys_view(const Grid& g, const StencilRow* row
                                             int64_t i) {
   if constexpr (BATCH) {
     const SysGrid& S = sys[sys_of[i]];
-    return SysView{S.g, rows + S.row_off, S.nrows, cs + S.elem_off, S.elem_off};
+    return SysView{S.g, rows + S.row_off, S.nrows, cs + S.cell_off, S.elem_off, S.cell_off};
   } else {
-    return SysView{g, rows, nrows, cs, 0};
+    return SysView{g, rows, nrows, cs, 0, 0};
   }
 }
 
 template <bool BATCH>
 __global__ void k_bin(Grid g, const SysGrid* __restrict__ sys, const int32_t* __restrict__ sys_part,
                       int64_t n, const double* __restrict__ x, int32_t* __restrict__ elem, int use_prev,
-                      int32_t* __restrict__ count, int32_t* __restrict__ flag) {
+                      int32_t* __restrict__ pcell, int32_t* __restrict__ count, int32_t* __restrict__ flag) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SysView v = sys_view<BATCH>(g, nullptr, 0, nullptr, sys, sys_part, i);
@@ -91,9 +99,10 @@ __global__ void k_bin(Grid g, const SysGrid* __restrict__ sys, const int32_t* __
     atomicOr(flag, 1);
     e = 0;  // keep later kernels in bounds; the host reports the error
   }
-  e += v.eoff;
-  elem[i] = int32_t(e);
-  atomicAdd(&count[e], 1);
+  const int64_t c = v.coff + elem_cell(v.g, e);
+  elem[i] = int32_t(v.eoff + e);
+  pcell[i] = int32_t(c);
+  atomicAdd(&count[c], 1);
 }
 
 // system of each slot (batched contexts)
@@ -115,7 +124,7 @@ __global__ void k_cell_rel(Grid g, const SysGrid* __restrict__ sys, const int32_
   if (s >= n) return;
   const SysView v = sys_view<BATCH>(g, nullptr, 0, nullptr, sys, sys_slot, s);
   int ci, cj, ck;
-  cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
+  cell_ijk(v.g, slot_cell[s] - v.coff, ci, cj, ck);
   const double4 p = x4[s];
   const double cx = v.g.bmin[0] + (double(ci) + 0.5) * v.g.h[0];
   const double cy = v.g.bmin[1] + (double(cj) + 0.5) * v.g.h[1];
@@ -204,22 +213,23 @@ __global__ void k_scan_add(int32_t* __restrict__ out, int64_t len, const int32_t
 
 // Place every particle into its cell's slot range (arbitrary order within
 // the cell), then sort each cell by particle id and fill the slot arrays.
-__global__ void k_place(int64_t n, const int32_t* __restrict__ elem,
+__global__ void k_place(int64_t n, const int32_t* __restrict__ pcell,
                         const int32_t* __restrict__ cell_start, int32_t* __restrict__ cursor,
                         int32_t* __restrict__ perm) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int32_t e = elem[i];
-  const int32_t s = cell_start[e] + atomicAdd(&cursor[e], 1);
+  const int32_t c = pcell[i];
+  const int32_t s = cell_start[c] + atomicAdd(&cursor[c], 1);
   perm[s] = int32_t(i);
 }
 
-__global__ void k_sort_cells(int64_t n_elem, int64_t n, const int32_t* __restrict__ cell_start,
+// e runs over brick-order cell indices
+__global__ void k_sort_cells(int64_t n_cells, int64_t n, const int32_t* __restrict__ cell_start,
                              int32_t* __restrict__ perm, int32_t* __restrict__ slot_of,
                              int32_t* __restrict__ slot_cell, const double* __restrict__ x,
                              double* __restrict__ xs, double4* __restrict__ x4) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
+  if (e >= n_cells) return;
   const int32_t a = cell_start[e], b = cell_start[e + 1];
   for (int32_t s = a + 1; s < b; ++s) {  // insertion sort by particle id
     const int32_t v = perm[s];
@@ -276,7 +286,7 @@ __global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int n
   if (s < n) {
     const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
     int ci, cj, ck;
-    cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
+    cell_ijk(v.g, slot_cell[s] - v.coff, ci, cj, ck);
     for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck,
                      [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
     c -= 1;
@@ -304,7 +314,7 @@ __global__ void k_fill_ell(Grid g, const StencilRow* __restrict__ rows, int nrow
   if (s >= n) return;
   const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
   int ci, cj, ck;
-  cell_ijk(v.g, slot_cell[s] - v.eoff, ci, cj, ck);
+  cell_ijk(v.g, slot_cell[s] - v.coff, ci, cj, ck);
   uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
   int64_t k = 0;
   for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
@@ -324,7 +334,7 @@ __global__ void k_elist_count(Grid g, const StencilRow* __restrict__ rows, int n
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= g.n_elem) return;
   int ci, cj, ck;
-  cell_ijk(g, e, ci, cj, ck);
+  elem_ijk(g, e, ci, cj, ck);
   int64_t c = 0;
   for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
                    [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
@@ -349,7 +359,7 @@ __global__ void __launch_bounds__(256) k_elist_fill(Grid g, const StencilRow* __
   const int64_t e = e0 + threadIdx.x;
   if (e < g.n_elem) {
     int ci, cj, ck;
-    cell_ijk(g, e, ci, cj, ck);
+    elem_ijk(g, e, ci, cj, ck);
     const int64_t my = off[e] - base;
     int64_t k = 0;
     for_each_segment(g, rows, nrows, cell_start, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
@@ -453,14 +463,13 @@ __global__ void k_count_pairs(Grid g, const StencilRow* __restrict__ rows, int n
 
 // Reference order of a particle list: own element's particles (ascending id,
 // itself removed), then the other stencil elements by ascending element id,
-// each by ascending particle id.  Slots are sorted by (element, id), so the
-// second part is the ascending sequence of slots outside the own element.
-__device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows,
-                          const int32_t* cell_start, int64_t e, int32_t self_slot,
-                          const int32_t* perm, int32_t* out) {
-  int ci, cj, ck;
-  cell_ijk(g, e, ci, cj, ck);
-  const int32_t oa = cell_start[e], ob = cell_start[e + 1];
+// each by ascending particle id.  Slots within a cell are in particle-id
+// order; the tail (cells of the stencil other than the own) is sorted by
+// (element id, particle id).  c: brick-order index of cell (ci, cj, ck).
+__device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows, const int32_t* cell_start,
+                          int ci, int cj, int ck, int64_t c, int32_t self_slot, const int32_t* perm,
+                          const int32_t* elem, int32_t* out) {
+  const int32_t oa = cell_start[c], ob = cell_start[c + 1];
   int64_t w = 0;
   for (int32_t t = oa; t < ob; ++t)
     if (t != self_slot) out[w++] = t;
@@ -469,10 +478,15 @@ __device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows,
     for (int32_t t = a; t < b; ++t)
       if (t < oa || t >= ob) out[w++] = t;
   });
+  auto key = [&](int32_t t) {
+    const int32_t p = perm[t];
+    return (uint64_t(uint32_t(elem[p])) << 32) | uint32_t(p);
+  };
   for (int64_t q = head + 1; q < w; ++q) {  // insertion sort of the tail
     const int32_t v = out[q];
+    const uint64_t kv = key(v);
     int64_t t = q - 1;
-    while (t >= head && out[t] > v) {
+    while (t >= head && key(out[t]) > kv) {
       out[t + 1] = out[t];
       --t;
     }
@@ -484,29 +498,36 @@ __device__ void emit_list(const Grid& g, const StencilRow* rows, int nrows,
 __global__ void k_fill_csr(Grid g, const StencilRow* __restrict__ rows, int nrows,
                            const int32_t* __restrict__ cell_start,
                            const int32_t* __restrict__ slot_cell, const int32_t* __restrict__ perm,
-                           int64_t n, const int64_t* __restrict__ offsets, int32_t* __restrict__ ids) {
+                           const int32_t* __restrict__ elem, int64_t n, const int64_t* __restrict__ offsets,
+                           int32_t* __restrict__ ids) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int32_t p = perm[s];
-  emit_list(g, rows, nrows, cell_start, slot_cell[s], int32_t(s), perm, ids + offsets[p]);
+  const int64_t c = slot_cell[s];
+  int ci, cj, ck;
+  cell_ijk(g, c, ci, cj, ck);
+  emit_list(g, rows, nrows, cell_start, ci, cj, ck, c, int32_t(s), perm, elem, ids + offsets[p]);
 }
 
+// e runs over reference element ids
 __global__ void k_elem_point(Grid g, const StencilRow* __restrict__ rows, int nrows,
                              const int32_t* __restrict__ cell_start,
-                             const int32_t* __restrict__ perm, int64_t e0, int64_t ne,
-                             int64_t* __restrict__ offsets, int32_t* __restrict__ ids, int pass) {
+                             const int32_t* __restrict__ perm, const int32_t* __restrict__ elem,
+                             int64_t e0, int64_t ne, int64_t* __restrict__ offsets, int32_t* __restrict__ ids,
+                             int pass) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= ne) return;
   const int64_t e = e0 + q;
+  int ci, cj, ck;
+  elem_ijk(g, e, ci, cj, ck);
   if (pass == 0) {
-    int ci, cj, ck;
-    cell_ijk(g, e, ci, cj, ck);
     int64_t cnt = 0;
     for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
                      [&](int32_t a, int32_t b, uint32_t) { cnt += b - a; });
     offsets[q + 1] = cnt;
   } else {
-    emit_list(g, rows, nrows, cell_start, e, -1, perm, ids + offsets[q]);
+    emit_list(g, rows, nrows, cell_start, ci, cj, ck, cell_index(g, ci, cj, ck), -1, perm, elem,
+              ids + offsets[q]);
   }
 }
 
@@ -519,7 +540,7 @@ __global__ void k_locate(Grid g, int64_t m, const double* __restrict__ pts, int3
 }  // namespace
 
 hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis) {
-  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_elem, ctx->stream);
+  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_cells, ctx->stream);
   if (e != hipSuccess) return e;
   e = hipMemsetAsync(ctx->flag, 0, sizeof(int32_t), ctx->stream);
   if (e != hipSuccess) return e;
@@ -528,16 +549,16 @@ hipError_t launch_bin(ggem_ctx* ctx, bool hysteresis) {
   if (ctx->nsys > 1)
     hipLaunchKernelGGL(k_bin<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
                        ctx->sys_d, ctx->sys_part, ctx->n, ctx->x_in, ctx->elem, hysteresis ? 1 : 0,
-                       ctx->cell_count, ctx->flag);
+                       ctx->pcell, ctx->cell_count, ctx->flag);
   else
     hipLaunchKernelGGL(k_bin<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                       nullptr, nullptr, ctx->n, ctx->x_in, ctx->elem, hysteresis ? 1 : 0, ctx->cell_count,
-                       ctx->flag);
+                       nullptr, nullptr, ctx->n, ctx->x_in, ctx->elem, hysteresis ? 1 : 0, ctx->pcell,
+                       ctx->cell_count, ctx->flag);
   return hipGetLastError();
 }
 
 hipError_t launch_scan(ggem_ctx* ctx) {
-  const int64_t len = ctx->grid.n_elem;
+  const int64_t len = ctx->grid.n_cells;
   const int64_t ntiles = (len + kScanTile - 1) / kScanTile;
   ProfScope ps(ctx, "scan");
   hipLaunchKernelGGL(k_scan_tiles, dim3(unsigned(ntiles)), dim3(kScanThreads), 0, ctx->stream,
@@ -549,14 +570,14 @@ hipError_t launch_scan(ggem_ctx* ctx) {
 }
 
 hipError_t launch_place(ggem_ctx* ctx) {
-  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_elem, ctx->stream);
+  hipError_t e = hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->grid.n_cells, ctx->stream);
   if (e != hipSuccess) return e;
   ProfScope ps(ctx, "place_sort");
   if (ctx->n > 0)
     hipLaunchKernelGGL(k_place, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
-                       ctx->elem, ctx->cell_start, ctx->cell_count, ctx->perm);
-  hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
-                     ctx->grid.n_elem, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
+                       ctx->pcell, ctx->cell_start, ctx->cell_count, ctx->perm);
+  hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(ctx->grid.n_cells)), dim3(kBlock), 0, ctx->stream,
+                     ctx->grid.n_cells, ctx->n, ctx->cell_start, ctx->perm, ctx->slot_of,
                      ctx->slot_cell, ctx->x_in, ctx->xs, ctx->x4);
   if (ctx->n == 0) return hipGetLastError();
   if (ctx->nsys > 1) {
@@ -657,7 +678,7 @@ hipError_t launch_fill_csr(ggem_ctx* ctx, const int64_t* offsets_d, int32_t* ids
   if (ctx->n == 0) return hipSuccess;
   ProfScope ps(ctx, "fill_csr");
   hipLaunchKernelGGL(k_fill_csr, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->n,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->elem, ctx->n,
                      offsets_d, ids_d);
   return hipGetLastError();
 }
@@ -666,7 +687,7 @@ hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t*
                                  int32_t* ids_d, int pass) {
   if (ne == 0) return hipSuccess;
   hipLaunchKernelGGL(k_elem_point, dim3(blocks_for(ne)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->perm, e0, ne, offsets_d, ids_d,
+                     ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->perm, ctx->elem, e0, ne, offsets_d, ids_d,
                      pass);
   return hipGetLastError();
 }

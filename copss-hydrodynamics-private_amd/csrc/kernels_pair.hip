@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_points(PairArgs a, const Stenci
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= m) return;
   int ci, cj, ck;
-  cell_ijk(a.g, elem[q], ci, cj, ck);
+  elem_ijk(a.g, elem[q], ci, cj, ck);
   const double4 xq = make_double4(pts[3 * q], pts[3 * q + 1], pts[3 * q + 2], 0.0);
   double ux = 0, uy = 0, uz = 0;
   KahanF kx, ky, kz;

@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
   const bool active = e < a.g.n_elem;
   if (active) {
     int ci, cj, ck;
-    cell_ijk(a.g, e, ci, cj, ck);
+    elem_ijk(a.g, e, ci, cj, ck);
     const double4 cen = make_double4(a.g.bmin[0] + (double(ci) + 0.5) * a.g.h[0],
                                      a.g.bmin[1] + (double(cj) + 0.5) * a.g.h[1],
                                      a.g.bmin[2] + (double(ck) + 0.5) * a.g.h[2], 0.0);

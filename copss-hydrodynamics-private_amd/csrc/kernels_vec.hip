@@ -122,7 +122,7 @@ __global__ void k_seg_dot(const SysGrid* __restrict__ sys, const int32_t* __rest
                           const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
   __shared__ double sh[kBlock / 64];
   const SysGrid& S = sys[blockIdx.x];
-  const int64_t s0 = cell_start[S.elem_off], s1 = cell_start[S.elem_off + S.g.n_elem];
+  const int64_t s0 = cell_start[S.cell_off], s1 = cell_start[S.cell_off + S.g.n_cells];
   double acc = 0.0;
   for (int c = 0; c < 3; ++c)
     for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) acc += a[c * n + s] * b[c * n + s];
