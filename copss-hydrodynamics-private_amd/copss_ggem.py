@@ -45,7 +45,7 @@ EXPORTS = [
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
     "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
-    "ggem_set_autotune", "ggem_tuning_report", "ggem_util_reference_noise",
+    "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
     "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
 ]
@@ -122,6 +122,7 @@ def lib():
             "ggem_cheb_brownian": [P, P, I32, D, D, D, I32, I32, P, P, P, P],
             "ggem_set_graph": [P, I32],
             "ggem_set_autotune": [P, I32],
+            "ggem_tune_set": [P, C.c_char_p, I32],
             "ggem_tuning_report": [P, P, I64, P],
             "ggem_create_batch": [P, I32, C.POINTER(C.c_void_p)],
             "ggem_batch_set_particles": [P, P, P, P],
@@ -403,6 +404,10 @@ class Context:
     def set_autotune(self, enable=True):
         """Pick launch shapes by timing the candidates on first use."""
         self._c(lib().ggem_set_autotune(self._h, int(enable)))
+
+    def tune_set(self, signature, pick):
+        """Pin the launch-shape pick of one signature."""
+        self._c(lib().ggem_tune_set(self._h, signature.encode(), int(pick)))
 
     def tuning(self):
         """{signature: (pick, [ms per candidate])} of the autotuner."""

@@ -74,10 +74,10 @@ static void prof_collect(ggem_ctx* ctx) {
   ctx->prof.pending.clear();
 }
 
-int tune_lookup(const ggem_ctx* ctx, const std::string& key) {
-  if (!ctx->tune.on) return 0;
+int tune_lookup(const ggem_ctx* ctx, const std::string& key, int dflt) {
+  if (!ctx->tune.on) return dflt;
   auto it = ctx->tune.pick.find(key);
-  return it == ctx->tune.pick.end() ? 0 : it->second;
+  return it == ctx->tune.pick.end() ? dflt : it->second;
 }
 
 std::string tune_key(const char* kernel, int precision, int extra, int64_t n) {
@@ -87,19 +87,20 @@ std::string tune_key(const char* kernel, int precision, int extra, int64_t n) {
          std::to_string(lg);
 }
 
-int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch) {
+int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch,
+              int dflt) {
   Tuner& T = ctx->tune;
-  if (!T.on) return 0;
+  if (!T.on) return dflt;
   auto it = T.pick.find(key);
   if (it != T.pick.end()) return it->second;
-  if (ncand <= 1) return 0;
+  if (ncand <= 1) return dflt;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+  if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return dflt;
   hipEvent_t a = nullptr, b = nullptr;
-  if (hipEventCreate(&a) != hipSuccess) return 0;
+  if (hipEventCreate(&a) != hipSuccess) return dflt;
   if (hipEventCreate(&b) != hipSuccess) {
     (void)hipEventDestroy(a);
-    return 0;
+    return dflt;
   }
   const bool prof = ctx->prof.on;
   ctx->prof.on = false;
@@ -116,8 +117,8 @@ int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::funct
   ctx->prof.on = prof;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  int best = 0;
-  for (int c = 1; c < ncand; ++c)
+  int best = dflt;
+  for (int c = 0; c < ncand; ++c)
     if (ms[c] < ms[best]) best = c;
   T.pick[key] = best;
   T.ms[key] = ms;
@@ -435,6 +436,7 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
     sg[k].cell_off = nc_total;
     sg[k].row_off = int32_t(rows.size());
     sg[k].nrows = int32_t(rk.size());
+    for (int d = 0; d < 3; ++d) sg[k].reach[d] = reach[d];
     ne_total += g.n_elem;
     nc_total += g.n_cells;
     rows.insert(rows.end(), rk.begin(), rk.end());
@@ -477,6 +479,7 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
   ctx->bsign = c.ksi >= c.alpha ? 1 : -1;  // sign of (c1 - c2): that of xi - alpha
   ctx->rows_h = rows;
   ctx->n_rows = int32_t(rows.size());
+  for (int d = 0; d < 3; ++d) ctx->reach[d] = sg[0].reach[d];
 
   auto bail = [&](hipError_t e, const char* what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -932,6 +935,13 @@ int ggem_set_graph(ggem_ctx* ctx, int32_t enable) {
 int ggem_set_autotune(ggem_ctx* ctx, int32_t enable) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   ctx->tune.on = enable != 0;
+  return GGEM_OK;
+}
+
+int ggem_tune_set(ggem_ctx* ctx, const char* signature, int32_t pick) {
+  if (!ctx || !signature || pick < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid tuning entry");
+  ctx->tune.pick[signature] = pick;
+  ctx->tune.ms.erase(signature);
   return GGEM_OK;
 }
 

@@ -56,6 +56,7 @@ struct SysGrid {
   int64_t elem_off;  // first element id of the system
   int64_t cell_off;  // first brick-order cell index of the system
   int32_t row_off, nrows;
+  int32_t reach[3];  // largest stencil offset per axis, in cells
 };
 
 // Neighbour entries carry the particle slot in the low 26 bits and the
@@ -112,6 +113,7 @@ struct ggem_ctx {
   std::vector<ggem::StencilRow> rows_h;
   ggem::StencilRow* rows_d = nullptr;
   int32_t n_rows = 0;
+  int32_t reach[3] = {0, 0, 0};
 
   // particle store
   int64_t n = 0;
@@ -229,11 +231,12 @@ struct ProfScope {
 
 // Candidate index for `key`: the recorded pick, else (autotuning on, stream
 // not capturing) each candidate is launched once to warm up and three times
-// under events and the fastest is kept; 0 (the default shape) otherwise.
-// launch(c) must be idempotent.
-int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch);
-// The recorded pick without tuning (0 when none or autotuning is off).
-int tune_lookup(const ggem_ctx* ctx, const std::string& key);
+// under events and the fastest is kept; dflt (the default shape) otherwise.
+// launch(c) must be idempotent; a candidate whose launch fails is skipped.
+int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::function<hipError_t(int)>& launch,
+              int dflt = 0);
+// The recorded pick without tuning (dflt when none or autotuning is off).
+int tune_lookup(const ggem_ctx* ctx, const std::string& key, int dflt = 0);
 // "name/precision/.../log2 n" signature of a launch
 std::string tune_key(const char* kernel, int precision, int extra, int64_t n);
 

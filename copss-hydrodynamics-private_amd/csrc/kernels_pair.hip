@@ -24,6 +24,8 @@
 // displacement in FP64, tensor in FP32 (erfc form, shared exponentials),
 // partial sums in FP32 merged every 8 neighbours into a Kahan accumulator;
 // the epilogue (combination, recurrence) runs in FP64.
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ggem_device.h"
@@ -123,6 +125,71 @@ __global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float2* __restrict_
       float A, Bx;
       green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
       c32[o] = make_float2(A, Bx);
+    }
+  }
+}
+
+// Epilogue of the pair sum of target slot s, m = acc (see Epi).
+template <int NR, int EPI>
+__device__ __forceinline__ void apply_epilogue(const PairArgs& a, int64_t s, const double (&acc)[NR][3],
+                                               const double* __restrict__ xsoa, const SlotVec& v,
+                                               const ChebEpi& epi, const double* __restrict__ u_glob,
+                                               double* __restrict__ out) {
+  const int64_t n = a.n;
+  if (EPI == EPI_LOCAL || EPI == EPI_MOBILITY) {
+    // forces are the operand: own force from the FP64 store
+    const double fo[3] = {a.fs[s], a.fs[n + s], a.fs[2 * n + s]};
+    double u[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      u[q] = acc[0][q];
+      if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] += a.k.coeff_2 * fo[q];  // self term
+    }
+    const int64_t p = a.perm[s];
+    if (EPI == EPI_MOBILITY) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double ug = u_glob ? u_glob[3 * p + q] : 0.0;
+        if (a.k.point_type == GGEM_POLYMER_BEAD) u[q] = ug + u[q] - a.k.coeff_1 * fo[q] + fo[q];
+        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] = ug + u[q];
+        else u[q] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[3 * p + q] = u[q];
+  } else {
+    double da = epi.da, db = epi.db, ck = epi.c;
+    if (epi.dpar) {
+      const double* pp = epi.dpar + int64_t(epi.sys ? epi.sys[s] : 0) * epi.pstride;
+      da = pp[0];
+      db = pp[1];
+      ck = pp[2 + epi.term];
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double nv[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int64_t idx = int64_t(r) * 3 * n + int64_t(q) * n + s;
+        const double xv = xsoa[idx];
+        double m = acc[r][q];
+        if (a.k.point_type == GGEM_POLYMER_BEAD) m = m - a.k.coeff_1 * xv + xv;
+        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) m = m + a.k.coeff_2 * xv;
+        else m = 0.0;
+        if (epi.kind == 0) {
+          nv[q] = m;
+        } else if (epi.kind == 1) {
+          nv[q] = db * xv + da * m;
+        } else {
+          double tv = 2. * db * xv + 2. * da * m;
+          tv += -1.0 * epi.xm1.soa[idx];
+          nv[q] = tv;
+          epi.y[idx] += ck * tv;
+        }
+        v.soa[idx] = nv[q];
+      }
+      if (v.g32) v.g32[r * n + s] = make_float4(float(nv[0]), float(nv[1]), float(nv[2]), 0.f);
+      if (v.g64) v.g64[r * n + s] = make_double4(nv[0], nv[1], nv[2], 0.0);
     }
   }
 }
@@ -310,61 +377,329 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
       }
   }
 
-  if (EPI == EPI_LOCAL || EPI == EPI_MOBILITY) {
-    // forces are the operand: own force from the FP64 store
-    const double fo[3] = {a.fs[s], a.fs[n + s], a.fs[2 * n + s]};
-    double u[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      u[q] = acc[0][q];
-      if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] += a.k.coeff_2 * fo[q];  // self term
-    }
-    const int64_t p = a.perm[s];
-    if (EPI == EPI_MOBILITY) {
+  apply_epilogue<NR, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
+}
+
+// FP32C pair sum of one right-hand side with the neighbour records staged
+// in LDS.  One workgroup per brick of 8 x 8 x 8 cells: the records
+// (cell-relative FP32 position, operand) of every particle in the brick's
+// box (the brick widened by the stencil reach, periodic images unwrapped)
+// are copied to LDS in box order: cells x-major through the box, particles
+// of a cell in slot order.  A target then walks its stencil rows across the
+// box; a row's x-run is contiguous in that order and is visited in the same
+// order as its neighbour list (the part wrapped past the high x face first,
+// as for_each_segment emits it), so the pair tensors still stream in list
+// order and the result is bit-identical to k_apply's.  The displacement
+// uses the unwrapped box cells: x_ij = (box_t - box_s) h + (offset_t -
+// offset_s).  A brick whose box exceeds the LDS capacity takes k_apply's
+// path (neighbour gathers from global memory).
+struct TileArgs {
+  const StencilRow* rows;
+  const int32_t* cs;  // cell starts, brick order
+  int32_t nrows;      // plain context
+  int32_t reach[3];   // plain context
+  int32_t cells_cap, rec_cap;
+};
+constexpr int kTileThreads = 256;
+constexpr int kTileMaxCells = 8192;  // box cells (LDS index table entries)
+constexpr int kTileMaxRecs = 4096;   // staged records
+constexpr int kBoxBits = 6;        // box coordinates packed 6 bits per axis
+
+// pair sum of slot s through global gathers (FP32C, one operand)
+__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float2* __restrict__ coef,
+                                                   const float4* __restrict__ xg, double (&acc)[1][3]) {
+  const int cnt = a.cnt[s];
+  const int64_t base = a.slice_off[s >> 6] + (s & 63);
+  const uint32_t* lst = a.ell + base;
+  const float2* cf = coef + base;
+  const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
+  const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const float4 po = a.pc32[s];
+  const uint32_t co = __float_as_uint(po.w);
+  const int io = int(co & ((1u << kCellBitsX) - 1));
+  const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+  const int ko = int(co >> (kCellBitsX + kCellBitsY));
+  float ps[3] = {0.f, 0.f, 0.f};
+  KahanF kh[3];
+  for (int k = 0; k < cnt; ++k) {
+    const uint32_t e = lst[int64_t(k) * kSlice];
+    const uint32_t t = e & kSlotMask;
+    const float2 c = cf[int64_t(k) * kSlice];
+    const float4 pt = a.pc32[t];
+    const float4 f = xg[t];
+    const uint32_t ct = __float_as_uint(pt.w);
+    const uint32_t code = e >> kSlotBits;
+    const int di = int(ct & ((1u << kCellBitsX) - 1)) - io + (int(code & 1u) - int((code >> 1) & 1u)) * nx;
+    const int dj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1)) - jo +
+                   (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * ny;
+    const int dk = int(ct >> (kCellBitsX + kCellBitsY)) - ko + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * nz;
+    const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+    const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+    const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+    const float b = c.y * (dx * f.x + dy * f.y + dz * f.z);
+    ps[0] += c.x * f.x + b * dx;
+    ps[1] += c.x * f.y + b * dy;
+    ps[2] += c.x * f.z + b * dz;
+    if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const double ug = u_glob ? u_glob[3 * p + q] : 0.0;
-        if (a.k.point_type == GGEM_POLYMER_BEAD) u[q] = ug + u[q] - a.k.coeff_1 * fo[q] + fo[q];
-        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) u[q] = ug + u[q];
-        else u[q] = 0.0;
+        kh[q].add(ps[q]);
+        ps[q] = 0.f;
       }
     }
+  }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) out[3 * p + q] = u[q];
-  } else {
-    double da = epi.da, db = epi.db, ck = epi.c;
-    if (epi.dpar) {
-      const double* pp = epi.dpar + int64_t(epi.sys ? epi.sys[s] : 0) * epi.pstride;
-      da = pp[0];
-      db = pp[1];
-      ck = pp[2 + epi.term];
+  for (int q = 0; q < 3; ++q) {
+    kh[q].add(ps[q]);
+    acc[0][q] = kh[q].value();
+  }
+}
+
+// x mod n for x a few periods at most outside [0, n) (no integer division)
+__device__ __forceinline__ int wrap_axis(int x, int n) {
+  while (x < 0) x += n;
+  while (x >= n) x -= n;
+  return x;
+}
+
+// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
+// away from an integer, far more than the FP32 rounding of the product
+__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
+
+template <int EPI>
+__global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArgs ta,
+                                                            const float2* __restrict__ coef,
+                                                            const float4* __restrict__ xg,
+                                                            const double* __restrict__ xsoa, SlotVec v,
+                                                            ChebEpi epi, const double* __restrict__ u_glob,
+                                                            double* __restrict__ out) {
+  extern __shared__ float4 lds[];
+  float4* rpos = lds;                 // [rec_cap] offset xyz, packed box cell
+  float4* rval = lds + ta.rec_cap;    // [rec_cap] operand
+  uint16_t* cst = reinterpret_cast<uint16_t*>(lds + 2 * ta.rec_cap);  // [cells_cap + 1] box cell starts
+  __shared__ int32_t s_wsum[kTileThreads / 64];
+  __shared__ int32_t s_total;
+
+  const int64_t brick = block_index(a.xcd);
+  const int64_t c0 = brick << (3 * kBrickBits);
+  const int32_t sb = ta.cs[c0], se = ta.cs[c0 + (int64_t(1) << (3 * kBrickBits))];
+  if (sb >= se) return;  // no targets in this brick (uniform over the block)
+  const SysGrid* S = a.sys ? a.sys + a.sys_slot[sb] : nullptr;
+  const Grid& g = S ? S->g : a.g;
+  const int64_t coff = S ? S->cell_off : 0;
+  const StencilRow* rows = S ? ta.rows + S->row_off : ta.rows;
+  const int nrows = S ? S->nrows : ta.nrows;
+  const int rx = S ? S->reach[0] : ta.reach[0];
+  const int ry = S ? S->reach[1] : ta.reach[1];
+  const int rz = S ? S->reach[2] : ta.reach[2];
+  const int64_t lb = brick - (coff >> (3 * kBrickBits));
+  const int64_t bxy = int64_t(g.nb[0]) * g.nb[1];
+  const int bk = int(lb / bxy);
+  const int64_t rem = lb - int64_t(bk) * bxy;
+  const int bj = int(rem / g.nb[0]);
+  const int bi = int(rem - int64_t(bj) * g.nb[0]);
+  const int ox = (bi << kBrickBits) - rx, oy = (bj << kBrickBits) - ry, oz = (bk << kBrickBits) - rz;
+  const int ex = (1 << kBrickBits) + 2 * rx, ey = (1 << kBrickBits) + 2 * ry, ez = (1 << kBrickBits) + 2 * rz;
+  const int nbc = ex * ey * ez;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const float inv_ex = 1.0f / float(ex), inv_ey = 1.0f / float(ey);
+  auto box_xyz = [&](int q, int& x, int& y, int& z) {
+    const int yz = div_small(q, inv_ex);
+    x = q - yz * ex;
+    z = div_small(yz, inv_ey);
+    y = yz - z * ey;
+  };
+  // global cell of box cell q; false outside a non-periodic domain
+  auto box_cell = [&](int q, int64_t& c) {
+    int x, y, z;
+    box_xyz(q, x, y, z);
+    int X = ox + x, Y = oy + y, Z = oz + z;
+    if (X < 0 || X >= nx) {
+      if (!g.periodic[0]) return false;
+      X = wrap_axis(X, nx);
     }
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      double nv[3];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int64_t idx = int64_t(r) * 3 * n + int64_t(q) * n + s;
-        const double xv = xsoa[idx];
-        double m = acc[r][q];
-        if (a.k.point_type == GGEM_POLYMER_BEAD) m = m - a.k.coeff_1 * xv + xv;
-        else if (a.k.point_type == GGEM_LAGRANGIAN_POINT) m = m + a.k.coeff_2 * xv;
-        else m = 0.0;
-        if (epi.kind == 0) {
-          nv[q] = m;
-        } else if (epi.kind == 1) {
-          nv[q] = db * xv + da * m;
-        } else {
-          double tv = 2. * db * xv + 2. * da * m;
-          tv += -1.0 * epi.xm1.soa[idx];
-          nv[q] = tv;
-          epi.y[idx] += ck * tv;
+    if (Y < 0 || Y >= ny) {
+      if (!g.periodic[1]) return false;
+      Y = wrap_axis(Y, ny);
+    }
+    if (Z < 0 || Z >= nz) {
+      if (!g.periodic[2]) return false;
+      Z = wrap_axis(Z, nz);
+    }
+    c = coff + cell_index(g, X, Y, Z);
+    return true;
+  };
+
+  bool staged = nbc <= ta.cells_cap && ex <= (1 << kBoxBits) && ey <= (1 << kBoxBits) && ez <= (1 << kBoxBits);
+  if (staged) {
+    // (1) particle count of every box cell; each thread's cells are strided
+    // and their cell-start loads are all issued before any is used
+    const int per = (nbc + kTileThreads - 1) / kTileThreads;
+#pragma unroll 6
+    for (int q = int(threadIdx.x); q < nbc; q += kTileThreads) {
+      int64_t c;
+      const int k = box_cell(q, c) ? ta.cs[c + 1] - ta.cs[c] : 0;
+      cst[q] = uint16_t(min(k, 65535));
+    }
+    __syncthreads();
+    // (2) exclusive scan of the counts in place (contiguous chunks per thread)
+    const int q0 = min(nbc, int(threadIdx.x) * per), q1 = min(nbc, q0 + per);
+    int mine = 0;
+    for (int q = q0; q < q1; ++q) mine += cst[q];
+    int incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off, 64);
+      if ((threadIdx.x & 63) >= off) incl += y;
+    }
+    if ((threadIdx.x & 63) == 63) s_wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < int(threadIdx.x >> 6); ++w) before += s_wsum[w];
+    if (threadIdx.x == kTileThreads - 1) s_total = before + incl;
+    __syncthreads();
+    const int total = s_total;
+    staged = total <= ta.rec_cap;  // uniform
+    if (staged) {
+      int run = before + incl - mine;
+      for (int q = q0; q < q1; ++q) {
+        const int k = cst[q];
+        cst[q] = uint16_t(run);
+        run += k;
+      }
+      if (threadIdx.x == kTileThreads - 1) cst[nbc] = uint16_t(total);
+    }
+    __syncthreads();
+    if (staged) {
+      // (3) records: record i of box cell q (binary search of the starts),
+      // the cell's first slot, then the record itself
+#pragma unroll 4
+      for (int i = int(threadIdx.x); i < total; i += kTileThreads) {
+        int lo = 0, hi = nbc;  // largest q with cst[q] <= i
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (int(cst[mid]) <= i) lo = mid;
+          else hi = mid;
         }
-        v.soa[idx] = nv[q];
+        int64_t c = 0;
+        box_cell(lo, c);
+        const int64_t t = int64_t(ta.cs[c]) + (i - int(cst[lo]));
+        int x, y, z;
+        box_xyz(lo, x, y, z);
+        const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
+        const float4 p = a.pc32[t];
+        rpos[i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
+        rval[i] = xg[t];
       }
-      if (v.g32) v.g32[r * n + s] = make_float4(float(nv[0]), float(nv[1]), float(nv[2]), 0.f);
-      if (v.g64) v.g64[r * n + s] = make_double4(nv[0], nv[1], nv[2], 0.0);
     }
+    __syncthreads();
+  }
+
+  const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
+  for (int64_t s = sb + threadIdx.x; s < se; s += kTileThreads) {
+    double acc[1][3];
+    if (!staged) {
+      fp32c_pairs_global(a, s, coef, xg, acc);
+    } else {
+      const float4 po = a.pc32[s];
+      const uint32_t co = __float_as_uint(po.w);
+      const int io = int(co & ((1u << kCellBitsX) - 1));
+      const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+      const int ko = int(co >> (kCellBitsX + kCellBitsY));
+      const int bx = io - ox, by = jo - oy, bz = ko - oz;  // own box cell
+      const int64_t cown = coff + cell_index(g, io, jo, ko);
+      const int selfq = int(cst[(bz * ey + by) * ex + bx]) + int(s - ta.cs[cown]);
+      const int cnt = a.cnt[s];
+      const int last = cnt > 0 ? cnt - 1 : 0;
+      const float2* cf = coef + a.slice_off[s >> 6] + (s & 63);
+      // tensors of the next entries in flight (list order); an empty list
+      // has no storage
+      const float2 zero = make_float2(0.f, 0.f);
+      float2 c0 = cnt > 0 ? cf[0] : zero;
+      float2 c1 = cnt > 0 ? cf[int64_t(min(1, last)) * kSlice] : zero;
+      float2 c2 = cnt > 0 ? cf[int64_t(min(2, last)) * kSlice] : zero;
+      float ps[3] = {0.f, 0.f, 0.f};
+      KahanF kh[3];
+      // Cursor over the list in list order: the record range of the current
+      // stencil row (a row wrapped past the high x face has two ranges, the
+      // wrapped one first).  The loop below steps k uniformly across the
+      // wave, so the tensor loads stay coalesced as in k_apply.
+      int r = -1, q = 0, qend = 0, q2 = 0, qend2 = 0;
+      bool has2 = false;
+      auto advance = [&]() {
+        for (;;) {
+          if (q < qend) {
+            if (q != selfq) return;
+            ++q;
+            continue;
+          }
+          if (has2) {
+            q = q2;
+            qend = qend2;
+            has2 = false;
+            continue;
+          }
+          if (++r >= nrows) return;
+          const StencilRow R = rows[r];
+          const int J = jo + R.dj, K = ko + R.dk;
+          if (!g.periodic[1] && (J < 0 || J >= ny)) continue;
+          if (!g.periodic[2] && (K < 0 || K >= nz)) continue;
+          int X0 = io + R.di_lo, X1 = io + R.di_hi;
+          if (!g.periodic[0]) {
+            X0 = max(X0, 0);
+            X1 = min(X1, nx - 1);
+          }
+          if (X0 > X1) continue;
+          const int rb = ((K - oz) * ey + (J - oy)) * ex - ox;
+          if (X1 >= nx) {  // the part wrapped past the high x face comes first
+            q = cst[rb + nx];
+            qend = cst[rb + X1 + 1];
+            q2 = cst[rb + X0];
+            qend2 = cst[rb + nx];
+            has2 = true;
+          } else {
+            q = cst[rb + X0];
+            qend = cst[rb + X1 + 1];
+          }
+        }
+      };
+      advance();
+      const int qmax = s_total - 1;  // staged records (a consistent walk never reaches it)
+      for (int k = 0; k < cnt; ++k) {
+        const int qq = min(q++, qmax);
+        const float2 c = c0;
+        c0 = c1;
+        c1 = c2;
+        c2 = cf[int64_t(min(k + 3, last)) * kSlice];
+        const float4 pt = rpos[qq];
+        const float4 f = rval[qq];
+        advance();
+        const uint32_t bt = __float_as_uint(pt.w);
+        const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
+        const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
+        const int dk = int(bt >> (2 * kBoxBits)) - bz;
+        const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+        const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+        const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+        const float b = c.y * (dx * f.x + dy * f.y + dz * f.z);
+        ps[0] += c.x * f.x + b * dx;
+        ps[1] += c.x * f.y + b * dy;
+        ps[2] += c.x * f.z + b * dz;
+        if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+#pragma unroll
+          for (int u = 0; u < 3; ++u) {
+            kh[u].add(ps[u]);
+            ps[u] = 0.f;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        kh[u].add(ps[u]);
+        acc[0][u] = kh[u].value();
+      }
+    }
+    apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
   }
 }
 
@@ -463,10 +798,76 @@ void launch_kb(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float
                        ctx->coef32, g64, g32, soa, v, epi, u_glob, out);
 }
 
+// LDS layout of the tile kernel, or false when the context cannot use it
+// (explicit minimum image, or boxes too large for 64 KiB of LDS)
+bool tile_args(const ggem_ctx* ctx, TileArgs* ta, size_t* lds) {
+  if (ctx->exact_minimg || ctx->n == 0) return false;
+  int cells = 0;
+  double recs = 0.0;
+  auto one = [&](const int* reach, int64_t n_elem, int64_t count) {
+    const int box = ((1 << kBrickBits) + 2 * reach[0]) * ((1 << kBrickBits) + 2 * reach[1]) *
+                    ((1 << kBrickBits) + 2 * reach[2]);
+    cells = std::max(cells, box);
+    // capacity: the expected box population plus 4.5 standard deviations of
+    // a Poisson count (bricks above it take the global-gather path)
+    const double expect = double(count) / double(std::max<int64_t>(n_elem, 1)) * box;
+    recs = std::max(recs, expect + 4.5 * std::sqrt(expect) + 16.0);
+  };
+  if (ctx->nsys > 1) {
+    for (int k = 0; k < ctx->nsys; ++k)
+      one(ctx->sys_h[k].reach, ctx->sys_h[k].g.n_elem, ctx->part_off[k + 1] - ctx->part_off[k]);
+  } else {
+    one(ctx->reach, ctx->grid.n_elem, ctx->n);
+  }
+  if (cells > kTileMaxCells) return false;
+  const size_t budget = 65536;
+  const size_t cell_bytes = (size_t(cells + 1) * sizeof(uint16_t) + 15) / 16 * 16;
+  if (cell_bytes + 64 * 32 > budget) return false;
+  const int rec_cap = int(std::min<double>(std::min<double>(recs, double((budget - cell_bytes) / 32)),
+                                           double(kTileMaxRecs)));
+  ta->rows = ctx->rows_d;
+  ta->cs = ctx->cell_start;
+  ta->nrows = ctx->n_rows;
+  for (int d = 0; d < 3; ++d) ta->reach[d] = ctx->reach[d];
+  ta->cells_cap = cells;
+  ta->rec_cap = rec_cap;
+  *lds = size_t(rec_cap) * 32 + cell_bytes;
+  return true;
+}
+
+template <int EPI>
+hipError_t launch_tile(ggem_ctx* ctx, PairArgs a, const float4* xg, const double* soa, const SlotVec& v,
+                       const ChebEpi& epi, const double* u_glob, double* out) {
+  TileArgs ta;
+  size_t lds = 0;
+  if (!tile_args(ctx, &ta, &lds)) return hipErrorInvalidConfiguration;
+  a.xcd = 1;
+  const unsigned nb = unsigned(ctx->grid.n_cells >> (3 * kBrickBits));
+  hipLaunchKernelGGL((k_apply_tile<EPI>), dim3(nb), dim3(kTileThreads), lds, ctx->stream, a, ta, ctx->coef32, xg,
+                     soa, v, epi, u_glob, out);
+  return hipGetLastError();
+}
+
+// candidate index of the tile kernel; the default shape of an apply is the
+// global-gather kernel (measured faster on the 1M-bead workload), the tile
+// kernel is one more candidate for the autotuner where it applies
+constexpr int kTileShape = kNApply;
+int apply_default(const ggem_ctx*, int, int) { return 0; }
+int apply_candidates(const ggem_ctx* ctx, int precision, int nr) {
+  TileArgs ta;
+  size_t lds;
+  return precision == GGEM_FP32C && nr == 1 && tile_args(ctx, &ta, &lds) ? kNApply + 1 : kNApply;
+}
+
 // one apply launch in candidate shape c
 template <int P, int NR, int EPI>
 hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const float4* g32, const double* soa,
                     const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
+  if (c == kTileShape) {
+    if constexpr (P == GGEM_FP32C && NR == 1) return launch_tile<EPI>(ctx, a, g32, soa, v, epi, u_glob, out);
+    return hipErrorInvalidConfiguration;
+  }
+  if (c < 0 || c >= kNApply) return hipErrorInvalidValue;
   const Shape sh = kApplyShapes[c];
   a.xcd = sh.xcd;
   switch (sh.block) {
@@ -497,6 +898,7 @@ void launch_tb(ggem_ctx* ctx, const PairArgs& a) {
 
 template <int P, bool E, bool B>
 hipError_t launch_t(ggem_ctx* ctx, PairArgs a, int c) {
+  if (c < 0 || c >= kNTensor) return hipErrorInvalidValue;
   const Shape sh = kTensorShapes[c];
   a.xcd = sh.xcd;
   switch (sh.block) {
@@ -545,7 +947,8 @@ hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double
                                                        out);
   };
   // both epilogues write out[] only: idempotent, so the shape can be tuned here
-  const int c = tune_pick(ctx, tune_key("pair_beads", precision, epi, ctx->n), kNApply, run);
+  const int c = tune_pick(ctx, tune_key("pair_beads", precision, epi, ctx->n), apply_candidates(ctx, precision, 1),
+                          run, apply_default(ctx, precision, 1));
   ProfScope ps(ctx, precision == GGEM_FP64 ? "pair_beads_fp64" : "pair_beads_fp32c");
   return run(c);
 }
@@ -599,10 +1002,12 @@ hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x
   std::vector<int> pick(chunks.size());
   for (size_t i = 0; i < chunks.size(); ++i) {
     const std::string key = tune_key("apply", precision, chunks[i].nr, n);
+    const int dflt = apply_default(ctx, precision, chunks[i].nr);
     if (epi.kind != 2 && v.soa != x.soa)
-      pick[i] = tune_pick(ctx, key, kNApply, [&](int c) { return run(chunks[i], c); });
+      pick[i] = tune_pick(ctx, key, apply_candidates(ctx, precision, chunks[i].nr),
+                          [&](int c) { return run(chunks[i], c); }, dflt);
     else
-      pick[i] = tune_lookup(ctx, key);
+      pick[i] = tune_lookup(ctx, key, dflt);
   }
   ProfScope ps(ctx, precision == GGEM_FP64 ? "apply_fp64" : "apply_fp32c");
   for (size_t i = 0; i < chunks.size(); ++i) {

@@ -191,6 +191,7 @@ hipError_t launch_sp(ggem_ctx* ctx, int c, const double* gp, const double* IF_d,
   a.x4 = ctx->x4;
   a.fg32 = ctx->fg32;
   a.fg64 = ctx->fg64;
+  if (c < 0 || c >= kNElem) return hipErrorInvalidValue;
   switch (kElemChoices[c]) {
     case 32: launch_spk<T, NQ1, 32>(ctx, a, IF_d, n_u, rho, Fe); break;
     case 16: launch_spk<T, NQ1, 16>(ctx, a, IF_d, n_u, rho, Fe); break;

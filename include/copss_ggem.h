@@ -275,6 +275,11 @@ int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*n
  * bit-identical results.  Off (the default): the default shape.  No
  * counterpart in the reference (CPU loops have no launch shape). */
 int ggem_set_autotune(ggem_ctx* ctx, int32_t enable);
+/* Pin the pick of one signature (as printed by ggem_tuning_report), e.g. to
+ * replay a tuning table recorded in an earlier run; used while autotuning
+ * is on.  A pick outside the kernel's candidate list makes that kernel's
+ * launches fail with GGEM_ERR_HIP. */
+int ggem_tune_set(ggem_ctx* ctx, const char* signature, int32_t pick);
 /* The picks so far, one line per signature: "<signature> <pick> <ms per
  * candidate>...".  buf (cap bytes, NUL-terminated) may be NULL; *len gets
  * the full length. */

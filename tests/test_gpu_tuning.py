@@ -64,3 +64,48 @@ def test_autotune_results_identical_and_picks_recorded():
         assert np.array_equal(yg, ref[G.FP32C, "y"]) and np.array_equal(dg, ref[G.FP32C, "d"])
     finally:
         ctx.close()
+
+
+def test_tile_kernel_pinned_matches_default():
+    """The LDS-staged brick kernel (an autotuner candidate for FP32C single
+    right-hand sides) gives the default kernel's results bit for bit, on a
+    plain and on a batched context."""
+    import math
+    TILE = 5  # candidate index of the tile kernel (after the five block shapes)
+
+    def pin(ctx, n):
+        lg = int(math.floor(math.log2(n)))
+        for key in ("apply/fp32c/1/%d" % lg, "pair_beads/fp32c/0/%d" % lg, "pair_beads/fp32c/1/%d" % lg):
+            ctx.tune_set(key, TILE)
+        ctx.set_autotune(True)
+
+    cfg, x, f = W.make_inputs("beads20k")
+    cfg["precision"] = G.FP32C
+    dw = G.normal(3 * len(x), 23).reshape(1, -1, 3)
+    ctx = G.Context(cfg)
+    try:
+        ctx.reinit(x, f)
+        u0 = ctx.local_velocity_beads()
+        U0 = ctx.mobility_apply(f)
+        y0, d0, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+        pin(ctx, len(x))
+        u1 = ctx.local_velocity_beads()
+        U1 = ctx.mobility_apply(f)
+        y1, d1, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+        assert np.array_equal(u0, u1) and np.array_equal(U0, U1)
+        assert np.array_equal(y0, y1) and np.array_equal(d0, d1)
+    finally:
+        ctx.close()
+    cfgs, xs, fs, dwb = W.make_batch(nsys=4, nrhs=1, seed=12, n_lo=400, n_hi=2000)
+    for c in cfgs:
+        c["precision"] = G.FP32C
+    b = G.BatchContext(cfgs)
+    try:
+        b.set_particles(xs, fs)
+        b.build_neighbors()
+        u0 = b.local_velocity_beads()
+        pin(b, b.n)
+        u1 = b.local_velocity_beads()
+        assert np.array_equal(u0, u1)
+    finally:
+        b.close()
