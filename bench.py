@@ -50,12 +50,12 @@ def parse():
 
 
 # Algorithmic HBM bytes of one FP32C Chebyshev application (k_apply, one
-# right-hand side): per ordered pair the 4-byte list entry and the 16-byte
-# pair tensor; per slot the operand record read once (16), x, x_{k-1} (24
-# each), y read and written (48), v and its record written (24 + 16), the
-# list length (4).
-B_PAIR_APPLY32 = 4 + 16
-B_SLOT_APPLY = 16 + 24 + 24 + 48 + 24 + 16 + 4
+# right-hand side): per ordered pair the 4-byte list entry and the 8-byte
+# pair tensor (A, Bx); per slot the operand and position records read once
+# (16 + 16), x, x_{k-1} (24 each), y read and written (48), v and its
+# record written (24 + 16), the list length (4).
+B_PAIR_APPLY32 = 4 + 8
+B_SLOT_APPLY = 16 + 16 + 24 + 24 + 48 + 24 + 16 + 4
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_beads1m.txt")
 
 
