@@ -59,6 +59,13 @@ struct SysGrid {
   int32_t reach[3];  // largest stencil offset per axis, in cells
 };
 
+// FP32C pair record: the neighbour entry and the tensor (A, Bx) of the pair,
+// one 12-byte load per list step
+struct Ent32 {
+  uint32_t e;
+  float a, bx;
+};
+
 // Neighbour entries carry the particle slot in the low 26 bits and the
 // periodic image of the pair in the high 6 bits: 2 bits per axis, 1 = add
 // the box length, 2 = subtract it (the single shift of point_vector).
@@ -154,9 +161,9 @@ struct ggem_ctx {
   int64_t ell_cap = 0;
   int64_t ell_len = 0;
   // pair tensors of the current lists, same layout as ell, G = A I + Bx x x^T:
-  // FP32C keeps (A, Bx) and forms x_ij from the cell-relative positions pc32;
-  // FP64 keeps (A, w) with w = sqrt(|Bx|) x_ij and Bx = sgn |Bx|
-  float2* coef32 = nullptr;
+  // FP32C keeps (entry, A, Bx) and forms x_ij from the cell-relative
+  // positions pc32; FP64 keeps (A, w) with w = sqrt(|Bx|) x_ij, Bx = sgn |Bx|
+  ggem::Ent32* coef32 = nullptr;
   int64_t coef32_cap = 0;
   bool coef32_valid = false;
   float4* pc32 = nullptr;     // [n] by slot: FP32 position relative to its cell centre, packed cell (i, j, k)

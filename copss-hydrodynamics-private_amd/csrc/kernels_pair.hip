@@ -93,7 +93,7 @@ struct V4<double> {
 
 // Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
 template <int P, bool EXACT, bool BATCH, int BLK>
-__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float2* __restrict__ c32,
+__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, Ent32* __restrict__ c32,
                                                 double4* __restrict__ c64) {
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, float2* __restrict_
       const float fx = float(dx), fy = float(dy), fz = float(dz);
       float A, Bx;
       green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
-      c32[o] = make_float2(A, Bx);
+      c32[o] = Ent32{e, A, Bx};
     }
   }
 }
@@ -219,11 +219,11 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
   // dependent gathers).  The summation order is the list order.
   constexpr int U = NR >= 4 ? 1 : (NR == 2 ? 2 : 4);
   const int last = cnt > 0 ? cnt - 1 : 0;
-  uint32_t en[U];
-#pragma unroll
-  for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? lst[uint32_t(min(i, last)) * uint32_t(kSlice)] : 0u;
 
   if constexpr (P == GGEM_FP64) {
+    uint32_t en[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? lst[uint32_t(min(i, last)) * uint32_t(kSlice)] : 0u;
     const double4* __restrict__ cf = static_cast<const double4*>(coef_raw) + base;
     const double4* __restrict__ xg = xg64;
     const double sg = a.bsign;
@@ -275,10 +275,14 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
     // FP32C: G f = A f + Bx (x . f) x with the stored (A, Bx) and x_ij from
     // the cell-relative FP32 positions: x_ij = (cell_t - cell_s + shift) h
     // + (offset_t - offset_s), per axis
-    const float2* __restrict__ cf = static_cast<const float2*>(coef_raw) + base;
+    const Ent32* __restrict__ cf = static_cast<const Ent32*>(coef_raw) + base;
     const float4* __restrict__ xg = xg32;
     const float4* __restrict__ pc = a.pc32;
     const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
+    Ent32 en[U];  // records of the next group (entry and tensor together)
+    const Ent32 none{0u, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? cf[int64_t(min(i, last)) * kSlice] : none;
     const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const float Lx = float(g.L[0]), Ly = float(g.L[1]), Lz = float(g.L[2]);
@@ -311,15 +315,15 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
         if (g.periodic[2]) dz = dz <= -0.5f * Lz ? dz + Lz : (dz > 0.5f * Lz ? dz - Lz : dz);
       }
     };
-    auto pair_acc = [&](uint32_t e, const float2& c, const float4& pt, const float4* f) {
+    auto pair_acc = [&](const Ent32& c, const float4& pt, const float4* f) {
       float dx, dy, dz;
-      disp(e, pt, dx, dy, dz);
+      disp(c.e, pt, dx, dy, dz);
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const float b = c.y * (dx * f[r].x + dy * f[r].y + dz * f[r].z);
-        ps[r][0] += c.x * f[r].x + b * dx;
-        ps[r][1] += c.x * f[r].y + b * dy;
-        ps[r][2] += c.x * f[r].z + b * dz;
+        const float b = c.bx * (dx * f[r].x + dy * f[r].y + dz * f[r].z);
+        ps[r][0] += c.a * f[r].x + b * dx;
+        ps[r][1] += c.a * f[r].y + b * dy;
+        ps[r][2] += c.a * f[r].z + b * dz;
       }
     };
     auto merge = [&](int k) {
@@ -335,36 +339,35 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
     };
     int k = 0;
     for (; k + U <= cnt; k += U) {
-      uint32_t e[U];
-      float2 c[U];
+      Ent32 c[U];
       float4 pt[U];
 #pragma unroll
       for (int i = 0; i < U; ++i) {
-        e[i] = en[i];
-        en[i] = lst[uint32_t(min(k + U + i, last)) * uint32_t(kSlice)];
-        c[i] = cf[int64_t(k + i) * kSlice];
+        c[i] = en[i];
+        en[i] = cf[int64_t(min(k + U + i, last)) * kSlice];
       }
       float4 f[U][NR];
 #pragma unroll
       for (int i = 0; i < U; ++i) {
-        pt[i] = pc[e[i] & kSlotMask];
+        const uint32_t t = c[i].e & kSlotMask;
+        pt[i] = pc[t];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + (e[i] & kSlotMask)];
+        for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + t];
       }
 #pragma unroll
       for (int i = 0; i < U; ++i) {
-        pair_acc(e[i], c[i], pt[i], f[i]);
+        pair_acc(c[i], pt[i], f[i]);
         merge(k + i);
       }
     }
 #pragma unroll
     for (int i = 0; i < U - 1; ++i) {
-      if (k + i < cnt) {  // entries en[0 .. cnt - k)
-        const uint32_t t = en[i] & kSlotMask;
+      if (k + i < cnt) {  // records en[0 .. cnt - k)
+        const uint32_t t = en[i].e & kSlotMask;
         float4 f[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) f[r] = xg[r * n + t];
-        pair_acc(en[i], cf[int64_t(k + i) * kSlice], pc[t], f);
+        pair_acc(en[i], pc[t], f);
         merge(k + i);
       }
     }
@@ -406,12 +409,11 @@ constexpr int kTileMaxRecs = 4096;   // staged records
 constexpr int kBoxBits = 6;        // box coordinates packed 6 bits per axis
 
 // pair sum of slot s through global gathers (FP32C, one operand)
-__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float2* __restrict__ coef,
+__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const Ent32* __restrict__ coef,
                                                    const float4* __restrict__ xg, double (&acc)[1][3]) {
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
-  const uint32_t* lst = a.ell + base;
-  const float2* cf = coef + base;
+  const Ent32* cf = coef + base;
   const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
   const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -423,9 +425,9 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   float ps[3] = {0.f, 0.f, 0.f};
   KahanF kh[3];
   for (int k = 0; k < cnt; ++k) {
-    const uint32_t e = lst[int64_t(k) * kSlice];
+    const Ent32 c = cf[int64_t(k) * kSlice];
+    const uint32_t e = c.e;
     const uint32_t t = e & kSlotMask;
-    const float2 c = cf[int64_t(k) * kSlice];
     const float4 pt = a.pc32[t];
     const float4 f = xg[t];
     const uint32_t ct = __float_as_uint(pt.w);
@@ -437,10 +439,10 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
     const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
     const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
     const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-    const float b = c.y * (dx * f.x + dy * f.y + dz * f.z);
-    ps[0] += c.x * f.x + b * dx;
-    ps[1] += c.x * f.y + b * dy;
-    ps[2] += c.x * f.z + b * dz;
+    const float b = c.bx * (dx * f.x + dy * f.y + dz * f.z);
+    ps[0] += c.a * f.x + b * dx;
+    ps[1] += c.a * f.y + b * dy;
+    ps[2] += c.a * f.z + b * dz;
     if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -469,7 +471,7 @@ __device__ __forceinline__ int div_small(int q, float inv_d) { return int((float
 
 template <int EPI>
 __global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArgs ta,
-                                                            const float2* __restrict__ coef,
+                                                            const Ent32* __restrict__ coef,
                                                             const float4* __restrict__ xg,
                                                             const double* __restrict__ xsoa, SlotVec v,
                                                             ChebEpi epi, const double* __restrict__ u_glob,
@@ -611,13 +613,13 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArg
       const int selfq = int(cst[(bz * ey + by) * ex + bx]) + int(s - ta.cs[cown]);
       const int cnt = a.cnt[s];
       const int last = cnt > 0 ? cnt - 1 : 0;
-      const float2* cf = coef + a.slice_off[s >> 6] + (s & 63);
+      const Ent32* cf = coef + a.slice_off[s >> 6] + (s & 63);
       // tensors of the next entries in flight (list order); an empty list
       // has no storage
-      const float2 zero = make_float2(0.f, 0.f);
-      float2 c0 = cnt > 0 ? cf[0] : zero;
-      float2 c1 = cnt > 0 ? cf[int64_t(min(1, last)) * kSlice] : zero;
-      float2 c2 = cnt > 0 ? cf[int64_t(min(2, last)) * kSlice] : zero;
+      const Ent32 zero{0u, 0.f, 0.f};
+      Ent32 c0 = cnt > 0 ? cf[0] : zero;
+      Ent32 c1 = cnt > 0 ? cf[int64_t(min(1, last)) * kSlice] : zero;
+      Ent32 c2 = cnt > 0 ? cf[int64_t(min(2, last)) * kSlice] : zero;
       float ps[3] = {0.f, 0.f, 0.f};
       KahanF kh[3];
       // Cursor over the list in list order: the record range of the current
@@ -667,7 +669,7 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArg
       const int qmax = s_total - 1;  // staged records (a consistent walk never reaches it)
       for (int k = 0; k < cnt; ++k) {
         const int qq = min(q++, qmax);
-        const float2 c = c0;
+        const Ent32 c = c0;
         c0 = c1;
         c1 = c2;
         c2 = cf[int64_t(min(k + 3, last)) * kSlice];
@@ -681,10 +683,10 @@ __global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArg
         const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
         const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
         const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-        const float b = c.y * (dx * f.x + dy * f.y + dz * f.z);
-        ps[0] += c.x * f.x + b * dx;
-        ps[1] += c.x * f.y + b * dy;
-        ps[2] += c.x * f.z + b * dz;
+        const float b = c.bx * (dx * f.x + dy * f.y + dz * f.z);
+        ps[0] += c.a * f.x + b * dx;
+        ps[1] += c.a * f.y + b * dy;
+        ps[2] += c.a * f.z + b * dz;
         if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
           for (int u = 0; u < 3; ++u) {
