@@ -16,6 +16,8 @@ namespace {
 thread_local std::string g_err;  // errors without a context
 const double PI = 3.14159265358979323846;
 
+int grow_scan64(ggem_ctx* ctx, int64_t len);
+
 }  // namespace
 
 namespace ggem {
@@ -125,6 +127,45 @@ int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::funct
   return best;
 }
 
+// Staged neighbour records of the FP32C lists (per brick of cells): counts,
+// their scan, one synchronisation for the sizes, then records and
+// re-indexed entries.
+int prepare_box(ggem_ctx* ctx) {
+  ctx->box_valid = false;
+  ctx->box_max = 0;
+  if (!box_supported(ctx)) return GGEM_OK;
+  const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
+  if (ctx->box_brick_cap < nb + 1) {
+    if (ctx->box_cnt) (void)hipFree(ctx->box_cnt);
+    ctx->box_cnt = ctx->box_off = nullptr;
+    ctx->box_brick_cap = 0;
+    GGEM_HIP(ctx, hipMalloc(&ctx->box_cnt, sizeof(int64_t) * 2 * (nb + 1)));
+    ctx->box_brick_cap = nb + 1;
+  }
+  ctx->box_off = ctx->box_cnt + ctx->box_brick_cap;
+  if (!ctx->box_max_d) GGEM_HIP(ctx, hipMalloc(&ctx->box_max_d, sizeof(int32_t)));
+  int rc = grow_scan64(ctx, int64_t(nb) + 1);
+  if (rc) return rc;
+  hipError_t e = launch_box_count(ctx);
+  if (e != hipSuccess) return hip_fail(ctx, e, "box_count");
+  if ((e = launch_scan64(ctx, ctx->box_cnt, int64_t(nb), ctx->box_off)) != hipSuccess)
+    return hip_fail(ctx, e, "box scan");
+  int64_t total = 0;
+  int32_t mx = 0;
+  GGEM_HIP(ctx, hipMemcpyAsync(&total, ctx->box_off + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&mx, ctx->box_max_d, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (total <= 0 || mx <= 0) return GGEM_OK;  // no brick staged: the plain kernel serves
+  if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total)))) return rc;
+  if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total)))) return rc;
+  if ((rc = ensure(ctx, &ctx->box_coef, &ctx->box_coef_cap, size_t(std::max<int64_t>(ctx->ell_len, 1)))))
+    return rc;
+  if ((e = launch_box_fill(ctx)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  ctx->box_max = mx;
+  ctx->box_valid = true;
+  return GGEM_OK;
+}
+
 int ensure_tensors(ggem_ctx* ctx) {
   const int prec = ctx->p.precision;
   if (prec == GGEM_FP64 ? ctx->coef64_valid : ctx->coef32_valid) return GGEM_OK;
@@ -142,8 +183,12 @@ int ensure_tensors(ggem_ctx* ctx) {
   if (rc) return rc;
   hipError_t e = launch_pair_tensors(ctx, prec);
   if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
-  if (prec == GGEM_FP64) ctx->coef64_valid = true;
-  else ctx->coef32_valid = true;
+  if (prec == GGEM_FP64) {
+    ctx->coef64_valid = true;
+  } else {
+    if ((rc = prepare_box(ctx))) return rc;
+    ctx->coef32_valid = true;
+  }
   return GGEM_OK;
 }
 
@@ -312,6 +357,11 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->elist_off);
   f(ctx->elist);
   f(ctx->coef32);
+  f(ctx->box_cnt);
+  f(ctx->box_slot);
+  f(ctx->box_pos);
+  f(ctx->box_coef);
+  f(ctx->box_max_d);
   f(ctx->pc32);
   f(ctx->coef64);
   f(ctx->noise);

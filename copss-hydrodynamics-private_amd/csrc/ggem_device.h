@@ -35,10 +35,14 @@ __device__ __forceinline__ void elem_ijk(const Grid& g, int64_t e, int& i, int& 
 }
 
 // Brick-order cell index of cell (i, j, k) and its inverse.
-__device__ __forceinline__ int64_t cell_index(const Grid& g, int i, int j, int k) {
+__device__ __forceinline__ int64_t cell_index_nb(int nb0, int nb1, int i, int j, int k) {
   constexpr int B = kBrickBits, M = (1 << kBrickBits) - 1;
-  const int64_t brick = (i >> B) + int64_t(g.nb[0]) * ((j >> B) + int64_t(g.nb[1]) * (k >> B));
+  const int64_t brick = (i >> B) + int64_t(nb0) * ((j >> B) + int64_t(nb1) * (k >> B));
   return (brick << (3 * B)) | ((k & M) << (2 * B)) | ((j & M) << B) | (i & M);
+}
+
+__device__ __forceinline__ int64_t cell_index(const Grid& g, int i, int j, int k) {
+  return cell_index_nb(g.nb[0], g.nb[1], i, j, k);
 }
 
 __device__ __forceinline__ void cell_ijk(const Grid& g, int64_t c, int& i, int& j, int& k) {

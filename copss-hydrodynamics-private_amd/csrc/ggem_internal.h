@@ -170,6 +170,21 @@ struct ggem_ctx {
   double4* coef64 = nullptr;
   int64_t coef64_cap = 0;
   bool coef64_valid = false;
+  // FP32C neighbour records staged per brick of 8 x 8 x 8 cells (the brick
+  // widened by the stencil reach): built with coef32 when the lists carry
+  // periodic shifts; a brick without records gathers from global memory
+  int64_t* box_cnt = nullptr;     // [n_bricks] records of each staged brick, 0 otherwise
+  int64_t* box_off = nullptr;     // [n_bricks + 1] first record of each brick
+  size_t box_brick_cap = 0;
+  uint32_t* box_slot = nullptr;   // [records] slot of each record
+  size_t box_rec_cap = 0;
+  float4* box_pos = nullptr;      // [records] cell-relative FP32 position, packed box cell
+  size_t box_pos_cap = 0;
+  ggem::Ent32* box_coef = nullptr;  // coef32 with the record number as index (staged bricks)
+  size_t box_coef_cap = 0;
+  int32_t* box_max_d = nullptr;   // largest record count of a staged brick
+  int32_t box_max = 0;            // same on the host; 0 when no brick is staged
+  bool box_valid = false;
   int bsign = 1;  // sign of Bx: +1 when xi > alpha
 
   // element particle lists (spreading), CSR over elements
@@ -298,6 +313,12 @@ struct ChebEpi {
 };
 // pair tensors of every list entry for the given precision
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
+// FP32C staged records: box_cnt (and box_max_d, cleared first) per brick
+hipError_t launch_box_count(ggem_ctx* ctx);
+// records and re-indexed entries, with box_off scanned from box_cnt
+hipError_t launch_box_fill(ggem_ctx* ctx);
+// whether the context's FP32C lists can be staged per brick
+bool box_supported(const ggem_ctx* ctx);
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,
                              double* out);
 hipError_t launch_pair_points(ggem_ctx* ctx, int precision, int64_t m, const double* pts,

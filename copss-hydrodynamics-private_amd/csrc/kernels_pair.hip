@@ -383,30 +383,267 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
   apply_epilogue<NR, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
 }
 
-// FP32C pair sum of one right-hand side with the neighbour records staged
-// in LDS.  One workgroup per brick of 8 x 8 x 8 cells: the records
-// (cell-relative FP32 position, operand) of every particle in the brick's
-// box (the brick widened by the stencil reach, periodic images unwrapped)
-// are copied to LDS in box order: cells x-major through the box, particles
-// of a cell in slot order.  A target then walks its stencil rows across the
-// box; a row's x-run is contiguous in that order and is visited in the same
-// order as its neighbour list (the part wrapped past the high x face first,
-// as for_each_segment emits it), so the pair tensors still stream in list
-// order and the result is bit-identical to k_apply's.  The displacement
-// uses the unwrapped box cells: x_ij = (box_t - box_s) h + (offset_t -
-// offset_s).  A brick whose box exceeds the LDS capacity takes k_apply's
-// path (neighbour gathers from global memory).
-struct TileArgs {
-  const StencilRow* rows;
-  const int32_t* cs;  // cell starts, brick order
-  int32_t nrows;      // plain context
-  int32_t reach[3];   // plain context
-  int32_t cells_cap, rec_cap;
+// x mod n for x a few periods at most outside [0, n) (no integer division)
+__device__ __forceinline__ int wrap_axis(int x, int n) {
+  while (x < 0) x += n;
+  while (x >= n) x -= n;
+  return x;
+}
+
+// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
+// away from an integer, far more than the FP32 rounding of the product
+__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
+
+// ---- FP32C pair sum with the neighbour records staged in LDS ----
+//
+// One workgroup per brick of 8 x 8 x 8 cells.  The brick's box is the brick
+// widened by the stencil reach (periodic images unwrapped); each particle of
+// the box is one record, in box order: box cells x-major through the box,
+// particles of a cell in slot order.  Once per list build k_box_fill writes
+// every brick's records (slot, cell-relative FP32 position with the packed
+// box cell) and a copy of the FP32C entries whose index field is the
+// neighbour's record number in its target's box.  k_apply_box copies the
+// records and their operands to LDS and streams the entries as k_apply
+// does, reading the neighbour from LDS.  The summation order (list order)
+// and the arithmetic are k_apply's; the displacement (box_t - box_s) h +
+// (offset_t - offset_s) has the same integer cell difference as the list
+// entry's periodic shift gives, so the results are bit-identical.  A brick
+// whose box holds more than kBoxMaxRecs records keeps the plain entries and
+// gathers from global memory.
+constexpr int kBoxThreads = 256;
+constexpr int kBoxMaxCells = 8192;  // box cells of a staged brick (16-bit starts in LDS)
+constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
+constexpr int kBoxBits = 6;         // box coordinates packed 6 bits per axis
+constexpr int kBrickCells = 1 << (3 * kBrickBits);
+
+struct BoxArgs {
+  const int32_t* cs;     // cell starts, brick order
+  int32_t reach[3];      // stencil reach of a plain context
+  int64_t* cnt;          // [n_bricks] records of each staged brick (k_box_count)
+  int32_t* maxrec;       // largest staged count (k_box_count)
+  const int64_t* off;    // [n_bricks + 1] first record of each brick
+  uint32_t* slot;        // [records] slot of each record
+  float4* pos;           // [records] offset xyz, packed box cell
+  Ent32* coef;           // entries with record numbers (staged bricks), else as coef32
 };
-constexpr int kTileThreads = 256;
-constexpr int kTileMaxCells = 8192;  // box cells (LDS index table entries)
-constexpr int kTileMaxRecs = 4096;   // staged records
-constexpr int kBoxBits = 6;        // box coordinates packed 6 bits per axis
+
+// Box of one brick: origin (ox, oy, oz) and extent (ex, ey, ez) in cells of
+// the brick's system grid.  The grid fields are copied (no pointer into the
+// kernel arguments, which would force a private copy of them).
+struct BoxGeom {
+  int64_t coff;
+  int n[3], per[3], nb0, nb1;
+  float h[3];
+  int ox, oy, oz, ex, ey, ez, nbc;
+  float inv_ex, inv_ey;
+
+  __device__ __forceinline__ void grid(const Grid& g) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      n[d] = g.n[d];
+      per[d] = g.periodic[d];
+      h[d] = float(g.h[d]);
+    }
+    nb0 = g.nb[0];
+    nb1 = g.nb[1];
+  }
+  // first_slot: any slot of the brick (picks the system of a batch)
+  __device__ __forceinline__ void init(const PairArgs& a, const int32_t* reach, int64_t brick,
+                                       int64_t first_slot) {
+    int rx, ry, rz;
+    if (a.sys) {
+      const SysGrid* S = a.sys + a.sys_slot[first_slot];
+      grid(S->g);
+      coff = S->cell_off;
+      rx = S->reach[0];
+      ry = S->reach[1];
+      rz = S->reach[2];
+    } else {
+      grid(a.g);
+      coff = 0;
+      rx = reach[0];
+      ry = reach[1];
+      rz = reach[2];
+    }
+    const int64_t lb = brick - (coff >> (3 * kBrickBits));
+    const int64_t bxy = int64_t(nb0) * nb1;
+    const int bk = int(lb / bxy);
+    const int64_t rem = lb - int64_t(bk) * bxy;
+    const int bj = int(rem / nb0);
+    const int bi = int(rem - int64_t(bj) * nb0);
+    ox = (bi << kBrickBits) - rx;
+    oy = (bj << kBrickBits) - ry;
+    oz = (bk << kBrickBits) - rz;
+    ex = (1 << kBrickBits) + 2 * rx;
+    ey = (1 << kBrickBits) + 2 * ry;
+    ez = (1 << kBrickBits) + 2 * rz;
+    nbc = ex * ey * ez;
+    inv_ex = 1.0f / float(ex);
+    inv_ey = 1.0f / float(ey);
+  }
+  __device__ __forceinline__ bool fits() const {
+    return nbc <= kBoxMaxCells && ex <= (1 << kBoxBits) && ey <= (1 << kBoxBits) && ez <= (1 << kBoxBits);
+  }
+  __device__ __forceinline__ void xyz(int q, int& x, int& y, int& z) const {
+    const int yz = div_small(q, inv_ex);
+    x = q - yz * ex;
+    z = div_small(yz, inv_ey);
+    y = yz - z * ey;
+  }
+  // brick-order index of grid cell (i, j, k) of the system
+  __device__ __forceinline__ int64_t index(int i, int j, int k) const {
+    return coff + cell_index_nb(nb0, nb1, i, j, k);
+  }
+  // global cell of box cell q; false outside a non-periodic domain
+  __device__ __forceinline__ bool cell(int q, int64_t& c) const {
+    int x, y, z;
+    xyz(q, x, y, z);
+    int X = ox + x, Y = oy + y, Z = oz + z;
+    if (X < 0 || X >= n[0]) {
+      if (!per[0]) return false;
+      X = wrap_axis(X, n[0]);
+    }
+    if (Y < 0 || Y >= n[1]) {
+      if (!per[1]) return false;
+      Y = wrap_axis(Y, n[1]);
+    }
+    if (Z < 0 || Z >= n[2]) {
+      if (!per[2]) return false;
+      Z = wrap_axis(Z, n[2]);
+    }
+    c = index(X, Y, Z);
+    return true;
+  }
+};
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Records of every brick's box: a brick is staged when it has targets, its
+// box fits the packing and it holds at most kBoxMaxRecs records.
+__global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b) {
+  __shared__ int32_t s_w[kBoxThreads / 64];
+  const int64_t brick = blockIdx.x;
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  int total = 0;
+  if (sb < se) {  // uniform over the block
+    BoxGeom B;
+    B.init(a, b.reach, brick, sb);
+    if (B.fits()) {
+      int mine = 0;
+      for (int q = int(threadIdx.x); q < B.nbc; q += kBoxThreads) {
+        int64_t c;
+        if (B.cell(q, c)) mine += b.cs[c + 1] - b.cs[c];
+      }
+      mine = wave_sum(mine);
+      if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = mine;
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < kBoxThreads / 64; ++w) total += s_w[w];
+    }
+  }
+  if (threadIdx.x == 0) {
+    const bool staged = total > 0 && total <= kBoxMaxRecs;
+    b.cnt[brick] = staged ? total : 0;
+    if (staged) atomicMax(b.maxrec, total);
+  }
+}
+
+// Records of staged bricks and the entries re-indexed to record numbers;
+// unstaged bricks copy their entries unchanged.
+__global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b, const Ent32* __restrict__ coef) {
+  __shared__ uint16_t cst[kBoxMaxCells + 1];  // record start of each box cell
+  __shared__ int32_t s_w[kBoxThreads / 64];
+  const int64_t brick = blockIdx.x;
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  if (sb >= se) return;
+  const int64_t r0 = b.off[brick];
+  const int nrec = int(b.off[brick + 1] - r0);
+  auto each_entry = [&](auto&& f) {
+    for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+      const int cnt = a.cnt[s];
+      const int64_t base = a.slice_off[s >> 6] + (s & 63);
+      for (int k = 0; k < cnt; ++k) f(s, base + int64_t(k) * kSlice);
+    }
+  };
+  if (nrec == 0) {
+    each_entry([&](int64_t, int64_t o) { b.coef[o] = coef[o]; });
+    return;
+  }
+  BoxGeom B;
+  B.init(a, b.reach, brick, sb);
+  // (1) counts of the box cells, (2) exclusive scan in place: contiguous
+  // chunks per thread, then across the waves (the total is nrec)
+  const int per = (B.nbc + kBoxThreads - 1) / kBoxThreads;
+  const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
+  int mine = 0;
+  for (int q = q0; q < q1; ++q) {
+    int64_t c;
+    const int k = B.cell(q, c) ? b.cs[c + 1] - b.cs[c] : 0;
+    cst[q] = uint16_t(k);
+    mine += k;
+  }
+  int incl = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if ((threadIdx.x & 63) >= o) incl += y;
+  }
+  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int run = incl - mine;
+  for (int w = 0; w < int(threadIdx.x >> 6); ++w) run += s_w[w];
+  for (int q = q0; q < q1; ++q) {
+    const int k = cst[q];
+    cst[q] = uint16_t(run);
+    run += k;
+  }
+  if (threadIdx.x == 0) cst[B.nbc] = uint16_t(nrec);
+  __syncthreads();
+  // (3) records: record i lies in the last box cell whose start is <= i
+  for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
+    int lo = 0, hi = B.nbc;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (int(cst[mid]) <= i) lo = mid;
+      else hi = mid;
+    }
+    int64_t c = 0;
+    B.cell(lo, c);
+    const int64_t t = int64_t(b.cs[c]) + (i - int(cst[lo]));
+    int x, y, z;
+    B.xyz(lo, x, y, z);
+    const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
+    const float4 p = a.pc32[t];
+    b.slot[r0 + i] = uint32_t(t);
+    b.pos[r0 + i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
+  }
+  // (4) entries: the neighbour's unwrapped cell (its cell plus the entry's
+  // periodic shift) gives its box cell, its rank in the cell the record
+  each_entry([&](int64_t, int64_t o) {
+    const Ent32 c = coef[o];
+    const uint32_t t = c.e & kSlotMask;
+    const uint32_t code = c.e >> kSlotBits;
+    const uint32_t ct = __float_as_uint(a.pc32[t].w);
+    const int ci = int(ct & ((1u << kCellBitsX) - 1));
+    const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+    const int ck = int(ct >> (kCellBitsX + kCellBitsY));
+    const int x = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0] - B.ox;
+    const int y = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1] - B.oy;
+    const int z = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2] - B.oz;
+    int r = 0;
+    if (x >= 0 && x < B.ex && y >= 0 && y < B.ey && z >= 0 && z < B.ez) {
+      const int64_t cell = B.index(ci, cj, ck);
+      r = int(cst[(z * B.ey + y) * B.ex + x]) + int(int64_t(t) - b.cs[cell]);
+    }
+    r = min(max(r, 0), nrec - 1);  // LDS reads stay inside the brick's records
+    b.coef[o] = Ent32{uint32_t(r), c.a, c.bx};
+  });
+}
 
 // pair sum of slot s through global gathers (FP32C, one operand)
 __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const Ent32* __restrict__ coef,
@@ -458,252 +695,127 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   }
 }
 
-// x mod n for x a few periods at most outside [0, n) (no integer division)
-__device__ __forceinline__ int wrap_axis(int x, int n) {
-  while (x < 0) x += n;
-  while (x >= n) x -= n;
-  return x;
-}
-
-// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
-// away from an integer, far more than the FP32 rounding of the product
-__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
-
 template <int EPI>
-__global__ void __launch_bounds__(kTileThreads) k_apply_tile(PairArgs a, TileArgs ta,
-                                                            const Ent32* __restrict__ coef,
-                                                            const float4* __restrict__ xg,
-                                                            const double* __restrict__ xsoa, SlotVec v,
-                                                            ChebEpi epi, const double* __restrict__ u_glob,
-                                                            double* __restrict__ out) {
+__global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b, const float4* __restrict__ xg,
+                                                          const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
+                                                          const double* __restrict__ u_glob,
+                                                          double* __restrict__ out) {
   extern __shared__ float4 lds[];
-  float4* rpos = lds;                 // [rec_cap] offset xyz, packed box cell
-  float4* rval = lds + ta.rec_cap;    // [rec_cap] operand
-  uint16_t* cst = reinterpret_cast<uint16_t*>(lds + 2 * ta.rec_cap);  // [cells_cap + 1] box cell starts
-  __shared__ int32_t s_wsum[kTileThreads / 64];
-  __shared__ int32_t s_total;
-
   const int64_t brick = block_index(a.xcd);
-  const int64_t c0 = brick << (3 * kBrickBits);
-  const int32_t sb = ta.cs[c0], se = ta.cs[c0 + (int64_t(1) << (3 * kBrickBits))];
-  if (sb >= se) return;  // no targets in this brick (uniform over the block)
-  const SysGrid* S = a.sys ? a.sys + a.sys_slot[sb] : nullptr;
-  const Grid& g = S ? S->g : a.g;
-  const int64_t coff = S ? S->cell_off : 0;
-  const StencilRow* rows = S ? ta.rows + S->row_off : ta.rows;
-  const int nrows = S ? S->nrows : ta.nrows;
-  const int rx = S ? S->reach[0] : ta.reach[0];
-  const int ry = S ? S->reach[1] : ta.reach[1];
-  const int rz = S ? S->reach[2] : ta.reach[2];
-  const int64_t lb = brick - (coff >> (3 * kBrickBits));
-  const int64_t bxy = int64_t(g.nb[0]) * g.nb[1];
-  const int bk = int(lb / bxy);
-  const int64_t rem = lb - int64_t(bk) * bxy;
-  const int bj = int(rem / g.nb[0]);
-  const int bi = int(rem - int64_t(bj) * g.nb[0]);
-  const int ox = (bi << kBrickBits) - rx, oy = (bj << kBrickBits) - ry, oz = (bk << kBrickBits) - rz;
-  const int ex = (1 << kBrickBits) + 2 * rx, ey = (1 << kBrickBits) + 2 * ry, ez = (1 << kBrickBits) + 2 * rz;
-  const int nbc = ex * ey * ez;
-  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  const float inv_ex = 1.0f / float(ex), inv_ey = 1.0f / float(ey);
-  auto box_xyz = [&](int q, int& x, int& y, int& z) {
-    const int yz = div_small(q, inv_ex);
-    x = q - yz * ex;
-    z = div_small(yz, inv_ey);
-    y = yz - z * ey;
-  };
-  // global cell of box cell q; false outside a non-periodic domain
-  auto box_cell = [&](int q, int64_t& c) {
-    int x, y, z;
-    box_xyz(q, x, y, z);
-    int X = ox + x, Y = oy + y, Z = oz + z;
-    if (X < 0 || X >= nx) {
-      if (!g.periodic[0]) return false;
-      X = wrap_axis(X, nx);
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  if (sb >= se) return;  // no targets (uniform over the block)
+  const int64_t r0 = b.off[brick];
+  const int nrec = int(b.off[brick + 1] - r0);
+  if (nrec == 0) {
+    for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+      double acc[1][3];
+      fp32c_pairs_global(a, s, b.coef, xg, acc);
+      apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
     }
-    if (Y < 0 || Y >= ny) {
-      if (!g.periodic[1]) return false;
-      Y = wrap_axis(Y, ny);
-    }
-    if (Z < 0 || Z >= nz) {
-      if (!g.periodic[2]) return false;
-      Z = wrap_axis(Z, nz);
-    }
-    c = coff + cell_index(g, X, Y, Z);
-    return true;
-  };
-
-  bool staged = nbc <= ta.cells_cap && ex <= (1 << kBoxBits) && ey <= (1 << kBoxBits) && ez <= (1 << kBoxBits);
-  if (staged) {
-    // (1) particle count of every box cell; each thread's cells are strided
-    // and their cell-start loads are all issued before any is used
-    const int per = (nbc + kTileThreads - 1) / kTileThreads;
-#pragma unroll 6
-    for (int q = int(threadIdx.x); q < nbc; q += kTileThreads) {
-      int64_t c;
-      const int k = box_cell(q, c) ? ta.cs[c + 1] - ta.cs[c] : 0;
-      cst[q] = uint16_t(min(k, 65535));
-    }
-    __syncthreads();
-    // (2) exclusive scan of the counts in place (contiguous chunks per thread)
-    const int q0 = min(nbc, int(threadIdx.x) * per), q1 = min(nbc, q0 + per);
-    int mine = 0;
-    for (int q = q0; q < q1; ++q) mine += cst[q];
-    int incl = mine;
-    for (int off = 1; off < 64; off <<= 1) {
-      const int y = __shfl_up(incl, off, 64);
-      if ((threadIdx.x & 63) >= off) incl += y;
-    }
-    if ((threadIdx.x & 63) == 63) s_wsum[threadIdx.x >> 6] = incl;
-    __syncthreads();
-    int before = 0;
-    for (int w = 0; w < int(threadIdx.x >> 6); ++w) before += s_wsum[w];
-    if (threadIdx.x == kTileThreads - 1) s_total = before + incl;
-    __syncthreads();
-    const int total = s_total;
-    staged = total <= ta.rec_cap;  // uniform
-    if (staged) {
-      int run = before + incl - mine;
-      for (int q = q0; q < q1; ++q) {
-        const int k = cst[q];
-        cst[q] = uint16_t(run);
-        run += k;
-      }
-      if (threadIdx.x == kTileThreads - 1) cst[nbc] = uint16_t(total);
-    }
-    __syncthreads();
-    if (staged) {
-      // (3) records: record i of box cell q (binary search of the starts),
-      // the cell's first slot, then the record itself
-#pragma unroll 4
-      for (int i = int(threadIdx.x); i < total; i += kTileThreads) {
-        int lo = 0, hi = nbc;  // largest q with cst[q] <= i
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (int(cst[mid]) <= i) lo = mid;
-          else hi = mid;
-        }
-        int64_t c = 0;
-        box_cell(lo, c);
-        const int64_t t = int64_t(ta.cs[c]) + (i - int(cst[lo]));
-        int x, y, z;
-        box_xyz(lo, x, y, z);
-        const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
-        const float4 p = a.pc32[t];
-        rpos[i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
-        rval[i] = xg[t];
-      }
-    }
-    __syncthreads();
+    return;
   }
-
-  const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
-  for (int64_t s = sb + threadIdx.x; s < se; s += kTileThreads) {
+  float4* rpos = lds;         // [nrec] offset xyz, packed box cell
+  float4* rval = lds + nrec;  // [nrec] operand
+  {
+    // four records in flight per thread, then the remainder one at a time
+    int i = int(threadIdx.x);
+    for (; i + 3 * kBoxThreads < nrec; i += 4 * kBoxThreads) {
+      const uint32_t t0 = b.slot[r0 + i], t1 = b.slot[r0 + i + kBoxThreads];
+      const uint32_t t2 = b.slot[r0 + i + 2 * kBoxThreads], t3 = b.slot[r0 + i + 3 * kBoxThreads];
+      const float4 p0 = b.pos[r0 + i], p1 = b.pos[r0 + i + kBoxThreads];
+      const float4 p2 = b.pos[r0 + i + 2 * kBoxThreads], p3 = b.pos[r0 + i + 3 * kBoxThreads];
+      const float4 f0 = xg[t0], f1 = xg[t1], f2 = xg[t2], f3 = xg[t3];
+      rpos[i] = p0;
+      rpos[i + kBoxThreads] = p1;
+      rpos[i + 2 * kBoxThreads] = p2;
+      rpos[i + 3 * kBoxThreads] = p3;
+      rval[i] = f0;
+      rval[i + kBoxThreads] = f1;
+      rval[i + 2 * kBoxThreads] = f2;
+      rval[i + 3 * kBoxThreads] = f3;
+    }
+    for (; i < nrec; i += kBoxThreads) {
+      rpos[i] = b.pos[r0 + i];
+      rval[i] = xg[b.slot[r0 + i]];
+    }
+  }
+  __syncthreads();
+  BoxGeom B;
+  B.init(a, b.reach, brick, sb);
+  const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
+  constexpr int U = 8;  // entries per group, the next group loaded one ahead
+  for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+    const float4 po = a.pc32[s];
+    const uint32_t co = __float_as_uint(po.w);
+    const int bx = int(co & ((1u << kCellBitsX) - 1)) - B.ox;
+    const int by = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1)) - B.oy;
+    const int bz = int(co >> (kCellBitsX + kCellBitsY)) - B.oz;
+    const int cnt = a.cnt[s];
+    float ps[3] = {0.f, 0.f, 0.f};
+    KahanF kh[3];
+    auto pair = [&](const Ent32& c) {
+      const float4 pt = rpos[c.e];
+      const float4 f = rval[c.e];
+      const uint32_t bt = __float_as_uint(pt.w);
+      const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
+      const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
+      const int dk = int(bt >> (2 * kBoxBits)) - bz;
+      const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+      const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+      const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+      const float bb = c.bx * (dx * f.x + dy * f.y + dz * f.z);
+      ps[0] += c.a * f.x + bb * dx;
+      ps[1] += c.a * f.y + bb * dy;
+      ps[2] += c.a * f.z + bb * dz;
+    };
+    auto merge = [&](int k) {
+      if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          kh[q].add(ps[q]);
+          ps[q] = 0.f;
+        }
+      }
+    };
+    if (cnt > 0) {  // an empty list has no storage
+      const Ent32* __restrict__ cf = b.coef + a.slice_off[s >> 6] + (s & 63);
+      const int last = cnt - 1;
+      Ent32 en[U];
+#pragma unroll
+      for (int i = 0; i < U; ++i) en[i] = cf[int64_t(min(i, last)) * kSlice];
+      int k = 0;
+      for (; k + U <= cnt; k += U) {
+        Ent32 c[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+          c[i] = en[i];
+          en[i] = cf[int64_t(min(k + U + i, last)) * kSlice];
+        }
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+          pair(c[i]);
+          merge(k + i);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < U - 1; ++i) {
+        if (k + i < cnt) {  // entries en[0 .. cnt - k)
+          pair(en[i]);
+          merge(k + i);
+        }
+      }
+    }
     double acc[1][3];
-    if (!staged) {
-      fp32c_pairs_global(a, s, coef, xg, acc);
-    } else {
-      const float4 po = a.pc32[s];
-      const uint32_t co = __float_as_uint(po.w);
-      const int io = int(co & ((1u << kCellBitsX) - 1));
-      const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
-      const int ko = int(co >> (kCellBitsX + kCellBitsY));
-      const int bx = io - ox, by = jo - oy, bz = ko - oz;  // own box cell
-      const int64_t cown = coff + cell_index(g, io, jo, ko);
-      const int selfq = int(cst[(bz * ey + by) * ex + bx]) + int(s - ta.cs[cown]);
-      const int cnt = a.cnt[s];
-      const int last = cnt > 0 ? cnt - 1 : 0;
-      const Ent32* cf = coef + a.slice_off[s >> 6] + (s & 63);
-      // tensors of the next entries in flight (list order); an empty list
-      // has no storage
-      const Ent32 zero{0u, 0.f, 0.f};
-      Ent32 c0 = cnt > 0 ? cf[0] : zero;
-      Ent32 c1 = cnt > 0 ? cf[int64_t(min(1, last)) * kSlice] : zero;
-      Ent32 c2 = cnt > 0 ? cf[int64_t(min(2, last)) * kSlice] : zero;
-      float ps[3] = {0.f, 0.f, 0.f};
-      KahanF kh[3];
-      // Cursor over the list in list order: the record range of the current
-      // stencil row (a row wrapped past the high x face has two ranges, the
-      // wrapped one first).  The loop below steps k uniformly across the
-      // wave, so the tensor loads stay coalesced as in k_apply.
-      int r = -1, q = 0, qend = 0, q2 = 0, qend2 = 0;
-      bool has2 = false;
-      auto advance = [&]() {
-        for (;;) {
-          if (q < qend) {
-            if (q != selfq) return;
-            ++q;
-            continue;
-          }
-          if (has2) {
-            q = q2;
-            qend = qend2;
-            has2 = false;
-            continue;
-          }
-          if (++r >= nrows) return;
-          const StencilRow R = rows[r];
-          const int J = jo + R.dj, K = ko + R.dk;
-          if (!g.periodic[1] && (J < 0 || J >= ny)) continue;
-          if (!g.periodic[2] && (K < 0 || K >= nz)) continue;
-          int X0 = io + R.di_lo, X1 = io + R.di_hi;
-          if (!g.periodic[0]) {
-            X0 = max(X0, 0);
-            X1 = min(X1, nx - 1);
-          }
-          if (X0 > X1) continue;
-          const int rb = ((K - oz) * ey + (J - oy)) * ex - ox;
-          if (X1 >= nx) {  // the part wrapped past the high x face comes first
-            q = cst[rb + nx];
-            qend = cst[rb + X1 + 1];
-            q2 = cst[rb + X0];
-            qend2 = cst[rb + nx];
-            has2 = true;
-          } else {
-            q = cst[rb + X0];
-            qend = cst[rb + X1 + 1];
-          }
-        }
-      };
-      advance();
-      const int qmax = s_total - 1;  // staged records (a consistent walk never reaches it)
-      for (int k = 0; k < cnt; ++k) {
-        const int qq = min(q++, qmax);
-        const Ent32 c = c0;
-        c0 = c1;
-        c1 = c2;
-        c2 = cf[int64_t(min(k + 3, last)) * kSlice];
-        const float4 pt = rpos[qq];
-        const float4 f = rval[qq];
-        advance();
-        const uint32_t bt = __float_as_uint(pt.w);
-        const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
-        const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
-        const int dk = int(bt >> (2 * kBoxBits)) - bz;
-        const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
-        const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
-        const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-        const float b = c.bx * (dx * f.x + dy * f.y + dz * f.z);
-        ps[0] += c.a * f.x + b * dx;
-        ps[1] += c.a * f.y + b * dy;
-        ps[2] += c.a * f.z + b * dz;
-        if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
-          for (int u = 0; u < 3; ++u) {
-            kh[u].add(ps[u]);
-            ps[u] = 0.f;
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        kh[u].add(ps[u]);
-        acc[0][u] = kh[u].value();
-      }
+    for (int q = 0; q < 3; ++q) {
+      kh[q].add(ps[q]);
+      acc[0][q] = kh[q].value();
     }
     apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
   }
 }
+
 
 // Field points: sum over the element's particle list, own element
 // included, no exclusion (ggem_stokes.C:396-420), full minimum image.
@@ -800,73 +912,49 @@ void launch_kb(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float
                        ctx->coef32, g64, g32, soa, v, epi, u_glob, out);
 }
 
-// LDS layout of the tile kernel, or false when the context cannot use it
-// (explicit minimum image, or boxes too large for 64 KiB of LDS)
-bool tile_args(const ggem_ctx* ctx, TileArgs* ta, size_t* lds) {
-  if (ctx->exact_minimg || ctx->n == 0) return false;
-  int cells = 0;
-  double recs = 0.0;
-  auto one = [&](const int* reach, int64_t n_elem, int64_t count) {
-    const int box = ((1 << kBrickBits) + 2 * reach[0]) * ((1 << kBrickBits) + 2 * reach[1]) *
-                    ((1 << kBrickBits) + 2 * reach[2]);
-    cells = std::max(cells, box);
-    // capacity: the expected box population plus 4.5 standard deviations of
-    // a Poisson count (bricks above it take the global-gather path)
-    const double expect = double(count) / double(std::max<int64_t>(n_elem, 1)) * box;
-    recs = std::max(recs, expect + 4.5 * std::sqrt(expect) + 16.0);
-  };
-  if (ctx->nsys > 1) {
-    for (int k = 0; k < ctx->nsys; ++k)
-      one(ctx->sys_h[k].reach, ctx->sys_h[k].g.n_elem, ctx->part_off[k + 1] - ctx->part_off[k]);
-  } else {
-    one(ctx->reach, ctx->grid.n_elem, ctx->n);
-  }
-  if (cells > kTileMaxCells) return false;
-  const size_t budget = 65536;
-  const size_t cell_bytes = (size_t(cells + 1) * sizeof(uint16_t) + 15) / 16 * 16;
-  if (cell_bytes + 64 * 32 > budget) return false;
-  const int rec_cap = int(std::min<double>(std::min<double>(recs, double((budget - cell_bytes) / 32)),
-                                           double(kTileMaxRecs)));
-  ta->rows = ctx->rows_d;
-  ta->cs = ctx->cell_start;
-  ta->nrows = ctx->n_rows;
-  for (int d = 0; d < 3; ++d) ta->reach[d] = ctx->reach[d];
-  ta->cells_cap = cells;
-  ta->rec_cap = rec_cap;
-  *lds = size_t(rec_cap) * 32 + cell_bytes;
-  return true;
+BoxArgs box_args(const ggem_ctx* ctx) {
+  BoxArgs b;
+  b.cs = ctx->cell_start;
+  for (int d = 0; d < 3; ++d) b.reach[d] = ctx->reach[d];
+  b.cnt = ctx->box_cnt;
+  b.maxrec = ctx->box_max_d;
+  b.off = ctx->box_off;
+  b.slot = ctx->box_slot;
+  b.pos = ctx->box_pos;
+  b.coef = ctx->box_coef;
+  return b;
 }
 
+unsigned n_bricks(const ggem_ctx* ctx) { return unsigned(ctx->grid.n_cells >> (3 * kBrickBits)); }
+
 template <int EPI>
-hipError_t launch_tile(ggem_ctx* ctx, PairArgs a, const float4* xg, const double* soa, const SlotVec& v,
-                       const ChebEpi& epi, const double* u_glob, double* out) {
-  TileArgs ta;
-  size_t lds = 0;
-  if (!tile_args(ctx, &ta, &lds)) return hipErrorInvalidConfiguration;
+hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double* soa, const SlotVec& v,
+                      const ChebEpi& epi, const double* u_glob, double* out) {
+  if (!ctx->box_valid || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs) return hipErrorInvalidConfiguration;
   a.xcd = 1;
-  const unsigned nb = unsigned(ctx->grid.n_cells >> (3 * kBrickBits));
-  hipLaunchKernelGGL((k_apply_tile<EPI>), dim3(nb), dim3(kTileThreads), lds, ctx->stream, a, ta, ctx->coef32, xg,
-                     soa, v, epi, u_glob, out);
+  const size_t lds = size_t(ctx->box_max) * 2 * sizeof(float4);
+  hipLaunchKernelGGL((k_apply_box<EPI>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
+                     box_args(ctx), xg, soa, v, epi, u_glob, out);
   return hipGetLastError();
 }
 
-// candidate index of the tile kernel; the default shape of an apply is the
-// global-gather kernel (measured faster on the 1M-bead workload), the tile
-// kernel is one more candidate for the autotuner where it applies
-constexpr int kTileShape = kNApply;
-int apply_default(const ggem_ctx*, int, int) { return 0; }
+// candidate index of the staged-record kernel (FP32C, one operand, after
+// the block shapes of k_apply); it is the default where some brick is staged
+constexpr int kBoxShape = kNApply;
+bool box_ready(const ggem_ctx* ctx, int precision, int nr) {
+  return precision == GGEM_FP32C && nr == 1 && ctx->box_valid && ctx->box_max > 0;
+}
+int apply_default(const ggem_ctx* ctx, int precision, int nr) { return box_ready(ctx, precision, nr) ? kBoxShape : 0; }
 int apply_candidates(const ggem_ctx* ctx, int precision, int nr) {
-  TileArgs ta;
-  size_t lds;
-  return precision == GGEM_FP32C && nr == 1 && tile_args(ctx, &ta, &lds) ? kNApply + 1 : kNApply;
+  return box_ready(ctx, precision, nr) ? kNApply + 1 : kNApply;
 }
 
 // one apply launch in candidate shape c
 template <int P, int NR, int EPI>
 hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const float4* g32, const double* soa,
                     const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
-  if (c == kTileShape) {
-    if constexpr (P == GGEM_FP32C && NR == 1) return launch_tile<EPI>(ctx, a, g32, soa, v, epi, u_glob, out);
+  if (c == kBoxShape) {
+    if constexpr (P == GGEM_FP32C && NR == 1) return launch_box<EPI>(ctx, a, g32, soa, v, epi, u_glob, out);
     return hipErrorInvalidConfiguration;
   }
   if (c < 0 || c >= kNApply) return hipErrorInvalidValue;
@@ -930,6 +1018,24 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
   const int c = tune_pick(ctx, tune_key("tensors", precision, ctx->nsys > 1, ctx->n), kNTensor, run);
   ProfScope ps(ctx, precision == GGEM_FP64 ? "tensors_fp64" : "tensors_fp32c");
   return run(c);
+}
+
+bool box_supported(const ggem_ctx* ctx) { return ctx->n > 0 && !ctx->exact_minimg; }
+
+hipError_t launch_box_count(ggem_ctx* ctx) {
+  hipError_t e = hipMemsetAsync(ctx->box_max_d, 0, sizeof(int32_t), ctx->stream);
+  if (e != hipSuccess) return e;
+  ProfScope ps(ctx, "box_count");
+  hipLaunchKernelGGL(k_box_count, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx),
+                     box_args(ctx));
+  return hipGetLastError();
+}
+
+hipError_t launch_box_fill(ggem_ctx* ctx) {
+  ProfScope ps(ctx, "box_fill");
+  hipLaunchKernelGGL(k_box_fill, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx),
+                     box_args(ctx), ctx->coef32);
+  return hipGetLastError();
 }
 
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob, double* out) {

@@ -4,8 +4,12 @@ element, same arithmetic), and every tunable kernel gets a pick."""
 import numpy as np
 import pytest
 
+import math
+
 import copss_ggem as G
+import oracle as O
 import workloads as W
+from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -66,34 +70,40 @@ def test_autotune_results_identical_and_picks_recorded():
         ctx.close()
 
 
-def test_tile_kernel_pinned_matches_default():
-    """The LDS-staged brick kernel (an autotuner candidate for FP32C single
-    right-hand sides) gives the default kernel's results bit for bit, on a
-    plain and on a batched context."""
-    import math
-    TILE = 5  # candidate index of the tile kernel (after the five block shapes)
+BOX = 5  # candidate index of the staged-record kernel (after the five block shapes)
 
-    def pin(ctx, n):
-        lg = int(math.floor(math.log2(n)))
-        for key in ("apply/fp32c/1/%d" % lg, "pair_beads/fp32c/0/%d" % lg, "pair_beads/fp32c/1/%d" % lg):
-            ctx.tune_set(key, TILE)
-        ctx.set_autotune(True)
 
+def pin(ctx, n, cand):
+    lg = int(math.floor(math.log2(n)))
+    for key in ("apply/fp32c/1/%d" % lg, "pair_beads/fp32c/0/%d" % lg, "pair_beads/fp32c/1/%d" % lg):
+        ctx.tune_set(key, cand)
+    ctx.set_autotune(True)
+
+
+def fp32c_results(ctx, f, dw):
+    u = ctx.local_velocity_beads()
+    U = ctx.mobility_apply(f)
+    y, d, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+    return u, U, y, d
+
+
+def test_box_kernel_matches_global_gather_kernel():
+    """The staged-record kernel (the FP32C default for one right-hand side)
+    gives the global-gather kernel's results bit for bit, pinned either way,
+    on a plain and on a batched context."""
     cfg, x, f = W.make_inputs("beads20k")
     cfg["precision"] = G.FP32C
     dw = G.normal(3 * len(x), 23).reshape(1, -1, 3)
     ctx = G.Context(cfg)
     try:
         ctx.reinit(x, f)
-        u0 = ctx.local_velocity_beads()
-        U0 = ctx.mobility_apply(f)
-        y0, d0, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
-        pin(ctx, len(x))
-        u1 = ctx.local_velocity_beads()
-        U1 = ctx.mobility_apply(f)
-        y1, d1, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
-        assert np.array_equal(u0, u1) and np.array_equal(U0, U1)
-        assert np.array_equal(y0, y1) and np.array_equal(d0, d1)
+        r0 = fp32c_results(ctx, f, dw)
+        pin(ctx, len(x), 0)
+        r1 = fp32c_results(ctx, f, dw)
+        pin(ctx, len(x), BOX)
+        r2 = fp32c_results(ctx, f, dw)
+        for a, b, c in zip(r0, r1, r2):
+            assert np.array_equal(a, b) and np.array_equal(a, c)
     finally:
         ctx.close()
     cfgs, xs, fs, dwb = W.make_batch(nsys=4, nrhs=1, seed=12, n_lo=400, n_hi=2000)
@@ -104,8 +114,40 @@ def test_tile_kernel_pinned_matches_default():
         b.set_particles(xs, fs)
         b.build_neighbors()
         u0 = b.local_velocity_beads()
-        pin(b, b.n)
+        pin(b, b.n, 0)
         u1 = b.local_velocity_beads()
         assert np.array_equal(u0, u1)
     finally:
         b.close()
+
+
+def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks():
+    """A dense cluster puts more records in some bricks' boxes than LDS
+    holds: those bricks gather from global memory, the rest read staged
+    records, and the results equal the global-gather kernel's bit for bit
+    and the oracle's to the FP32C bound."""
+    cfg, x, f = W.make_inputs("beads20k")
+    cfg["precision"] = G.FP32C
+    lo = np.array(cfg["box_min"], float)
+    hi = np.array(cfg["box_max"], float)
+    rng = np.random.default_rng(5)
+    c = lo + 0.5 * (hi - lo)
+    # 6000 particles within a few cells of the centre, the rest uniform
+    x = x.copy()
+    x[:6000] = c + rng.uniform(-6.0, 6.0, size=(6000, 3))
+    dw = G.normal(3 * len(x), 29).reshape(1, -1, 3)
+    ctx = G.Context(cfg)
+    try:
+        ctx.reinit(x, f)
+        r0 = fp32c_results(ctx, f, dw)
+        pin(ctx, len(x), 0)
+        r1 = fp32c_results(ctx, f, dw)
+        for a, b in zip(r0, r1):
+            assert np.array_equal(a, b)
+    finally:
+        ctx.close()
+    cfg64 = dict(cfg, precision=G.FP64)
+    m = O.Mesh(cfg64)
+    m.reinit(x, f)
+    u_ref = m.local_velocity_beads()
+    assert rel_l2(r0[0], u_ref) < 1e-6
