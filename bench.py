@@ -49,12 +49,15 @@ def parse():
     return ap.parse_args()
 
 
-# Algorithmic HBM bytes of one FP32C Chebyshev application (k_apply, one
-# right-hand side): per ordered pair the 4-byte list entry and the 8-byte
-# pair tensor (A, Bx); per slot the operand and position records read once
+# Algorithmic HBM bytes of one FP32C Chebyshev application (k_apply_box,
+# one right-hand side): per ordered pair the 2-byte neighbour record number
+# and the 8-byte pair tensor (A, Bx); per slot the operand and position
+# records read once
 # (16 + 16), x, x_{k-1} (24 each), y read and written (48), v and its
-# record written (24 + 16), the list length (4).
-B_PAIR_APPLY32 = 4 + 8
+# record written (24 + 16), the list length (4).  The staged neighbour
+# records are read once per brick box (~5x per slot at 1M beads): traffic
+# above the algorithmic bytes.
+B_PAIR_APPLY32 = 2 + 8
 B_SLOT_APPLY = 16 + 16 + 24 + 24 + 48 + 24 + 16 + 4
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_beads1m.txt")
 
@@ -208,8 +211,9 @@ def main():
         s_ms, s_n = ctx.profile_get(sname)
         l_ms, l_n = ctx.profile_get(lname)
         per_step = {}
+        tname = "tensors_fp32c" if prec == G.FP32C else "tensors_fp64"
         for nm in ("bin", "scan", "place_sort", "gather_forces", "count_lists", "scan64", "fill_ell",
-                   "elist_count", "elist_fill", lname, sname, kname, "axpby"):
+                   "elist_count", "elist_fill", tname, "box_count", "box_fill", lname, sname, kname, "axpby"):
             t_ms, _ = ctx.profile_get(nm)
             if t_ms > 0:
                 per_step[nm] = t_ms / args.steps
@@ -256,10 +260,10 @@ def main():
         "pair_evaluations_per_step": evals,
         "pair_kernel_interactions_per_s": pairs / apply_s,
         "graph": {"ms_per_step": r32["ms_graph"], "brownian_steps_per_s": 1e3 / r32["ms_graph"]},
-        "roofline": {"bound": "hbm", "kernel": "k_apply FP32C Chebyshev step (pair sum + recurrence update)",
+        "roofline": {"bound": "hbm", "kernel": "k_apply_box FP32C Chebyshev step (pair sum + recurrence update)",
                      "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM,
-                     "traffic": profiled_traffic("k_apply<1, 1, 2, 256>") if args.workload == "beads1m" else None,
+                     "traffic": profiled_traffic("k_apply_box<2>") if args.workload == "beads1m" else None,
                      "algorithmic": "%d B x %d pairs + %d B x %d slots per launch"
                                     % (B_PAIR_APPLY32, pairs, B_SLOT_APPLY, n),
                      "valu_tflops": F_PAIR * pairs / apply_s / 1e12,

@@ -158,37 +158,49 @@ int prepare_box(ggem_ctx* ctx) {
   if (total <= 0 || mx <= 0) return GGEM_OK;  // no brick staged: the plain kernel serves
   if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total)))) return rc;
   if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total)))) return rc;
-  if ((rc = ensure(ctx, &ctx->box_coef, &ctx->box_coef_cap, size_t(std::max<int64_t>(ctx->ell_len, 1)))))
-    return rc;
-  if ((e = launch_box_fill(ctx)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  const size_t ne = size_t(std::max<int64_t>(ctx->ell_len, 1));
+  if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
+  if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
+  if ((e = launch_box_fill(ctx, nullptr)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
   ctx->box_max = mx;
   ctx->box_valid = true;
   return GGEM_OK;
 }
 
-int ensure_tensors(ggem_ctx* ctx) {
-  const int prec = ctx->p.precision;
-  if (prec == GGEM_FP64 ? ctx->coef64_valid : ctx->coef32_valid) return GGEM_OK;
-  const size_t len = size_t(std::max<int64_t>(ctx->ell_len, 1));
+int ensure_coef32(ggem_ctx* ctx) {
+  size_t cap = size_t(ctx->coef32_cap);
+  int rc = ensure(ctx, &ctx->coef32, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1)));
+  ctx->coef32_cap = int64_t(cap);
+  return rc;
+}
+
+int ensure_tensors(ggem_ctx* ctx, int nrhs) {
   int rc;
-  if (prec == GGEM_FP64) {
+  if (ctx->p.precision == GGEM_FP64) {
+    if (ctx->coef64_valid) return GGEM_OK;
     size_t cap = size_t(ctx->coef64_cap);
-    rc = ensure(ctx, &ctx->coef64, &cap, len);
+    if ((rc = ensure(ctx, &ctx->coef64, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1))))) return rc;
     ctx->coef64_cap = int64_t(cap);
-  } else {
-    size_t cap = size_t(ctx->coef32_cap);
-    rc = ensure(ctx, &ctx->coef32, &cap, len);
-    ctx->coef32_cap = int64_t(cap);
-  }
-  if (rc) return rc;
-  hipError_t e = launch_pair_tensors(ctx, prec);
-  if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
-  if (prec == GGEM_FP64) {
+    hipError_t e = launch_pair_tensors(ctx, GGEM_FP64);
+    if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
     ctx->coef64_valid = true;
-  } else {
-    if ((rc = prepare_box(ctx))) return rc;
-    ctx->coef32_valid = true;
+    return GGEM_OK;
   }
+  // FP32C: the staged-brick records serve one right-hand side; the plain
+  // list-order tensors serve several, the autotuner's other candidates and
+  // contexts without staged bricks
+  if (!ctx->box_built) {
+    if ((rc = prepare_box(ctx))) return rc;
+    ctx->box_built = true;
+  }
+  const bool global = nrhs > 1 || ctx->tune.on || !ctx->box_valid;
+  if (!global || ctx->coef32_valid) return GGEM_OK;
+  if ((rc = ensure_coef32(ctx))) return rc;
+  // with staged bricks the list-order records come from the kernel that
+  // wrote the staged ones (bit-identical tensors)
+  hipError_t e = ctx->box_valid ? launch_box_fill(ctx, ctx->coef32) : launch_pair_tensors(ctx, GGEM_FP32C);
+  if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
+  ctx->coef32_valid = true;
   return GGEM_OK;
 }
 
@@ -360,7 +372,8 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->box_cnt);
   f(ctx->box_slot);
   f(ctx->box_pos);
-  f(ctx->box_coef);
+  f(ctx->box_ab);
+  f(ctx->box_idx);
   f(ctx->box_max_d);
   f(ctx->pc32);
   f(ctx->coef64);
@@ -726,6 +739,7 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
   ctx->built = false;
   ctx->elist_valid = false;
   ctx->coef32_valid = ctx->coef64_valid = false;
+  ctx->box_built = false;
   ctx->pairs = -1;
   if (n > 0) {
     int rc = h2d(ctx, ctx->x_in, xyz, sizeof(double) * 3 * n);
@@ -761,6 +775,7 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   ctx->built = false;
   ctx->elist_valid = false;
   ctx->coef32_valid = ctx->coef64_valid = false;
+  ctx->box_built = false;
   hipError_t e = launch_bin(ctx, hyst);
   if (e != hipSuccess) return hip_fail(ctx, e, "bin");
   if ((e = launch_scan(ctx)) != hipSuccess) return hip_fail(ctx, e, "scan");

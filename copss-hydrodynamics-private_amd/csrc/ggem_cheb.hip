@@ -200,7 +200,7 @@ int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
       uintptr_t(ctx->x4), uintptr_t(ctx->nbr_cnt), uintptr_t(ctx->slice_off), uintptr_t(ctx->ell),
       uintptr_t(ctx->perm), uintptr_t(ctx->fs), uintptr_t(ctx->coef32), uintptr_t(ctx->coef64),
       uintptr_t(ctx->box_valid), uintptr_t(ctx->box_max), uintptr_t(ctx->box_cnt), uintptr_t(ctx->box_slot),
-      uintptr_t(ctx->box_pos), uintptr_t(ctx->box_coef), uintptr_t(ctx->cell_start),
+      uintptr_t(ctx->box_pos), uintptr_t(ctx->box_ab), uintptr_t(ctx->box_idx), uintptr_t(ctx->cell_start),
       uintptr_t(s.sys), uintptr_t(ctx->nsys)};
   if (!ctx->cheb_exec || key != ctx->cheb_key) {
     if (ctx->cheb_exec) {
@@ -296,7 +296,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
   cheb_coefficients(N, da, db, &X);
 
   if ((rc = reserve(ctx, 4, nrhs))) return rc;
-  if (!op && (rc = ensure_tensors(ctx))) return rc;
+  if (!op && (rc = ensure_tensors(ctx, nrhs))) return rc;
   Pools& P = pools(ctx);
   const int prec = ctx->p.precision;
   SlotVec x0 = slot_vec(P, 0, nrhs, n, prec);
@@ -495,7 +495,7 @@ extern "C" int ggem_batch_cheb_brownian(ggem_ctx* ctx, int32_t N, const double* 
     std::copy(X.begin(), X.end(), pk + 2);
   }
   if ((rc = reserve(ctx, 4, nrhs))) return rc;
-  if ((rc = ensure_tensors(ctx))) return rc;
+  if ((rc = ensure_tensors(ctx, nrhs))) return rc;
   Pools& P = pools(ctx);
   const int prec = ctx->p.precision;
   const SlotVec x0 = slot_vec(P, 0, nrhs, n, prec), x1 = slot_vec(P, 1, nrhs, n, prec);

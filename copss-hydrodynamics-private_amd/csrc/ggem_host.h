@@ -53,8 +53,8 @@ struct Pools {
 inline Pools& pools(ggem_ctx* ctx) { return *reinterpret_cast<Pools*>(ctx->scratch); }
 
 // Pair tensors of the current lists in the context's precision, evaluated
-// on first use after a neighbour build.
-int ensure_tensors(ggem_ctx* ctx);
+// on first use after a neighbour build, for applies of up to nrhs operands.
+int ensure_tensors(ggem_ctx* ctx, int nrhs = 1);
 
 inline int check_ready(ggem_ctx* ctx) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");

@@ -180,11 +180,14 @@ struct ggem_ctx {
   size_t box_rec_cap = 0;
   float4* box_pos = nullptr;      // [records] cell-relative FP32 position, packed box cell
   size_t box_pos_cap = 0;
-  ggem::Ent32* box_coef = nullptr;  // coef32 with the record number as index (staged bricks)
-  size_t box_coef_cap = 0;
+  float2* box_ab = nullptr;       // [ell] pair tensor (A, Bx) of every entry, list layout
+  size_t box_ab_cap = 0;
+  uint16_t* box_idx = nullptr;    // [ell] the neighbour's record number (staged bricks)
+  size_t box_idx_cap = 0;
   int32_t* box_max_d = nullptr;   // largest record count of a staged brick
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
-  bool box_valid = false;
+  bool box_built = false;         // box_* describe the current lists
+  bool box_valid = false;         // ... and some brick is staged
   int bsign = 1;  // sign of Bx: +1 when xi > alpha
 
   // element particle lists (spreading), CSR over elements
@@ -315,8 +318,10 @@ struct ChebEpi {
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
 // FP32C staged records: box_cnt (and box_max_d, cleared first) per brick
 hipError_t launch_box_count(ggem_ctx* ctx);
-// records and re-indexed entries, with box_off scanned from box_cnt
-hipError_t launch_box_fill(ggem_ctx* ctx);
+// records, record numbers and pair tensors of every brick (box_idx,
+// box_ab), with box_off scanned from box_cnt; the list-order records into
+// plain when set
+hipError_t launch_box_fill(ggem_ctx* ctx, ggem::Ent32* plain);
 // whether the context's FP32C lists can be staged per brick
 bool box_supported(const ggem_ctx* ctx);
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,

@@ -401,15 +401,16 @@ __device__ __forceinline__ int div_small(int q, float inv_d) { return int((float
 // the box is one record, in box order: box cells x-major through the box,
 // particles of a cell in slot order.  Once per list build k_box_fill writes
 // every brick's records (slot, cell-relative FP32 position with the packed
-// box cell) and a copy of the FP32C entries whose index field is the
-// neighbour's record number in its target's box.  k_apply_box copies the
-// records and their operands to LDS and streams the entries as k_apply
-// does, reading the neighbour from LDS.  The summation order (list order)
-// and the arithmetic are k_apply's; the displacement (box_t - box_s) h +
-// (offset_t - offset_s) has the same integer cell difference as the list
-// entry's periodic shift gives, so the results are bit-identical.  A brick
-// whose box holds more than kBoxMaxRecs records keeps the plain entries and
-// gathers from global memory.
+// box cell), the pair tensor (A, Bx) of every list entry and, in staged
+// bricks, the neighbour's record number in its target's box (16 bits).
+// k_apply_box copies the records and their operands to LDS and streams
+// record numbers and tensors in list order (10 bytes per pair), reading the
+// neighbour from LDS.  The summation order (list order) and the arithmetic
+// are k_apply's; the displacement (box_t - box_s) h + (offset_t - offset_s)
+// has the same integer cell difference as the list entry's periodic shift
+// gives, so the results are bit-identical.  A brick whose box holds more
+// than kBoxMaxRecs records gathers from global memory (list entries and
+// the same tensors).
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxCells = 8192;  // box cells of a staged brick (16-bit starts in LDS)
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
@@ -424,7 +425,9 @@ struct BoxArgs {
   const int64_t* off;    // [n_bricks + 1] first record of each brick
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
-  Ent32* coef;           // entries with record numbers (staged bricks), else as coef32
+  float2* ab;            // [ell] pair tensor (A, Bx) of every entry, list layout
+  uint16_t* idx;         // [ell] neighbour's record number (staged bricks)
+  Ent32* plain;          // k_box_fill: the list-order records (coef32) too, when set
 };
 
 // Box of one brick: origin (ox, oy, oz) and extent (ex, ey, ez) in cells of
@@ -553,10 +556,18 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
   }
 }
 
-// Records of staged bricks and the entries re-indexed to record numbers;
-// unstaged bricks copy their entries unchanged.
-__global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b, const Ent32* __restrict__ coef) {
+// Records of staged bricks, and the FP32C pair records of every brick's
+// targets: the tensor (A, Bx) as k_tensors evaluates it (FP64 displacement
+// with the entry's periodic shift, then green_coef_f32), with the
+// neighbour's record number as index in staged bricks and the plain list
+// entry otherwise.  The list-order records of the global-gather kernels are
+// written from the same registers when requested, so that both kernels use
+// bit-identical tensors (two inlined copies of the evaluation may contract
+// their products differently).
+template <bool BATCH>
+__global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b) {
   __shared__ uint16_t cst[kBoxMaxCells + 1];  // record start of each box cell
+  __shared__ int32_t cfirst[kBoxMaxCells];    // first slot of each box cell
   __shared__ int32_t s_w[kBoxThreads / 64];
   const int64_t brick = blockIdx.x;
   const int64_t c0 = brick * kBrickCells;
@@ -564,93 +575,114 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b,
   if (sb >= se) return;
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
-  auto each_entry = [&](auto&& f) {
-    for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
-      const int cnt = a.cnt[s];
-      const int64_t base = a.slice_off[s >> 6] + (s & 63);
-      for (int k = 0; k < cnt; ++k) f(s, base + int64_t(k) * kSlice);
-    }
-  };
-  if (nrec == 0) {
-    each_entry([&](int64_t, int64_t o) { b.coef[o] = coef[o]; });
-    return;
-  }
   BoxGeom B;
   B.init(a, b.reach, brick, sb);
-  // (1) counts of the box cells, (2) exclusive scan in place: contiguous
-  // chunks per thread, then across the waves (the total is nrec)
-  const int per = (B.nbc + kBoxThreads - 1) / kBoxThreads;
-  const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
-  int mine = 0;
-  for (int q = q0; q < q1; ++q) {
-    int64_t c;
-    const int k = B.cell(q, c) ? b.cs[c + 1] - b.cs[c] : 0;
-    cst[q] = uint16_t(k);
-    mine += k;
-  }
-  int incl = mine;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if ((threadIdx.x & 63) >= o) incl += y;
-  }
-  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
-  __syncthreads();
-  int run = incl - mine;
-  for (int w = 0; w < int(threadIdx.x >> 6); ++w) run += s_w[w];
-  for (int q = q0; q < q1; ++q) {
-    const int k = cst[q];
-    cst[q] = uint16_t(run);
-    run += k;
-  }
-  if (threadIdx.x == 0) cst[B.nbc] = uint16_t(nrec);
-  __syncthreads();
-  // (3) records: record i lies in the last box cell whose start is <= i
-  for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
-    int lo = 0, hi = B.nbc;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (int(cst[mid]) <= i) lo = mid;
-      else hi = mid;
+  if (nrec > 0) {
+    // (1) counts and first slots of the box cells, (2) exclusive scan of the
+    // counts in place: contiguous chunks per thread, then across the waves
+    // (the total is nrec)
+    const int per = (B.nbc + kBoxThreads - 1) / kBoxThreads;
+    const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
+    int mine = 0;
+    for (int q = q0; q < q1; ++q) {
+      int64_t c;
+      int k = 0, f = 0;
+      if (B.cell(q, c)) {
+        f = b.cs[c];
+        k = b.cs[c + 1] - f;
+      }
+      cst[q] = uint16_t(k);
+      cfirst[q] = f;
+      mine += k;
     }
-    int64_t c = 0;
-    B.cell(lo, c);
-    const int64_t t = int64_t(b.cs[c]) + (i - int(cst[lo]));
-    int x, y, z;
-    B.xyz(lo, x, y, z);
-    const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
-    const float4 p = a.pc32[t];
-    b.slot[r0 + i] = uint32_t(t);
-    b.pos[r0 + i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
-  }
-  // (4) entries: the neighbour's unwrapped cell (its cell plus the entry's
-  // periodic shift) gives its box cell, its rank in the cell the record
-  each_entry([&](int64_t, int64_t o) {
-    const Ent32 c = coef[o];
-    const uint32_t t = c.e & kSlotMask;
-    const uint32_t code = c.e >> kSlotBits;
-    const uint32_t ct = __float_as_uint(a.pc32[t].w);
-    const int ci = int(ct & ((1u << kCellBitsX) - 1));
-    const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
-    const int ck = int(ct >> (kCellBitsX + kCellBitsY));
-    const int x = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0] - B.ox;
-    const int y = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1] - B.oy;
-    const int z = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2] - B.oz;
-    int r = 0;
-    if (x >= 0 && x < B.ex && y >= 0 && y < B.ey && z >= 0 && z < B.ez) {
-      const int64_t cell = B.index(ci, cj, ck);
-      r = int(cst[(z * B.ey + y) * B.ex + x]) + int(int64_t(t) - b.cs[cell]);
+    int incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if ((threadIdx.x & 63) >= o) incl += y;
     }
-    r = min(max(r, 0), nrec - 1);  // LDS reads stay inside the brick's records
-    b.coef[o] = Ent32{uint32_t(r), c.a, c.bx};
-  });
+    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int run = incl - mine;
+    for (int w = 0; w < int(threadIdx.x >> 6); ++w) run += s_w[w];
+    for (int q = q0; q < q1; ++q) {
+      const int k = cst[q];
+      cst[q] = uint16_t(run);
+      run += k;
+    }
+    if (threadIdx.x == 0) cst[B.nbc] = uint16_t(nrec);
+    __syncthreads();
+    // (3) records: record i lies in the last box cell whose start is <= i
+    for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
+      int lo = 0, hi = B.nbc;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (int(cst[mid]) <= i) lo = mid;
+        else hi = mid;
+      }
+      const int64_t t = int64_t(cfirst[lo]) + (i - int(cst[lo]));
+      int x, y, z;
+      B.xyz(lo, x, y, z);
+      const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
+      const float4 p = a.pc32[t];
+      b.slot[r0 + i] = uint32_t(t);
+      b.pos[r0 + i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
+    }
+  }
+  // (4) pair records of the brick's targets; the next entry's position
+  // records are loaded while the current one is evaluated
+  for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+    const Grid& g = slot_grid<BATCH>(a, s);
+    const int cnt = a.cnt[s];
+    if (cnt == 0) continue;  // an empty list has no storage
+    const int64_t base = a.slice_off[s >> 6] + (s & 63);
+    const double4 xs = a.x4[s];
+    const int last = cnt - 1;
+    uint32_t e1 = a.ell[base];
+    double4 x1 = a.x4[e1 & kSlotMask];
+    float4 p1 = a.pc32[e1 & kSlotMask];
+    for (int k = 0; k < cnt; ++k) {
+      const int64_t o = base + int64_t(k) * kSlice;
+      const uint32_t e = e1;
+      const double4 xt = x1;
+      const float4 pt = p1;
+      e1 = a.ell[base + int64_t(min(k + 1, last)) * kSlice];
+      x1 = a.x4[e1 & kSlotMask];
+      p1 = a.pc32[e1 & kSlotMask];
+      const uint32_t t = e & kSlotMask;
+      const uint32_t code = e >> kSlotBits;
+      double dx, dy, dz;
+      pair_disp<false>(g, xs, xt, code, dx, dy, dz);
+      const float fx = float(dx), fy = float(dy), fz = float(dz);
+      float A, Bx;
+      green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
+      b.ab[o] = make_float2(A, Bx);
+      if (b.plain) b.plain[o] = Ent32{e, A, Bx};
+      if (nrec > 0) {
+        // the neighbour's unwrapped cell (its cell plus the entry's periodic
+        // shift) gives its box cell, its rank in the cell the record
+        const uint32_t ct = __float_as_uint(pt.w);
+        const int ci = int(ct & ((1u << kCellBitsX) - 1));
+        const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+        const int ck = int(ct >> (kCellBitsX + kCellBitsY));
+        const int x = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0] - B.ox;
+        const int y = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1] - B.oy;
+        const int z = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2] - B.oz;
+        int r = 0;
+        if (x >= 0 && x < B.ex && y >= 0 && y < B.ey && z >= 0 && z < B.ez) {
+          const int q = (z * B.ey + y) * B.ex + x;
+          r = int(cst[q]) + int(int64_t(t) - cfirst[q]);
+        }
+        b.idx[o] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay inside the brick's records
+      }
+    }
+  }
 }
 
 // pair sum of slot s through global gathers (FP32C, one operand)
-__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const Ent32* __restrict__ coef,
+__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float2* __restrict__ ab,
                                                    const float4* __restrict__ xg, double (&acc)[1][3]) {
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
-  const Ent32* cf = coef + base;
   const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
   const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -662,7 +694,8 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   float ps[3] = {0.f, 0.f, 0.f};
   KahanF kh[3];
   for (int k = 0; k < cnt; ++k) {
-    const Ent32 c = cf[int64_t(k) * kSlice];
+    const int64_t o = base + int64_t(k) * kSlice;
+    const Ent32 c{a.ell[o], ab[o].x, ab[o].y};
     const uint32_t e = c.e;
     const uint32_t t = e & kSlotMask;
     const float4 pt = a.pc32[t];
@@ -710,7 +743,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   if (nrec == 0) {
     for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
       double acc[1][3];
-      fp32c_pairs_global(a, s, b.coef, xg, acc);
+      fp32c_pairs_global(a, s, b.ab, xg, acc);
       apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
     }
     return;
@@ -779,18 +812,25 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
       }
     };
     if (cnt > 0) {  // an empty list has no storage
-      const Ent32* __restrict__ cf = b.coef + a.slice_off[s >> 6] + (s & 63);
+      const int64_t base = a.slice_off[s >> 6] + (s & 63);
+      const uint16_t* __restrict__ ci = b.idx + base;
+      const float2* __restrict__ cab = b.ab + base;
       const int last = cnt - 1;
+      auto load = [&](int k) {
+        const int64_t o = int64_t(min(k, last)) * kSlice;
+        const float2 t = cab[o];
+        return Ent32{uint32_t(ci[o]), t.x, t.y};
+      };
       Ent32 en[U];
 #pragma unroll
-      for (int i = 0; i < U; ++i) en[i] = cf[int64_t(min(i, last)) * kSlice];
+      for (int i = 0; i < U; ++i) en[i] = load(i);
       int k = 0;
       for (; k + U <= cnt; k += U) {
         Ent32 c[U];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
           c[i] = en[i];
-          en[i] = cf[int64_t(min(k + U + i, last)) * kSlice];
+          en[i] = load(k + U + i);
         }
 #pragma unroll
         for (int i = 0; i < U; ++i) {
@@ -921,7 +961,9 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.off = ctx->box_off;
   b.slot = ctx->box_slot;
   b.pos = ctx->box_pos;
-  b.coef = ctx->box_coef;
+  b.ab = ctx->box_ab;
+  b.idx = ctx->box_idx;
+  b.plain = nullptr;
   return b;
 }
 
@@ -958,6 +1000,8 @@ hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const 
     return hipErrorInvalidConfiguration;
   }
   if (c < 0 || c >= kNApply) return hipErrorInvalidValue;
+  // list-order FP32C tensors are evaluated only where a launch needs them
+  if (P == GGEM_FP32C && !ctx->coef32_valid) return hipErrorInvalidConfiguration;
   const Shape sh = kApplyShapes[c];
   a.xcd = sh.xcd;
   switch (sh.block) {
@@ -1031,10 +1075,14 @@ hipError_t launch_box_count(ggem_ctx* ctx) {
   return hipGetLastError();
 }
 
-hipError_t launch_box_fill(ggem_ctx* ctx) {
+hipError_t launch_box_fill(ggem_ctx* ctx, Ent32* plain) {
   ProfScope ps(ctx, "box_fill");
-  hipLaunchKernelGGL(k_box_fill, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx),
-                     box_args(ctx), ctx->coef32);
+  BoxArgs b = box_args(ctx);
+  b.plain = plain;
+  if (ctx->nsys > 1)
+    hipLaunchKernelGGL(k_box_fill<true>, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx), b);
+  else
+    hipLaunchKernelGGL(k_box_fill<false>, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx), b);
   return hipGetLastError();
 }
 
