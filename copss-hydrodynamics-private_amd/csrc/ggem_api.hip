@@ -130,7 +130,7 @@ int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::funct
 // Staged neighbour records of the FP32C lists (per brick of cells): counts,
 // their scan, one synchronisation for the sizes, then records and
 // re-indexed entries.
-int prepare_box(ggem_ctx* ctx) {
+int prepare_box(ggem_ctx* ctx, ggem::Ent32* plain) {
   ctx->box_valid = false;
   ctx->box_max = 0;
   if (!box_supported(ctx)) return GGEM_OK;
@@ -161,9 +161,10 @@ int prepare_box(ggem_ctx* ctx) {
   const size_t ne = size_t(std::max<int64_t>(ctx->ell_len, 1));
   if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
-  if ((e = launch_box_fill(ctx, nullptr)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  if ((e = launch_box_fill(ctx, plain)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
   ctx->box_max = mx;
   ctx->box_valid = true;
+  if (plain) ctx->coef32_valid = true;
   return GGEM_OK;
 }
 
@@ -190,11 +191,16 @@ int ensure_tensors(ggem_ctx* ctx, int nrhs) {
   // list-order tensors serve several, the autotuner's other candidates and
   // contexts without staged bricks
   if (!ctx->box_built) {
-    if ((rc = prepare_box(ctx))) return rc;
+    // the list-order records ride along when this call or the last build
+    // needed them
+    const bool plain = nrhs > 1 || ctx->tune.on || ctx->plain_wanted;
+    if (plain && (rc = ensure_coef32(ctx))) return rc;
+    if ((rc = prepare_box(ctx, plain ? ctx->coef32 : nullptr))) return rc;
     ctx->box_built = true;
   }
   const bool global = nrhs > 1 || ctx->tune.on || !ctx->box_valid;
   if (!global || ctx->coef32_valid) return GGEM_OK;
+  ctx->plain_wanted = ctx->box_valid;
   if ((rc = ensure_coef32(ctx))) return rc;
   // with staged bricks the list-order records come from the kernel that
   // wrote the staged ones (bit-identical tensors)

@@ -187,6 +187,7 @@ struct ggem_ctx {
   int32_t* box_max_d = nullptr;   // largest record count of a staged brick
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
+  bool plain_wanted = false;      // the last lists also needed coef32 next to the staged records
   bool box_valid = false;         // ... and some brick is staged
   int bsign = 1;  // sign of Bx: +1 when xi > alpha
 
