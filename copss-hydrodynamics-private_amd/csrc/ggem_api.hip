@@ -137,32 +137,40 @@ int prepare_box(ggem_ctx* ctx, ggem::Ent32* plain) {
   const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
   if (ctx->box_brick_cap < nb + 1) {
     if (ctx->box_cnt) (void)hipFree(ctx->box_cnt);
-    ctx->box_cnt = ctx->box_off = nullptr;
+    ctx->box_cnt = ctx->box_off = ctx->box_ecnt = ctx->box_eoff = nullptr;
     ctx->box_brick_cap = 0;
-    GGEM_HIP(ctx, hipMalloc(&ctx->box_cnt, sizeof(int64_t) * 2 * (nb + 1)));
+    GGEM_HIP(ctx, hipMalloc(&ctx->box_cnt, sizeof(int64_t) * 4 * (nb + 1)));
     ctx->box_brick_cap = nb + 1;
   }
   ctx->box_off = ctx->box_cnt + ctx->box_brick_cap;
-  if (!ctx->box_max_d) GGEM_HIP(ctx, hipMalloc(&ctx->box_max_d, sizeof(int32_t)));
-  int rc = grow_scan64(ctx, int64_t(nb) + 1);
-  if (rc) return rc;
+  ctx->box_ecnt = ctx->box_cnt + 2 * ctx->box_brick_cap;
+  ctx->box_eoff = ctx->box_cnt + 3 * ctx->box_brick_cap;
+  if (!ctx->box_max_d) GGEM_HIP(ctx, hipMalloc(&ctx->box_max_d, 2 * sizeof(int32_t)));
+  int rc;
+  if ((rc = ensure(ctx, &ctx->box_tgt, &ctx->box_tgt_cap, size_t(ctx->n)))) return rc;
+  if ((rc = ensure(ctx, &ctx->box_base, &ctx->box_base_cap, size_t(ctx->n)))) return rc;
+  if ((rc = grow_scan64(ctx, int64_t(nb) + 1))) return rc;
   hipError_t e = launch_box_count(ctx);
   if (e != hipSuccess) return hip_fail(ctx, e, "box_count");
-  if ((e = launch_scan64(ctx, ctx->box_cnt, int64_t(nb), ctx->box_off)) != hipSuccess)
+  if ((e = launch_scan64(ctx, ctx->box_cnt, int64_t(nb), ctx->box_off)) != hipSuccess ||
+      (e = launch_scan64(ctx, ctx->box_ecnt, int64_t(nb), ctx->box_eoff)) != hipSuccess)
     return hip_fail(ctx, e, "box scan");
-  int64_t total = 0;
-  int32_t mx = 0;
-  GGEM_HIP(ctx, hipMemcpyAsync(&total, ctx->box_off + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&mx, ctx->box_max_d, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  int64_t total[2] = {0, 0};
+  int32_t mx[2] = {0, 0};
+  GGEM_HIP(ctx, hipMemcpyAsync(&total[0], ctx->box_off + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&total[1], ctx->box_eoff + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(mx, ctx->box_max_d, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (total <= 0 || mx <= 0) return GGEM_OK;  // no brick staged: the plain kernel serves
-  if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total)))) return rc;
-  if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total)))) return rc;
-  const size_t ne = size_t(std::max<int64_t>(ctx->ell_len, 1));
+  // no brick staged, or a brick too crowded for the entry layout: the
+  // list-order kernels serve
+  if (total[0] <= 0 || mx[0] <= 0 || mx[1] > kBoxMaxTargets) return GGEM_OK;
+  if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total[0])))) return rc;
+  if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total[0])))) return rc;
+  const size_t ne = size_t(std::max<int64_t>(total[1], 1));
   if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
   if ((e = launch_box_fill(ctx, plain)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
-  ctx->box_max = mx;
+  ctx->box_max = mx[0];
   ctx->box_valid = true;
   if (plain) ctx->coef32_valid = true;
   return GGEM_OK;
@@ -376,6 +384,8 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->elist);
   f(ctx->coef32);
   f(ctx->box_cnt);
+  f(ctx->box_tgt);
+  f(ctx->box_base);
   f(ctx->box_slot);
   f(ctx->box_pos);
   f(ctx->box_ab);

@@ -75,6 +75,8 @@ constexpr int64_t kMaxParticles = int64_t(1) << kSlotBits;
 constexpr int kSlice = 64;  // targets per neighbour-list slice (one wave)
 // packed cell coordinates of the FP32C position records: i | j << 11 | k << 22
 constexpr int kCellBitsX = 11, kCellBitsY = 11, kCellBitsZ = 10;
+// most particles of one brick of cells for the staged FP32C entry layout
+constexpr int kBoxMaxTargets = 65536;
 
 struct Profiler {
   struct Rec {
@@ -175,16 +177,21 @@ struct ggem_ctx {
   // periodic shifts; a brick without records gathers from global memory
   int64_t* box_cnt = nullptr;     // [n_bricks] records of each staged brick, 0 otherwise
   int64_t* box_off = nullptr;     // [n_bricks + 1] first record of each brick
-  size_t box_brick_cap = 0;
+  int64_t* box_ecnt = nullptr;    // [n_bricks] entries of each brick's layout
+  int64_t* box_eoff = nullptr;    // [n_bricks + 1] first entry of each brick
+  size_t box_brick_cap = 0;       // (one allocation at box_cnt holds the four)
+  int32_t* box_tgt = nullptr;     // [n] slots of each brick, longest lists first in staged bricks
+  int64_t* box_base = nullptr;    // [n] entry 0 of the target at each position of box_tgt
+  size_t box_tgt_cap = 0, box_base_cap = 0;
   uint32_t* box_slot = nullptr;   // [records] slot of each record
   size_t box_rec_cap = 0;
   float4* box_pos = nullptr;      // [records] cell-relative FP32 position, packed box cell
   size_t box_pos_cap = 0;
-  float2* box_ab = nullptr;       // [ell] pair tensor (A, Bx) of every entry, list layout
+  float2* box_ab = nullptr;       // [entries] pair tensor (A, Bx) of every entry, brick layout
   size_t box_ab_cap = 0;
-  uint16_t* box_idx = nullptr;    // [ell] the neighbour's record number (staged bricks)
+  uint16_t* box_idx = nullptr;    // [entries] the neighbour's record number (staged bricks)
   size_t box_idx_cap = 0;
-  int32_t* box_max_d = nullptr;   // largest record count of a staged brick
+  int32_t* box_max_d = nullptr;   // [2] largest record count of a staged brick, most targets of a brick
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
   bool plain_wanted = false;      // the last lists also needed coef32 next to the staged records

@@ -421,14 +421,26 @@ struct BoxArgs {
   const int32_t* cs;     // cell starts, brick order
   int32_t reach[3];      // stencil reach of a plain context
   int64_t* cnt;          // [n_bricks] records of each staged brick (k_box_count)
-  int32_t* maxrec;       // largest staged count (k_box_count)
+  int64_t* ecnt;         // [n_bricks] entries of each brick's layout (k_box_count)
+  int32_t* maxrec;       // [2] largest staged record count, most targets of a brick
   const int64_t* off;    // [n_bricks + 1] first record of each brick
+  const int64_t* eoff;   // [n_bricks + 1] first entry of each brick
+  int32_t* tgt;          // [n] slots of each brick, longest lists first (staged bricks)
+  int64_t* base;         // [n] entry 0 of the target at each position of tgt
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
-  float2* ab;            // [ell] pair tensor (A, Bx) of every entry, list layout
-  uint16_t* idx;         // [ell] neighbour's record number (staged bricks)
+  float2* ab;            // [entries] pair tensor (A, Bx), brick layout
+  uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
   Ent32* plain;          // k_box_fill: the list-order records (coef32) too, when set
 };
+
+// Entry layout of a brick: its targets in the order tgt, in waves of 64;
+// entry k of the target at position i = 64 w + l lies at
+// eoff + 64 (W_0 + .. + W_{w-1}) + 64 k + l, W_w being the longest list of
+// wave w.  Staged bricks order their targets by list length, longest first
+// (ties by slot), so the lanes of a wave do similar work.
+constexpr int kBoxMaxWaves = kBoxMaxTargets / 64;  // waves of one brick
+constexpr int kBoxSortMax = 2048;   // targets of a brick ordered in LDS
 
 // Box of one brick: origin (ox, oy, oz) and extent (ex, ey, ez) in cells of
 // the brick's system grid.  The grid fields are copied (no pointer into the
@@ -525,15 +537,27 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// Records of every brick's box: a brick is staged when it has targets, its
-// box fits the packing and it holds at most kBoxMaxRecs records.
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Records of every brick's box (a brick is staged when it has targets, its
+// box fits the packing and holds at most kBoxMaxRecs records), the order of
+// its targets and the size of its entry layout.
 __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b) {
   __shared__ int32_t s_w[kBoxThreads / 64];
+  __shared__ int32_t keys[kBoxSortMax];   // list lengths in slot order
+  __shared__ int32_t skeys[kBoxSortMax];  // the same, longest first
+  __shared__ int64_t s_e[kBoxThreads / 64];
   const int64_t brick = blockIdx.x;
   const int64_t c0 = brick * kBrickCells;
   const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  const int nt = se - sb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int total = 0;
-  if (sb < se) {  // uniform over the block
+  if (nt > 0) {  // uniform over the block
     BoxGeom B;
     B.init(a, b.reach, brick, sb);
     if (B.fits()) {
@@ -543,16 +567,49 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
         if (B.cell(q, c)) mine += b.cs[c + 1] - b.cs[c];
       }
       mine = wave_sum(mine);
-      if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = mine;
+      if (lane == 0) s_w[wv] = mine;
       __syncthreads();
 #pragma unroll
       for (int w = 0; w < kBoxThreads / 64; ++w) total += s_w[w];
     }
   }
+  const bool staged = total > 0 && total <= kBoxMaxRecs;
+  const bool sorted = staged && nt <= kBoxSortMax;
+  if (sorted) {
+    for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) keys[i] = a.cnt[sb + i];
+    __syncthreads();
+    for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+      const int ki = keys[i];
+      int r = 0;
+      for (int j = 0; j < nt; ++j) {
+        const int kj = keys[j];
+        r += (kj > ki || (kj == ki && j < i)) ? 1 : 0;
+      }
+      skeys[r] = ki;
+      b.tgt[sb + r] = sb + i;
+    }
+    __syncthreads();
+  } else {
+    for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) b.tgt[sb + i] = sb + i;
+  }
+  // entries: 64 x the longest list of every wave of the order
+  int64_t esum = 0;
+  const int nw = (nt + 63) / 64;
+  for (int w = wv; w < nw; w += kBoxThreads / 64) {
+    const int i = 64 * w + lane;
+    const int c = i < nt ? (sorted ? skeys[i] : a.cnt[sb + i]) : 0;
+    esum += 64 * int64_t(wave_max(c));
+  }
+  if (lane == 0) s_e[wv] = esum;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const bool staged = total > 0 && total <= kBoxMaxRecs;
+    int64_t e = 0;
+#pragma unroll
+    for (int w = 0; w < kBoxThreads / 64; ++w) e += s_e[w];
+    b.ecnt[brick] = e;
     b.cnt[brick] = staged ? total : 0;
     if (staged) atomicMax(b.maxrec, total);
+    atomicMax(b.maxrec + 1, nt);
   }
 }
 
@@ -568,11 +625,14 @@ template <bool BATCH>
 __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b) {
   __shared__ uint16_t cst[kBoxMaxCells + 1];  // record start of each box cell
   __shared__ int32_t cfirst[kBoxMaxCells];    // first slot of each box cell
+  __shared__ int32_t wpre[kBoxMaxWaves + 1];  // entry rows before each wave
   __shared__ int32_t s_w[kBoxThreads / 64];
   const int64_t brick = blockIdx.x;
   const int64_t c0 = brick * kBrickCells;
   const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
   if (sb >= se) return;
+  const int nt = se - sb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   BoxGeom B;
@@ -628,9 +688,30 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       b.pos[r0 + i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
     }
   }
-  // (4) pair records of the brick's targets; the next entry's position
+  // (4) the entry layout: the longest list of every wave of the order,
+  // exclusive prefix (the host keeps nt within kBoxMaxWaves waves)
+  const int nw = (nt + 63) / 64;
+  for (int w = wv; w < nw; w += kBoxThreads / 64) {
+    const int i = 64 * w + lane;
+    const int m = wave_max(i < nt ? a.cnt[b.tgt[sb + i]] : 0);
+    if (lane == 0) wpre[w] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < nw; ++w) {
+      const int m = wpre[w];
+      wpre[w] = run;
+      run += m;
+    }
+  }
+  __syncthreads();
+  // (5) pair records of the brick's targets; the next entry's position
   // records are loaded while the current one is evaluated
-  for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+  for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+    const int64_t s = b.tgt[sb + i];
+    const int64_t bb = b.eoff[brick] + 64 * int64_t(wpre[i >> 6]) + (i & 63);
+    b.base[sb + i] = bb;
     const Grid& g = slot_grid<BATCH>(a, s);
     const int cnt = a.cnt[s];
     if (cnt == 0) continue;  // an empty list has no storage
@@ -642,6 +723,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     float4 p1 = a.pc32[e1 & kSlotMask];
     for (int k = 0; k < cnt; ++k) {
       const int64_t o = base + int64_t(k) * kSlice;
+      const int64_t ob = bb + int64_t(k) * kSlice;
       const uint32_t e = e1;
       const double4 xt = x1;
       const float4 pt = p1;
@@ -655,7 +737,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       const float fx = float(dx), fy = float(dy), fz = float(dz);
       float A, Bx;
       green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
-      b.ab[o] = make_float2(A, Bx);
+      b.ab[ob] = make_float2(A, Bx);
       if (b.plain) b.plain[o] = Ent32{e, A, Bx};
       if (nrec > 0) {
         // the neighbour's unwrapped cell (its cell plus the entry's periodic
@@ -672,7 +754,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
           const int q = (z * B.ey + y) * B.ex + x;
           r = int(cst[q]) + int(int64_t(t) - cfirst[q]);
         }
-        b.idx[o] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay inside the brick's records
+        b.idx[ob] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay inside the brick's records
       }
     }
   }
@@ -680,7 +762,8 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
 
 // pair sum of slot s through global gathers (FP32C, one operand)
 __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float2* __restrict__ ab,
-                                                   const float4* __restrict__ xg, double (&acc)[1][3]) {
+                                                   int64_t bb, const float4* __restrict__ xg,
+                                                   double (&acc)[1][3]) {
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
   const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
@@ -694,8 +777,8 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   float ps[3] = {0.f, 0.f, 0.f};
   KahanF kh[3];
   for (int k = 0; k < cnt; ++k) {
-    const int64_t o = base + int64_t(k) * kSlice;
-    const Ent32 c{a.ell[o], ab[o].x, ab[o].y};
+    const float2 t2 = ab[bb + int64_t(k) * kSlice];
+    const Ent32 c{a.ell[base + int64_t(k) * kSlice], t2.x, t2.y};
     const uint32_t e = c.e;
     const uint32_t t = e & kSlotMask;
     const float4 pt = a.pc32[t];
@@ -740,10 +823,12 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   if (sb >= se) return;  // no targets (uniform over the block)
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
+  const int nt = se - sb;
   if (nrec == 0) {
-    for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+    for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+      const int64_t s = b.tgt[sb + i];
       double acc[1][3];
-      fp32c_pairs_global(a, s, b.ab, xg, acc);
+      fp32c_pairs_global(a, s, b.ab, b.base[sb + i], xg, acc);
       apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
     }
     return;
@@ -778,7 +863,9 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   B.init(a, b.reach, brick, sb);
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
   constexpr int U = 8;  // entries per group, the next group loaded one ahead
-  for (int64_t s = sb + threadIdx.x; s < se; s += kBoxThreads) {
+  for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+    const int64_t s = b.tgt[sb + i];
+    const int64_t base = b.base[sb + i];
     const float4 po = a.pc32[s];
     const uint32_t co = __float_as_uint(po.w);
     const int bx = int(co & ((1u << kCellBitsX) - 1)) - B.ox;
@@ -812,7 +899,6 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
       }
     };
     if (cnt > 0) {  // an empty list has no storage
-      const int64_t base = a.slice_off[s >> 6] + (s & 63);
       const uint16_t* __restrict__ ci = b.idx + base;
       const float2* __restrict__ cab = b.ab + base;
       const int last = cnt - 1;
@@ -957,8 +1043,12 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.cs = ctx->cell_start;
   for (int d = 0; d < 3; ++d) b.reach[d] = ctx->reach[d];
   b.cnt = ctx->box_cnt;
+  b.ecnt = ctx->box_ecnt;
   b.maxrec = ctx->box_max_d;
   b.off = ctx->box_off;
+  b.eoff = ctx->box_eoff;
+  b.tgt = ctx->box_tgt;
+  b.base = ctx->box_base;
   b.slot = ctx->box_slot;
   b.pos = ctx->box_pos;
   b.ab = ctx->box_ab;
@@ -1067,7 +1157,7 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
 bool box_supported(const ggem_ctx* ctx) { return ctx->n > 0 && !ctx->exact_minimg; }
 
 hipError_t launch_box_count(ggem_ctx* ctx) {
-  hipError_t e = hipMemsetAsync(ctx->box_max_d, 0, sizeof(int32_t), ctx->stream);
+  hipError_t e = hipMemsetAsync(ctx->box_max_d, 0, 2 * sizeof(int32_t), ctx->stream);
   if (e != hipSuccess) return e;
   ProfScope ps(ctx, "box_count");
   hipLaunchKernelGGL(k_box_count, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx),
