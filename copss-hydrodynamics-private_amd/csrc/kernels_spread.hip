@@ -20,7 +20,7 @@
 // NQ1^2 x 3 accumulators.  The NQ1 threads of an element read the same
 // list entries (broadcast loads).  rho goes to LDS for the IF contraction
 // (FP64 arithmetic in both precisions).  FP32C: factors and kernel values
-// in FP32 after the FP64 displacement, partial sums merged every 8
+// in FP32 after the FP64 displacement, partial sums merged every 16
 // particles into Kahan accumulators, rho staged in LDS as FP32.
 #include "ggem_device.h"
 
@@ -29,7 +29,7 @@ namespace ggem {
 namespace {
 
 constexpr int kMaxNu = 27;
-constexpr int kMergeEvery = 8;
+constexpr int kMergeEvery = 16;
 // elements per block, candidates of the autotuner (index 0 is the default);
 // the per-element arithmetic does not depend on the choice
 constexpr int kElemChoices[] = {64, 32, 16};
@@ -47,14 +47,12 @@ struct SpreadArgs {
   const double4* fg64;
 };
 
-// e^{-a} for a >= 0 in FP32, argument carried with its rounding error.
+// e^{-a} for a >= 0 in FP32 from the hardware exp2: the rounding of a log2(e)
+// perturbs the result by at most |a| 2^-24 relative, and a e^{-a} <= 1/e
+// keeps the absolute error below 2^-25 of the peak weight.
 __device__ __forceinline__ float exp_neg(float a) {
   constexpr float L2E = 1.44269502163f;
-  constexpr float L2E_LO = 1.925963033e-08f;
-  constexpr float LN2 = 0.693147180560f;
-  const float y = -a * L2E;
-  const float ye = __builtin_fmaf(-a, L2E, -y) - a * L2E_LO;
-  return __builtin_amdgcn_exp2f(y) * __builtin_fmaf(ye, LN2, 1.0f);
+  return __builtin_amdgcn_exp2f(-a * L2E);
 }
 __device__ __forceinline__ double exp_neg(double a) { return exp(-a); }
 
@@ -94,12 +92,27 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
     KahanF kh[sizeof(T) == 4 ? NB : 1][3];
 #pragma unroll
     for (int b = 0; b < NB; ++b) p[b][0] = p[b][1] = p[b][2] = T(0);
+    // the next particle's position and force records are loaded while the
+    // current one is summed, its list entry one step earlier
+    const int64_t lastk = b1 > b0 ? b1 - 1 : b0;
+    uint32_t e1 = b1 > b0 ? a.elist[b0] : 0u;
+    uint32_t e2 = b1 > b0 ? a.elist[min(b0 + 1, lastk)] : 0u;
+    double4 x1 = make_double4(0.0, 0.0, 0.0, 0.0);
+    V4 f1{};
+    if (b1 > b0) {  // an empty list reads nothing (a context may hold no particles)
+      x1 = a.x4[e1 & kSlotMask];
+      f1 = load_f(a.fg32, a.fg64, e1 & kSlotMask, T(0));
+    }
     for (int64_t k = b0; k < b1; ++k) {
-      const uint32_t en = a.elist[k];
-      const uint32_t t = en & kSlotMask;
+      const uint32_t en = e1;
+      const double4 xt = x1;
+      const V4 f = f1;
+      e1 = e2;
+      x1 = a.x4[e1 & kSlotMask];
+      f1 = load_f(a.fg32, a.fg64, e1 & kSlotMask, T(0));
+      e2 = a.elist[min(k + 2, lastk)];
       double d0, d1, d2;
-      pair_disp<EXACT>(a.g, cen, a.x4[t], en >> kSlotBits, d0, d1, d2);
-      const V4 f = load_f(a.fg32, a.fg64, t, T(0));
+      pair_disp<EXACT>(a.g, cen, xt, en >> kSlotBits, d0, d1, d2);
       const T ux = T(d0) - gx;
       const T sx = a.alpha2 * (ux * ux);
       const T ex = a.alpha3_pi_23 * exp_neg(sx);
