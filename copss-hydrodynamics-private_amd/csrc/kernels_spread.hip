@@ -72,8 +72,6 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
   using R = typename std::conditional<sizeof(T) == 4, float, double>::type;  // LDS rho type
   using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
   __shared__ R s_rho[kElems * NQP * 3];
-  __shared__ double s_if[kMaxNu * NQP];
-  for (int q = threadIdx.x; q < n_u * NQP; q += blockDim.x) s_if[q] = IF[q];
 
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int le = threadIdx.x / NQ1;
@@ -164,18 +162,50 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
   }
   __syncthreads();
   if (!Fe) return;
+  // IF contraction, one (element, component) pair per thread and step: the
+  // element's NQP values of rho in registers, the table row of each shape
+  // function read with scalar loads (uniform over the block), FP64 FMAs in
+  // Gauss point order.  The results are staged in the rho buffer (free once
+  // rho sits in registers) and written out as one contiguous range per
+  // pass, so the stores are coalesced.
+  constexpr int M = (3 + NQ1 - 1) / NQ1;  // pairs per thread (blockDim = kElems NQ1)
   const int per = 3 * n_u;
-  for (int idx = threadIdx.x; idx < kElems * per; idx += blockDim.x) {
-    const int l2 = idx / per;
-    const int rem = idx - l2 * per;
-    const int j = rem / n_u;
-    const int l = rem - j * n_u;
-    const int64_t e2 = blk * kElems + l2;
-    if (e2 >= a.g.n_elem) continue;
-    double s = 0.0;
+  double r[M][NQP];
 #pragma unroll
-    for (int q = 0; q < NQP; ++q) s += s_if[l * NQP + q] * double(s_rho[(l2 * NQP + q) * 3 + j]);
-    Fe[e2 * per + rem] = s;
+  for (int m = 0; m < M; ++m) {
+    const int idx = threadIdx.x + m * blockDim.x;
+    const int l2 = min(idx / 3, kElems - 1);
+    const int j = idx - 3 * (idx / 3);
+#pragma unroll
+    for (int q = 0; q < NQP; ++q) r[m][q] = double(s_rho[(l2 * NQP + q) * 3 + min(j, 2)]);
+  }
+  __syncthreads();
+  double* stage = reinterpret_cast<double*>(s_rho);
+  constexpr int kStage = int(sizeof(s_rho) / sizeof(double));
+  const int step = kStage / per;  // elements per pass (n_u <= 27 keeps it >= 1)
+  const int64_t e_first = blk * kElems;
+  const int ne = int(min<int64_t>(kElems, a.g.n_elem - e_first));
+  for (int p0 = 0; p0 < ne; p0 += step) {
+    const int p1 = min(ne, p0 + step);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int idx = threadIdx.x + m * blockDim.x;
+      const int l2 = idx / 3;
+      const int j = idx - 3 * l2;
+      if (l2 < p0 || l2 >= p1) continue;
+      double* o = stage + (l2 - p0) * per + j * n_u;
+      for (int l = 0; l < n_u; ++l) {
+        const double* __restrict__ row = IF + l * NQP;
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQP; ++q) s = __builtin_fma(row[q], r[m][q], s);
+        o[l] = s;
+      }
+    }
+    __syncthreads();
+    double* __restrict__ dst = Fe + (e_first + p0) * per;
+    for (int i = threadIdx.x; i < (p1 - p0) * per; i += blockDim.x) dst[i] = stage[i];
+    __syncthreads();
   }
 }
 
