@@ -383,16 +383,6 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
   apply_epilogue<NR, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
 }
 
-// x mod n for x a few periods at most outside [0, n) (no integer division)
-__device__ __forceinline__ int wrap_axis(int x, int n) {
-  while (x < 0) x += n;
-  while (x >= n) x -= n;
-  return x;
-}
-
-// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
-// away from an integer, far more than the FP32 rounding of the product
-__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
 
 // ---- FP32C pair sum with the neighbour records staged in LDS ----
 //

@@ -111,14 +111,33 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
       e2 = a.elist[min(k + 2, lastk)];
       double d0, d1, d2;
       pair_disp<EXACT>(a.g, cen, xt, en >> kSlotBits, d0, d1, d2);
-      const T ux = T(d0) - gx;
+      // x_p - q per axis; with the explicit minimum image it is taken again
+      // relative to each Gauss point, as point_vector(q, x_p) does
+      // (assemble_stokes.C:643-672): in a box barely wider than the
+      // stencil, x_p - c and x_p - q may lie on different images
+      T ux, uyv[NQ1], uzv[NQ1];
+      if constexpr (EXACT) {
+        ux = T(min_image(d0 - double(gx), a.g.L[0], a.g.half_L[0], a.g.periodic[0]));
+#pragma unroll
+        for (int g = 0; g < NQ1; ++g) {
+          uyv[g] = T(min_image(d1 - double(a.goff[1][g]), a.g.L[1], a.g.half_L[1], a.g.periodic[1]));
+          uzv[g] = T(min_image(d2 - double(a.goff[2][g]), a.g.L[2], a.g.half_L[2], a.g.periodic[2]));
+        }
+      } else {
+        ux = T(d0) - gx;
+#pragma unroll
+        for (int g = 0; g < NQ1; ++g) {
+          uyv[g] = T(d1) - a.goff[1][g];
+          uzv[g] = T(d2) - a.goff[2][g];
+        }
+      }
       const T sx = a.alpha2 * (ux * ux);
       const T ex = a.alpha3_pi_23 * exp_neg(sx);
       T ey[NQ1], sy[NQ1], ez[NQ1], sz[NQ1];
 #pragma unroll
       for (int g = 0; g < NQ1; ++g) {
-        const T uy = T(d1) - a.goff[1][g];
-        const T uz = T(d2) - a.goff[2][g];
+        const T uy = uyv[g];
+        const T uz = uzv[g];
         sy[g] = a.alpha2 * (uy * uy);
         sz[g] = a.alpha2 * (uz * uz);
         ey[g] = ex * exp_neg(sy[g]);
