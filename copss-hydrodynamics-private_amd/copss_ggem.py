@@ -46,7 +46,7 @@ EXPORTS = [
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
     "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
     "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
-    "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table",
+    "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table", "ggem_util_fp32c_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
 ]
 
@@ -133,6 +133,7 @@ def lib():
             "ggem_util_rsa_positions": [P, I64, D, U64, P],
             "ggem_util_normal": [I64, U64, P],
             "ggem_util_hex20_if_table": [P, I32, P],
+            "ggem_util_fp32c_table": [C.c_double, C.c_double, C.c_double, I32, P, P, P],
             "ggem_profile_enable": [P, I32],
             "ggem_profile_reset": [P],
             "ggem_profile_get": [P, C.c_char_p, P, P],
@@ -201,6 +202,18 @@ def normal(n, seed):
     out = np.zeros(n)
     _check(lib().ggem_util_normal(n, seed, _p(out)))
     return out
+
+
+def fp32c_table(eps, alpha, ksi):
+    """(table [n, 16] float32, meta [8] float32) of the FP32C pair
+    coefficients as a piecewise polynomial in s = r^2 (see
+    ggem_util_fp32c_table)."""
+    meta = np.zeros(8, np.float32)
+    n = C.c_int32(0)
+    _check(lib().ggem_util_fp32c_table(eps, alpha, ksi, 0, None, _p(meta), C.byref(n)))
+    tab = np.zeros((n.value, 16), np.float32)
+    _check(lib().ggem_util_fp32c_table(eps, alpha, ksi, tab.size, _p(tab), _p(meta), C.byref(n)))
+    return tab, meta
 
 
 def hex20_if_table(h, nqp_1d):

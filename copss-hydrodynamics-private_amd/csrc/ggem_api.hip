@@ -128,9 +128,9 @@ int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::funct
 }
 
 // Staged neighbour records of the FP32C lists (per brick of cells): counts,
-// their scan, one synchronisation for the sizes, then records and
-// re-indexed entries.
-int prepare_box(ggem_ctx* ctx, ggem::Ent32* plain) {
+// their scans, one synchronisation for the sizes, then records, target
+// layout and record numbers.
+int prepare_box(ggem_ctx* ctx) {
   ctx->box_valid = false;
   ctx->box_max = 0;
   if (!box_supported(ctx)) return GGEM_OK;
@@ -167,23 +167,15 @@ int prepare_box(ggem_ctx* ctx, ggem::Ent32* plain) {
   if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total[0])))) return rc;
   if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total[0])))) return rc;
   const size_t ne = size_t(std::max<int64_t>(total[1], 1));
-  if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
-  if ((e = launch_box_fill(ctx, plain)) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
+  if ((e = launch_box_fill(ctx, mx[1])) != hipSuccess) return hip_fail(ctx, e, "box_fill");
   ctx->box_max = mx[0];
   ctx->box_valid = true;
-  if (plain) ctx->coef32_valid = true;
   return GGEM_OK;
 }
 
-int ensure_coef32(ggem_ctx* ctx) {
-  size_t cap = size_t(ctx->coef32_cap);
-  int rc = ensure(ctx, &ctx->coef32, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1)));
-  ctx->coef32_cap = int64_t(cap);
-  return rc;
-}
-
-int ensure_tensors(ggem_ctx* ctx, int nrhs) {
+int ensure_tensors(ggem_ctx* ctx, int) {
   int rc;
   if (ctx->p.precision == GGEM_FP64) {
     if (ctx->coef64_valid) return GGEM_OK;
@@ -195,26 +187,12 @@ int ensure_tensors(ggem_ctx* ctx, int nrhs) {
     ctx->coef64_valid = true;
     return GGEM_OK;
   }
-  // FP32C: the staged-brick records serve one right-hand side; the plain
-  // list-order tensors serve several, the autotuner's other candidates and
-  // contexts without staged bricks
+  // FP32C keeps no tensors (the pair sums evaluate the coefficient table);
+  // the staged-brick records serve one right-hand side
   if (!ctx->box_built) {
-    // the list-order records ride along when this call or the last build
-    // needed them
-    const bool plain = nrhs > 1 || ctx->tune.on || ctx->plain_wanted;
-    if (plain && (rc = ensure_coef32(ctx))) return rc;
-    if ((rc = prepare_box(ctx, plain ? ctx->coef32 : nullptr))) return rc;
+    if ((rc = prepare_box(ctx))) return rc;
     ctx->box_built = true;
   }
-  const bool global = nrhs > 1 || ctx->tune.on || !ctx->box_valid;
-  if (!global || ctx->coef32_valid) return GGEM_OK;
-  ctx->plain_wanted = ctx->box_valid;
-  if ((rc = ensure_coef32(ctx))) return rc;
-  // with staged bricks the list-order records come from the kernel that
-  // wrote the staged ones (bit-identical tensors)
-  hipError_t e = ctx->box_valid ? launch_box_fill(ctx, ctx->coef32) : launch_pair_tensors(ctx, GGEM_FP32C);
-  if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
-  ctx->coef32_valid = true;
   return GGEM_OK;
 }
 
@@ -382,14 +360,14 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->ell);
   f(ctx->elist_off);
   f(ctx->elist);
-  f(ctx->coef32);
+  f(ctx->tab32);
   f(ctx->box_cnt);
   f(ctx->box_tgt);
   f(ctx->box_base);
   f(ctx->box_slot);
   f(ctx->box_pos);
-  f(ctx->box_ab);
   f(ctx->box_idx);
+  f(ctx->box_ab);
   f(ctx->box_max_d);
   f(ctx->pc32);
   f(ctx->coef64);
@@ -592,6 +570,21 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
     e = hipMemcpy(ctx->rows_d, rows.data(), sizeof(StencilRow) * rows.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(e, "hipMemcpy");
   }
+  {  // FP32C coefficient table of (A, Bx) in r^2 (shared by all systems: same mu, alpha, xi)
+    float meta[8];
+    int32_t nint = 0;
+    if (ggem_util_fp32c_table(c.one_eight_pi_mu, c.alpha, c.ksi, 0, nullptr, meta, &nint) != GGEM_OK ||
+        nint > kTabMaxIntervals)
+      return bail(hipErrorInvalidValue, "FP32C coefficient table");
+    std::vector<float> tab(size_t(16) * nint);
+    if (ggem_util_fp32c_table(c.one_eight_pi_mu, c.alpha, c.ksi, int32_t(tab.size()), tab.data(), meta, &nint) !=
+        GGEM_OK)
+      return bail(hipErrorInvalidValue, "FP32C coefficient table");
+    if ((e = hipMalloc(&ctx->tab32, sizeof(float) * tab.size())) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemcpy(ctx->tab32, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(e, "hipMemcpy");
+    ctx->tab = CoefTab{meta[0], meta[1], meta[2], meta[3], meta[4], int32_t(meta[5]), int32_t(meta[6]), nint};
+  }
   if (grow_scan64(ctx, ne + 1) != GGEM_OK) return bail(hipErrorOutOfMemory, "hipMalloc");
   *out = ctx;
   return GGEM_OK;
@@ -754,7 +747,7 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
   ctx->n = n;
   ctx->built = false;
   ctx->elist_valid = false;
-  ctx->coef32_valid = ctx->coef64_valid = false;
+  ctx->coef64_valid = false;
   ctx->box_built = false;
   ctx->pairs = -1;
   if (n > 0) {
@@ -790,7 +783,7 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   const bool hyst = (ctx->p.compat_flags & GGEM_COMPAT_ELEM_HYSTERESIS) && ctx->elem_valid;
   ctx->built = false;
   ctx->elist_valid = false;
-  ctx->coef32_valid = ctx->coef64_valid = false;
+  ctx->coef64_valid = false;
   ctx->box_built = false;
   hipError_t e = launch_bin(ctx, hyst);
   if (e != hipSuccess) return hip_fail(ctx, e, "bin");

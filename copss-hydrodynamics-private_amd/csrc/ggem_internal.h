@@ -59,12 +59,14 @@ struct SysGrid {
   int32_t reach[3];  // largest stencil offset per axis, in cells
 };
 
-// FP32C pair record: the neighbour entry and the tensor (A, Bx) of the pair,
-// one 12-byte load per list step
-struct Ent32 {
-  uint32_t e;
-  float a, bx;
+// Layout of the FP32C coefficient table (ggem_util_fp32c_table): n1
+// intervals of width d1 on [0, s1), then n2 of width d2, then one zero
+// interval; 16 floats per interval.
+struct CoefTab {
+  float s1, inv1, inv2, d1, d2;
+  int32_t n1, n2, n;  // n = n1 + n2 + 1 intervals
 };
+constexpr int kTabMaxIntervals = 128;  // the kernels' LDS copy (8 KiB)
 
 // Neighbour entries carry the particle slot in the low 26 bits and the
 // periodic image of the pair in the high 6 bits: 2 bits per axis, 1 = add
@@ -162,19 +164,20 @@ struct ggem_ctx {
   uint32_t* ell = nullptr;
   int64_t ell_cap = 0;
   int64_t ell_len = 0;
-  // pair tensors of the current lists, same layout as ell, G = A I + Bx x x^T:
-  // FP32C keeps (entry, A, Bx) and forms x_ij from the cell-relative
-  // positions pc32; FP64 keeps (A, w) with w = sqrt(|Bx|) x_ij, Bx = sgn |Bx|
-  ggem::Ent32* coef32 = nullptr;
-  int64_t coef32_cap = 0;
-  bool coef32_valid = false;
+  // pair tensors, G = A I + Bx x x^T: FP64 keeps (A, w) per list entry
+  // (same layout as ell) with w = sqrt(|Bx|) x_ij, Bx = sgn |Bx|; FP32C
+  // forms x_ij from the cell-relative positions pc32 and (A, Bx) from the
+  // coefficient table at r^2, in every pair sum
   float4* pc32 = nullptr;     // [n] by slot: FP32 position relative to its cell centre, packed cell (i, j, k)
+  float4* tab32 = nullptr;    // FP32C coefficient table, 4 float4 per interval
+  ggem::CoefTab tab{};
   double4* coef64 = nullptr;
   int64_t coef64_cap = 0;
   bool coef64_valid = false;
   // FP32C neighbour records staged per brick of 8 x 8 x 8 cells (the brick
-  // widened by the stencil reach): built with coef32 when the lists carry
-  // periodic shifts; a brick without records gathers from global memory
+  // widened by the stencil reach), built on the first FP32C pair sum after a
+  // neighbour build when the lists carry periodic shifts; a brick without
+  // records gathers from global memory
   int64_t* box_cnt = nullptr;     // [n_bricks] records of each staged brick, 0 otherwise
   int64_t* box_off = nullptr;     // [n_bricks + 1] first record of each brick
   int64_t* box_ecnt = nullptr;    // [n_bricks] entries of each brick's layout
@@ -187,14 +190,13 @@ struct ggem_ctx {
   size_t box_rec_cap = 0;
   float4* box_pos = nullptr;      // [records] cell-relative FP32 position, packed box cell
   size_t box_pos_cap = 0;
-  float2* box_ab = nullptr;       // [entries] pair tensor (A, Bx) of every entry, brick layout
+  float2* box_ab = nullptr;       // [entries] pair tensor (A, Bx) from the coefficient table, brick layout
   size_t box_ab_cap = 0;
   uint16_t* box_idx = nullptr;    // [entries] the neighbour's record number (staged bricks)
   size_t box_idx_cap = 0;
   int32_t* box_max_d = nullptr;   // [2] largest record count of a staged brick, most targets of a brick
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
-  bool plain_wanted = false;      // the last lists also needed coef32 next to the staged records
   bool box_valid = false;         // ... and some brick is staged
   int bsign = 1;  // sign of Bx: +1 when xi > alpha
 
@@ -326,10 +328,9 @@ struct ChebEpi {
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
 // FP32C staged records: box_cnt (and box_max_d, cleared first) per brick
 hipError_t launch_box_count(ggem_ctx* ctx);
-// records, record numbers and pair tensors of every brick (box_idx,
-// box_ab), with box_off scanned from box_cnt; the list-order records into
-// plain when set
-hipError_t launch_box_fill(ggem_ctx* ctx, ggem::Ent32* plain);
+// records, target layout and record numbers of every brick, with box_off
+// and box_eoff scanned from the counts
+hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_targets);
 // whether the context's FP32C lists can be staged per brick
 bool box_supported(const ggem_ctx* ctx);
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,

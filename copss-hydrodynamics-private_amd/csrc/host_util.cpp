@@ -175,3 +175,127 @@ int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF) {
 }
 
 }  // extern "C"
+
+// ---- FP32C pair coefficients as a piecewise polynomial in s = r^2 ----
+//
+// G_reg(x) = A(s) I + Bx(s) x x^T with A = eps (c1 + c2), Bx = eps (c1 - c2)/s,
+// c1 = (erf(xi r) - erf(alpha r))/r, c2 = 2xi/sqrt(pi) e^{-xi^2 s} -
+// 2alpha/sqrt(pi) e^{-alpha^2 s} (ggem_stokes.C:304-357).  Both are entire
+// functions of s.  With k = max(alpha^2, xi^2) and m = min(alpha^2, xi^2),
+// [0, s1) is cut into intervals of width d1 = 2^floor(log2(1.4/k)) up to
+// s1 >= 36/k, where every term with the faster decay is below e^-36 of its
+// peak, and [s1, s_cut) into intervals of width d2 = 2^floor(log2(1.4/m))
+// up to s_cut >= 36/m; beyond, A = Bx = 0.  On each interval both
+// functions are interpolated at the 8 Chebyshev points and stored as
+// monomials in y = 2 (s - mid)/d in [-1, 1]: the interpolation error is at
+// most 2 (d k / 4)^8 / 8! <= 1.1e-8 of the function's size on the interval
+// (its 8th derivative is at most k^8 times that size), below FP32
+// rounding.  Widths are powers of two, so the kernels form the interval
+// and y exactly.
+namespace {
+
+const double kSqrtPi = 1.77245385090551602729816748334;
+
+// (A, Bx) in FP64: a power series near s = 0, the closed form elsewhere
+void coef_f64(double eps, double alpha, double ksi, double s, double* A, double* Bx) {
+  const double a2 = alpha * alpha, k2 = ksi * ksi;
+  if (s * std::max(a2, k2) < 0.25) {
+    // c1 = 2/sqrt(pi) sum (-1)^n (xi^{2n+1} - alpha^{2n+1}) s^n / (n! (2n+1)),
+    // c2 = 2/sqrt(pi) sum (-1)^n (xi^{2n+1} - alpha^{2n+1}) s^n / n!
+    // so c1 + c2 = 2/sqrt(pi) sum (-1)^n D_n s^n/n! (1 + 1/(2n+1)) and
+    // (c1 - c2)/s = 2/sqrt(pi) sum_{n>=1} (-1)^{n+1} D_n s^{n-1}/n! 2n/(2n+1),
+    // D_n = xi^{2n+1} - alpha^{2n+1}
+    double sa = 0.0, sb = 0.0, pk = ksi, pa = alpha, sn = 1.0, fact = 1.0;
+    for (int n = 0; n < 30; ++n) {
+      const double sign = n % 2 ? -1.0 : 1.0;
+      sa += sign * (pk - pa) * sn / fact * (1.0 + 1.0 / (2 * n + 1));
+      pk *= k2;
+      pa *= a2;
+      fact *= (n + 1);
+      // term n + 1 of (c1 - c2)/s: s^n / (n+1)!
+      sb -= sign * (pk - pa) * sn / fact * (2.0 * (n + 1) / (2 * n + 3));
+      sn *= s;
+    }
+    *A = eps * 2.0 / kSqrtPi * sa;
+    *Bx = -eps * 2.0 / kSqrtPi * sb;
+    return;
+  }
+  const double r = std::sqrt(s);
+  const double c1 = (std::erfc(alpha * r) - std::erfc(ksi * r)) / r;
+  const double c2 = 2.0 * ksi / kSqrtPi * std::exp(-k2 * s) - 2.0 * alpha / kSqrtPi * std::exp(-a2 * s);
+  *A = eps * (c1 + c2);
+  *Bx = eps * (c1 - c2) / s;
+}
+
+double pow2_floor(double v) { return std::ldexp(1.0, int(std::floor(std::log2(v)))); }
+
+// monomial coefficients in y of sum_k c_k T_k(y), degree 7
+void cheb_to_mono(const double* c, double* out) {
+  double T[8][8] = {};
+  T[0][0] = 1.0;
+  T[1][1] = 1.0;
+  for (int k = 2; k < 8; ++k)
+    for (int j = 0; j < 8; ++j) T[k][j] = (j ? 2.0 * T[k - 1][j - 1] : 0.0) - T[k - 2][j];
+  for (int j = 0; j < 8; ++j) {
+    out[j] = 0.0;
+    for (int k = 0; k < 8; ++k) out[j] += c[k] * T[k][j];
+  }
+}
+
+}  // namespace
+
+extern "C" int ggem_util_fp32c_table(double eps, double alpha, double ksi, int32_t cap, float* table,
+                                     float* meta, int32_t* n_intervals) {
+  if (!(eps > 0) || !(alpha > 0) || !(ksi > 0) || !meta || !n_intervals) return GGEM_ERR_INVALID_ARG;
+  const double k = std::max(alpha * alpha, ksi * ksi), m = std::min(alpha * alpha, ksi * ksi);
+  const double d1 = pow2_floor(1.4 / k), d2 = pow2_floor(1.4 / m);
+  const int n1 = int(std::ceil((36.0 / k) / d1));
+  const double s1 = n1 * d1;
+  const int n2 = std::max(0, int(std::ceil((36.0 / m - s1) / d2)));
+  const int nint = n1 + n2 + 1;  // the last interval is zero
+  *n_intervals = nint;
+  meta[0] = float(s1);
+  meta[1] = float(1.0 / d1);
+  meta[2] = float(1.0 / d2);
+  meta[3] = float(d1);
+  meta[4] = float(d2);
+  meta[5] = float(n1);
+  meta[6] = float(n2);
+  meta[7] = 0.f;
+  if (!table) return GGEM_OK;  // size query
+  if (cap < nint * 16) return GGEM_ERR_INVALID_ARG;
+  const double PI = 3.14159265358979323846;
+  for (int i = 0; i < nint; ++i) {
+    float* o = table + 16 * i;
+    if (i == nint - 1) {
+      for (int j = 0; j < 16; ++j) o[j] = 0.f;
+      break;
+    }
+    const double lo = i < n1 ? i * d1 : s1 + (i - n1) * d2;
+    const double d = i < n1 ? d1 : d2;
+    double fa[8], fb[8];
+    for (int j = 0; j < 8; ++j) {
+      const double y = std::cos((2 * j + 1) * PI / 16.0);
+      coef_f64(eps, alpha, ksi, lo + 0.5 * d * (y + 1.0), &fa[j], &fb[j]);
+    }
+    double ca[8], cb[8];
+    for (int kk = 0; kk < 8; ++kk) {
+      double sa = 0.0, sb = 0.0;
+      for (int j = 0; j < 8; ++j) {
+        const double t = std::cos(kk * (2 * j + 1) * PI / 16.0);
+        sa += fa[j] * t;
+        sb += fb[j] * t;
+      }
+      ca[kk] = (kk ? 2.0 : 1.0) / 8.0 * sa;
+      cb[kk] = (kk ? 2.0 : 1.0) / 8.0 * sb;
+    }
+    double ma[8], mb[8];
+    cheb_to_mono(ca, ma);
+    cheb_to_mono(cb, mb);
+    for (int j = 0; j < 8; ++j) {
+      o[j] = float(ma[j]);
+      o[8 + j] = float(mb[j]);
+    }
+  }
+  return GGEM_OK;
+}

@@ -56,7 +56,62 @@ struct PairArgs {
   const float4* pc32;       // FP32C position records (cell-relative offsets, packed cell)
   int32_t xcd;              // renumber blocks per XCD (xcd_block)
   int32_t exact;            // explicit minimum image (no list shift codes)
+  const float4* tab;        // FP32C (A, Bx) as a piecewise polynomial in r^2, 4 float4 per interval
+  CoefTab ct;
 };
+
+// FP32C (A, Bx) at s = r^2 from the interval table (copied to LDS): the
+// interval and y = 2 (s - mid)/d are formed exactly (widths are powers of
+// two), then two degree-7 Horner chains; past the cut-off the last
+// interval is zero.  One body for every FP32C kernel, so they all
+// evaluate bit-identical tensors.
+__device__ __forceinline__ void fp32c_coef(const float4* __restrict__ tab, const CoefTab& t, float s, float& A,
+                                           float& Bx) {
+  const bool r2 = s >= t.s1;
+  const float base = r2 ? t.s1 : 0.f;
+  const float inv = r2 ? t.inv2 : t.inv1;
+  const float d = r2 ? t.d2 : t.d1;
+  const int j = min(int((s - base) * inv), r2 ? t.n2 : t.n1 - 1);
+  const int i = (r2 ? t.n1 : 0) + j;
+  const float mid = __builtin_fmaf(float(j) + 0.5f, d, base);
+  const float y = (s - mid) * (2.f * inv);
+  const float4 a0 = tab[4 * i], a1 = tab[4 * i + 1], b0 = tab[4 * i + 2], b1 = tab[4 * i + 3];
+  float pa = __builtin_fmaf(a1.w, y, a1.z);
+  pa = __builtin_fmaf(pa, y, a1.y);
+  pa = __builtin_fmaf(pa, y, a1.x);
+  pa = __builtin_fmaf(pa, y, a0.w);
+  pa = __builtin_fmaf(pa, y, a0.z);
+  pa = __builtin_fmaf(pa, y, a0.y);
+  A = __builtin_fmaf(pa, y, a0.x);
+  float pb = __builtin_fmaf(b1.w, y, b1.z);
+  pb = __builtin_fmaf(pb, y, b1.y);
+  pb = __builtin_fmaf(pb, y, b1.x);
+  pb = __builtin_fmaf(pb, y, b0.w);
+  pb = __builtin_fmaf(pb, y, b0.z);
+  pb = __builtin_fmaf(pb, y, b0.y);
+  Bx = __builtin_fmaf(pb, y, b0.x);
+}
+
+// r^2 of an FP32C displacement, one fixed evaluation order
+__device__ __forceinline__ float fp32c_r2(float dx, float dy, float dz) {
+  return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+// p += A f + Bx (x . f) x with every product fused explicitly, so that no
+// kernel is left a choice of contraction: all FP32C pair sums add
+// bit-identical terms
+__device__ __forceinline__ void fp32c_acc(float A, float Bx, float dx, float dy, float dz, float fx, float fy,
+                                          float fz, float& p0, float& p1, float& p2) {
+  const float b = Bx * __builtin_fmaf(dz, fz, __builtin_fmaf(dy, fy, dx * fx));
+  p0 += __builtin_fmaf(A, fx, b * dx);
+  p1 += __builtin_fmaf(A, fy, b * dy);
+  p2 += __builtin_fmaf(A, fz, b * dz);
+}
+
+// the interval table into LDS (the whole block takes part)
+__device__ __forceinline__ void load_tab(float4* dst, const PairArgs& a) {
+  for (int i = threadIdx.x; i < 4 * a.ct.n; i += blockDim.x) dst[i] = a.tab[i];
+}
 
 // Launch shapes the autotuner chooses from (index 0 is the default); every
 // shape computes the same per-slot arithmetic, so results do not depend on
@@ -91,10 +146,9 @@ struct V4<double> {
   using type = double4;
 };
 
-// Pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
-template <int P, bool EXACT, bool BATCH, int BLK>
-__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, Ent32* __restrict__ c32,
-                                                double4* __restrict__ c64) {
+// FP64 pair tensors of all list entries: (A, w) with G = A I + sgn w w^T.
+template <bool EXACT, bool BATCH, int BLK>
+__global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, double4* __restrict__ c64) {
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   if (s >= a.n) return;
   const Grid& g = slot_grid<BATCH>(a, s);
@@ -115,17 +169,10 @@ __global__ void __launch_bounds__(BLK) k_tensors(PairArgs a, Ent32* __restrict__
     double dx, dy, dz;
     pair_disp<EXACT>(g, xs, xt, e >> kSlotBits, dx, dy, dz);
     const int64_t o = base + int64_t(k) * kSlice;
-    if (P == GGEM_FP64) {
-      double A, Bx;
-      green_coef_f64(a.k, dx * dx + dy * dy + dz * dz, A, Bx);
-      const double sb = sqrt(fabs(Bx));
-      c64[o] = make_double4(A, sb * dx, sb * dy, sb * dz);
-    } else {
-      const float fx = float(dx), fy = float(dy), fz = float(dz);
-      float A, Bx;
-      green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
-      c32[o] = Ent32{e, A, Bx};
-    }
+    double A, Bx;
+    green_coef_f64(a.k, dx * dx + dy * dy + dz * dz, A, Bx);
+    const double sb = sqrt(fabs(Bx));
+    c64[o] = make_double4(A, sb * dx, sb * dy, sb * dz);
   }
 }
 
@@ -205,8 +252,15 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
                                                  double* __restrict__ out) {
   // (the tensor pointer is untyped in the signature: a type computed from
   // the header's unnamed precision enum would be mangled differently by the
-  // host and device compilations)
+  // host and device compilations; FP32C keeps no tensors)
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
+  const float4* tab = nullptr;
+  if constexpr (P != GGEM_FP64) {
+    __shared__ float4 s_tab[4 * kTabMaxIntervals];
+    load_tab(s_tab, a);
+    __syncthreads();
+    tab = s_tab;
+  }
   if (s >= a.n) return;
   const int64_t n = a.n;
   const int cnt = a.cnt[s];
@@ -272,20 +326,19 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 #pragma unroll
       for (int q = 0; q < 3; ++q) acc[r][q] = ps[r][q];
   } else {
-    // FP32C: G f = A f + Bx (x . f) x with the stored (A, Bx) and x_ij from
-    // the cell-relative FP32 positions: x_ij = (cell_t - cell_s + shift) h
-    // + (offset_t - offset_s), per axis
-    const Ent32* __restrict__ cf = static_cast<const Ent32*>(coef_raw) + base;
+    // FP32C: G f = A f + Bx (x . f) x with x_ij from the cell-relative FP32
+    // positions, x_ij = (cell_t - cell_s + shift) h + (offset_t - offset_s)
+    // per axis, and (A, Bx) from the table at r^2
     const float4* __restrict__ xg = xg32;
     const float4* __restrict__ pc = a.pc32;
     const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
-    Ent32 en[U];  // records of the next group (entry and tensor together)
-    const Ent32 none{0u, 0.f, 0.f};
+    uint32_t en[U];  // entries of the next group
 #pragma unroll
-    for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? cf[int64_t(min(i, last)) * kSlice] : none;
+    for (int i = 0; i < U; ++i) en[i] = cnt > 0 ? lst[uint32_t(min(i, last)) * uint32_t(kSlice)] : 0u;
     const float hx = float(g.h[0]), hy = float(g.h[1]), hz = float(g.h[2]);
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const float Lx = float(g.L[0]), Ly = float(g.L[1]), Lz = float(g.L[2]);
+    const int px = g.periodic[0], py = g.periodic[1], pz = g.periodic[2];
     const float4 po = pc[s];
     const uint32_t co = __float_as_uint(po.w);
     const int io = int(co & ((1u << kCellBitsX) - 1));
@@ -310,21 +363,18 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
       dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
       dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
       if (a.exact) {  // minimum image (pm_periodic_boundary.C:350-374)
-        if (g.periodic[0]) dx = dx <= -0.5f * Lx ? dx + Lx : (dx > 0.5f * Lx ? dx - Lx : dx);
-        if (g.periodic[1]) dy = dy <= -0.5f * Ly ? dy + Ly : (dy > 0.5f * Ly ? dy - Ly : dy);
-        if (g.periodic[2]) dz = dz <= -0.5f * Lz ? dz + Lz : (dz > 0.5f * Lz ? dz - Lz : dz);
+        if (px) dx = dx <= -0.5f * Lx ? dx + Lx : (dx > 0.5f * Lx ? dx - Lx : dx);
+        if (py) dy = dy <= -0.5f * Ly ? dy + Ly : (dy > 0.5f * Ly ? dy - Ly : dy);
+        if (pz) dz = dz <= -0.5f * Lz ? dz + Lz : (dz > 0.5f * Lz ? dz - Lz : dz);
       }
     };
-    auto pair_acc = [&](const Ent32& c, const float4& pt, const float4* f) {
+    auto pair_acc = [&](uint32_t e, const float4& pt, const float4* f) {
       float dx, dy, dz;
-      disp(c.e, pt, dx, dy, dz);
+      disp(e, pt, dx, dy, dz);
+      float A, Bx;
+      fp32c_coef(tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const float b = c.bx * (dx * f[r].x + dy * f[r].y + dz * f[r].z);
-        ps[r][0] += c.a * f[r].x + b * dx;
-        ps[r][1] += c.a * f[r].y + b * dy;
-        ps[r][2] += c.a * f[r].z + b * dz;
-      }
+      for (int r = 0; r < NR; ++r) fp32c_acc(A, Bx, dx, dy, dz, f[r].x, f[r].y, f[r].z, ps[r][0], ps[r][1], ps[r][2]);
     };
     auto merge = [&](int k) {
       if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
@@ -339,17 +389,17 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
     };
     int k = 0;
     for (; k + U <= cnt; k += U) {
-      Ent32 c[U];
+      uint32_t c[U];
       float4 pt[U];
 #pragma unroll
       for (int i = 0; i < U; ++i) {
         c[i] = en[i];
-        en[i] = cf[int64_t(min(k + U + i, last)) * kSlice];
+        en[i] = lst[uint32_t(min(k + U + i, last)) * uint32_t(kSlice)];
       }
       float4 f[U][NR];
 #pragma unroll
       for (int i = 0; i < U; ++i) {
-        const uint32_t t = c[i].e & kSlotMask;
+        const uint32_t t = c[i] & kSlotMask;
         pt[i] = pc[t];
 #pragma unroll
         for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + t];
@@ -362,8 +412,8 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
     }
 #pragma unroll
     for (int i = 0; i < U - 1; ++i) {
-      if (k + i < cnt) {  // records en[0 .. cnt - k)
-        const uint32_t t = en[i].e & kSlotMask;
+      if (k + i < cnt) {  // entries en[0 .. cnt - k)
+        const uint32_t t = en[i] & kSlotMask;
         float4 f[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) f[r] = xg[r * n + t];
@@ -391,16 +441,16 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // the box is one record, in box order: box cells x-major through the box,
 // particles of a cell in slot order.  Once per list build k_box_fill writes
 // every brick's records (slot, cell-relative FP32 position with the packed
-// box cell), the pair tensor (A, Bx) of every list entry and, in staged
-// bricks, the neighbour's record number in its target's box (16 bits).
-// k_apply_box copies the records and their operands to LDS and streams
-// record numbers and tensors in list order (10 bytes per pair), reading the
-// neighbour from LDS.  The summation order (list order) and the arithmetic
-// are k_apply's; the displacement (box_t - box_s) h + (offset_t - offset_s)
-// has the same integer cell difference as the list entry's periodic shift
-// gives, so the results are bit-identical.  A brick whose box holds more
-// than kBoxMaxRecs records gathers from global memory (list entries and
-// the same tensors).
+// box cell) and, in staged bricks, each list entry's neighbour record
+// number in its target's box (16 bits).  k_apply_box copies the
+// coefficient table, the records and their operands to LDS and streams the
+// record numbers in list order (2 bytes per pair), reading the neighbour
+// from LDS and (A, Bx) from the table.  The summation order (list order)
+// and the arithmetic are k_apply's; the displacement (box_t - box_s) h +
+// (offset_t - offset_s) has the same integer cell difference as the list
+// entry's periodic shift gives, so the results are bit-identical.  A brick
+// whose box holds more than kBoxMaxRecs records gathers from global memory
+// (the list entries and the neighbours' records).
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxCells = 8192;  // box cells of a staged brick (16-bit starts in LDS)
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
@@ -419,9 +469,10 @@ struct BoxArgs {
   int64_t* base;         // [n] entry 0 of the target at each position of tgt
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
-  float2* ab;            // [entries] pair tensor (A, Bx), brick layout
   uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
-  Ent32* plain;          // k_box_fill: the list-order records (coef32) too, when set
+  float2* ab;            // [entries] pair tensor (A, Bx) from the coefficient table, brick layout
+  int32_t cells_cap;     // k_box_fill: box cells and waves of a brick its LDS tables hold
+  int32_t waves_cap;
 };
 
 // Entry layout of a brick: its targets in the order tgt, in waves of 64;
@@ -603,19 +654,15 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
   }
 }
 
-// Records of staged bricks, and the FP32C pair records of every brick's
-// targets: the tensor (A, Bx) as k_tensors evaluates it (FP64 displacement
-// with the entry's periodic shift, then green_coef_f32), with the
-// neighbour's record number as index in staged bricks and the plain list
-// entry otherwise.  The list-order records of the global-gather kernels are
-// written from the same registers when requested, so that both kernels use
-// bit-identical tensors (two inlined copies of the evaluation may contract
-// their products differently).
+// Records of staged bricks, the entry layout of every brick's targets, and
+// in staged bricks each entry's neighbour record number.
 template <bool BATCH>
 __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b) {
-  __shared__ uint16_t cst[kBoxMaxCells + 1];  // record start of each box cell
-  __shared__ int32_t cfirst[kBoxMaxCells];    // first slot of each box cell
-  __shared__ int32_t wpre[kBoxMaxWaves + 1];  // entry rows before each wave
+  extern __shared__ float4 dyn[];
+  float4* s_tab = dyn;                                                   // coefficient table
+  int32_t* cfirst = reinterpret_cast<int32_t*>(dyn + 4 * a.ct.n);        // first slot of each box cell
+  int32_t* wpre = cfirst + b.cells_cap;                                  // entry rows before each wave
+  uint16_t* cst = reinterpret_cast<uint16_t*>(wpre + b.waves_cap + 1);   // record start of each box cell
   __shared__ int32_t s_w[kBoxThreads / 64];
   const int64_t brick = blockIdx.x;
   const int64_t c0 = brick * kBrickCells;
@@ -623,6 +670,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
   if (sb >= se) return;
   const int nt = se - sb;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  load_tab(s_tab, a);  // (visible after the barriers below)
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   BoxGeom B;
@@ -696,55 +744,56 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     }
   }
   __syncthreads();
-  // (5) pair records of the brick's targets; the next entry's position
-  // records are loaded while the current one is evaluated
+  // (5) the entries of the brick's targets: the pair tensor (A, Bx) from
+  // the coefficient table at the FP32C displacement, exactly as the
+  // list-order kernel forms both, and in staged bricks the neighbour's
+  // record number; the next entry's position record is loaded while the
+  // current one is evaluated
+  const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
     const int64_t bb = b.eoff[brick] + 64 * int64_t(wpre[i >> 6]) + (i & 63);
     b.base[sb + i] = bb;
-    const Grid& g = slot_grid<BATCH>(a, s);
     const int cnt = a.cnt[s];
     if (cnt == 0) continue;  // an empty list has no storage
     const int64_t base = a.slice_off[s >> 6] + (s & 63);
-    const double4 xs = a.x4[s];
     const int last = cnt - 1;
+    const float4 po = a.pc32[s];
+    const uint32_t co = __float_as_uint(po.w);
+    const int io = int(co & ((1u << kCellBitsX) - 1));
+    const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+    const int ko = int(co >> (kCellBitsX + kCellBitsY));
     uint32_t e1 = a.ell[base];
-    double4 x1 = a.x4[e1 & kSlotMask];
     float4 p1 = a.pc32[e1 & kSlotMask];
     for (int k = 0; k < cnt; ++k) {
-      const int64_t o = base + int64_t(k) * kSlice;
-      const int64_t ob = bb + int64_t(k) * kSlice;
       const uint32_t e = e1;
-      const double4 xt = x1;
       const float4 pt = p1;
       e1 = a.ell[base + int64_t(min(k + 1, last)) * kSlice];
-      x1 = a.x4[e1 & kSlotMask];
       p1 = a.pc32[e1 & kSlotMask];
       const uint32_t t = e & kSlotMask;
       const uint32_t code = e >> kSlotBits;
-      double dx, dy, dz;
-      pair_disp<false>(g, xs, xt, code, dx, dy, dz);
-      const float fx = float(dx), fy = float(dy), fz = float(dz);
+      // the neighbour's unwrapped cell: its cell plus the entry's periodic shift
+      const uint32_t ct = __float_as_uint(pt.w);
+      const int ci = int(ct & ((1u << kCellBitsX) - 1));
+      const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
+      const int ck = int(ct >> (kCellBitsX + kCellBitsY));
+      const int ui = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0];
+      const int uj = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1];
+      const int uk = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2];
+      const float dx = __builtin_fmaf(float(ui - io), hx, pt.x - po.x);
+      const float dy = __builtin_fmaf(float(uj - jo), hy, pt.y - po.y);
+      const float dz = __builtin_fmaf(float(uk - ko), hz, pt.z - po.z);
       float A, Bx;
-      green_coef_f32(a.k32, fx * fx + fy * fy + fz * fz, A, Bx);
-      b.ab[ob] = make_float2(A, Bx);
-      if (b.plain) b.plain[o] = Ent32{e, A, Bx};
-      if (nrec > 0) {
-        // the neighbour's unwrapped cell (its cell plus the entry's periodic
-        // shift) gives its box cell, its rank in the cell the record
-        const uint32_t ct = __float_as_uint(pt.w);
-        const int ci = int(ct & ((1u << kCellBitsX) - 1));
-        const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
-        const int ck = int(ct >> (kCellBitsX + kCellBitsY));
-        const int x = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0] - B.ox;
-        const int y = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1] - B.oy;
-        const int z = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2] - B.oz;
+      fp32c_coef(s_tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
+      b.ab[bb + int64_t(k) * kSlice] = make_float2(A, Bx);
+      if (nrec > 0) {  // its box cell, and its rank in the cell, give the record
+        const int x = ui - B.ox, y = uj - B.oy, z = uk - B.oz;
         int r = 0;
         if (x >= 0 && x < B.ex && y >= 0 && y < B.ey && z >= 0 && z < B.ez) {
           const int q = (z * B.ey + y) * B.ex + x;
           r = int(cst[q]) + int(int64_t(t) - cfirst[q]);
         }
-        b.idx[ob] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay inside the brick's records
+        b.idx[bb + int64_t(k) * kSlice] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay in the records
       }
     }
   }
@@ -767,9 +816,7 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   float ps[3] = {0.f, 0.f, 0.f};
   KahanF kh[3];
   for (int k = 0; k < cnt; ++k) {
-    const float2 t2 = ab[bb + int64_t(k) * kSlice];
-    const Ent32 c{a.ell[base + int64_t(k) * kSlice], t2.x, t2.y};
-    const uint32_t e = c.e;
+    const uint32_t e = a.ell[base + int64_t(k) * kSlice];
     const uint32_t t = e & kSlotMask;
     const float4 pt = a.pc32[t];
     const float4 f = xg[t];
@@ -782,10 +829,8 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
     const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
     const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
     const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-    const float b = c.bx * (dx * f.x + dy * f.y + dz * f.z);
-    ps[0] += c.a * f.x + b * dx;
-    ps[1] += c.a * f.y + b * dy;
-    ps[2] += c.a * f.z + b * dz;
+    const float2 c = ab[bb + int64_t(k) * kSlice];
+    fp32c_acc(c.x, c.y, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
     if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -814,6 +859,8 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   const int nt = se - sb;
+  float4* rpos = lds;         // [nrec] offset xyz, packed box cell
+  float4* rval = lds + nrec;  // [nrec] operand
   if (nrec == 0) {
     for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
       const int64_t s = b.tgt[sb + i];
@@ -823,8 +870,6 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     }
     return;
   }
-  float4* rpos = lds;         // [nrec] offset xyz, packed box cell
-  float4* rval = lds + nrec;  // [nrec] operand
   {
     // four records in flight per thread, then the remainder one at a time
     int i = int(threadIdx.x);
@@ -864,9 +909,9 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     const int cnt = a.cnt[s];
     float ps[3] = {0.f, 0.f, 0.f};
     KahanF kh[3];
-    auto pair = [&](const Ent32& c) {
-      const float4 pt = rpos[c.e];
-      const float4 f = rval[c.e];
+    auto pair = [&](uint32_t rec, float2 c) {
+      const float4 pt = rpos[rec];
+      const float4 f = rval[rec];
       const uint32_t bt = __float_as_uint(pt.w);
       const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
       const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
@@ -874,10 +919,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
       const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
       const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
       const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-      const float bb = c.bx * (dx * f.x + dy * f.y + dz * f.z);
-      ps[0] += c.a * f.x + bb * dx;
-      ps[1] += c.a * f.y + bb * dy;
-      ps[2] += c.a * f.z + bb * dz;
+      fp32c_acc(c.x, c.y, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
     };
     auto merge = [&](int k) {
       if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
@@ -892,32 +934,36 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
       const uint16_t* __restrict__ ci = b.idx + base;
       const float2* __restrict__ cab = b.ab + base;
       const int last = cnt - 1;
-      auto load = [&](int k) {
-        const int64_t o = int64_t(min(k, last)) * kSlice;
-        const float2 t = cab[o];
-        return Ent32{uint32_t(ci[o]), t.x, t.y};
-      };
-      Ent32 en[U];
+      uint32_t en[U];  // record numbers and tensors of the next group
+      float2 et[U];
 #pragma unroll
-      for (int i = 0; i < U; ++i) en[i] = load(i);
+      for (int i = 0; i < U; ++i) {
+        const int64_t o = int64_t(min(i, last)) * kSlice;
+        en[i] = ci[o];
+        et[i] = cab[o];
+      }
       int k = 0;
       for (; k + U <= cnt; k += U) {
-        Ent32 c[U];
+        uint32_t c[U];
+        float2 t[U];
 #pragma unroll
         for (int i = 0; i < U; ++i) {
           c[i] = en[i];
-          en[i] = load(k + U + i);
+          t[i] = et[i];
+          const int64_t o = int64_t(min(k + U + i, last)) * kSlice;
+          en[i] = ci[o];
+          et[i] = cab[o];
         }
 #pragma unroll
         for (int i = 0; i < U; ++i) {
-          pair(c[i]);
+          pair(c[i], t[i]);
           merge(k + i);
         }
       }
 #pragma unroll
       for (int i = 0; i < U - 1; ++i) {
         if (k + i < cnt) {  // entries en[0 .. cnt - k)
-          pair(en[i]);
+          pair(en[i], et[i]);
           merge(k + i);
         }
       }
@@ -1014,6 +1060,8 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.pc32 = ctx->pc32;
   a.xcd = 1;
   a.exact = ctx->exact_minimg;
+  a.tab = ctx->tab32;
+  a.ct = ctx->tab;
   return a;
 }
 
@@ -1025,7 +1073,7 @@ void launch_kb(ggem_ctx* ctx, const PairArgs& a, const double4* g64, const float
                        ctx->coef64, g64, g32, soa, v, epi, u_glob, out);
   else
     hipLaunchKernelGGL((k_apply<P, NR, EPI, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
-                       ctx->coef32, g64, g32, soa, v, epi, u_glob, out);
+                       nullptr, g64, g32, soa, v, epi, u_glob, out);
 }
 
 BoxArgs box_args(const ggem_ctx* ctx) {
@@ -1041,9 +1089,8 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.base = ctx->box_base;
   b.slot = ctx->box_slot;
   b.pos = ctx->box_pos;
-  b.ab = ctx->box_ab;
   b.idx = ctx->box_idx;
-  b.plain = nullptr;
+  b.ab = ctx->box_ab;
   return b;
 }
 
@@ -1080,8 +1127,6 @@ hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const 
     return hipErrorInvalidConfiguration;
   }
   if (c < 0 || c >= kNApply) return hipErrorInvalidValue;
-  // list-order FP32C tensors are evaluated only where a launch needs them
-  if (P == GGEM_FP32C && !ctx->coef32_valid) return hipErrorInvalidConfiguration;
   const Shape sh = kApplyShapes[c];
   a.xcd = sh.xcd;
   switch (sh.block) {
@@ -1104,43 +1149,41 @@ hipError_t launch_cheb_nr(ggem_ctx* ctx, const PairArgs& a, int c, int nr, const
   }
 }
 
-template <int P, bool E, bool B, int BLK>
+template <bool E, bool B, int BLK>
 void launch_tb(ggem_ctx* ctx, const PairArgs& a) {
-  hipLaunchKernelGGL((k_tensors<P, E, B, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
-                     ctx->coef32, ctx->coef64);
+  hipLaunchKernelGGL((k_tensors<E, B, BLK>), dim3(blocks_for(a.n, BLK)), dim3(BLK), 0, ctx->stream, a,
+                     ctx->coef64);
 }
 
-template <int P, bool E, bool B>
+template <bool E, bool B>
 hipError_t launch_t(ggem_ctx* ctx, PairArgs a, int c) {
   if (c < 0 || c >= kNTensor) return hipErrorInvalidValue;
   const Shape sh = kTensorShapes[c];
   a.xcd = sh.xcd;
   switch (sh.block) {
-    case 64: launch_tb<P, E, B, 64>(ctx, a); break;
-    case 128: launch_tb<P, E, B, 128>(ctx, a); break;
-    case 512: launch_tb<P, E, B, 512>(ctx, a); break;
-    default: launch_tb<P, E, B, 256>(ctx, a); break;
+    case 64: launch_tb<E, B, 64>(ctx, a); break;
+    case 128: launch_tb<E, B, 128>(ctx, a); break;
+    case 512: launch_tb<E, B, 512>(ctx, a); break;
+    default: launch_tb<E, B, 256>(ctx, a); break;
   }
   return hipGetLastError();
 }
 
-template <int P>
 hipError_t launch_t_any(ggem_ctx* ctx, const PairArgs& a, int c) {
   const bool ex = ctx->exact_minimg != 0, bt = ctx->nsys > 1;
-  if (bt) return ex ? launch_t<P, true, true>(ctx, a, c) : launch_t<P, false, true>(ctx, a, c);
-  return ex ? launch_t<P, true, false>(ctx, a, c) : launch_t<P, false, false>(ctx, a, c);
+  if (bt) return ex ? launch_t<true, true>(ctx, a, c) : launch_t<false, true>(ctx, a, c);
+  return ex ? launch_t<true, false>(ctx, a, c) : launch_t<false, false>(ctx, a, c);
 }
 
 }  // namespace
 
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
+  if (precision != GGEM_FP64) return hipErrorInvalidValue;  // FP32C evaluates its table in the pair sums
   if (ctx->n == 0) return hipSuccess;
   const PairArgs a = make_args(ctx);
-  auto run = [&](int c) {
-    return precision == GGEM_FP64 ? launch_t_any<GGEM_FP64>(ctx, a, c) : launch_t_any<GGEM_FP32C>(ctx, a, c);
-  };
+  auto run = [&](int c) { return launch_t_any(ctx, a, c); };
   const int c = tune_pick(ctx, tune_key("tensors", precision, ctx->nsys > 1, ctx->n), kNTensor, run);
-  ProfScope ps(ctx, precision == GGEM_FP64 ? "tensors_fp64" : "tensors_fp32c");
+  ProfScope ps(ctx, "tensors_fp64");
   return run(c);
 }
 
@@ -1155,14 +1198,26 @@ hipError_t launch_box_count(ggem_ctx* ctx) {
   return hipGetLastError();
 }
 
-hipError_t launch_box_fill(ggem_ctx* ctx, Ent32* plain) {
+hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_targets) {
   ProfScope ps(ctx, "box_fill");
   BoxArgs b = box_args(ctx);
-  b.plain = plain;
+  // LDS tables sized for the largest box and the most crowded brick
+  auto cells = [](const int32_t* r) {
+    return ((1 << kBrickBits) + 2 * r[0]) * ((1 << kBrickBits) + 2 * r[1]) * ((1 << kBrickBits) + 2 * r[2]);
+  };
+  int cap = cells(ctx->reach);
+  for (int k = 0; k < ctx->nsys && ctx->nsys > 1; ++k) cap = std::max(cap, cells(ctx->sys_h[k].reach));
+  b.cells_cap = std::min(cap, kBoxMaxCells);
+  b.waves_cap = (max_targets + 63) / 64;
+  if (b.waves_cap > kBoxMaxWaves) return hipErrorInvalidValue;
+  const size_t lds = sizeof(float4) * 4 * size_t(ctx->tab.n) + sizeof(int32_t) * (b.cells_cap + b.waves_cap + 1) +
+                     sizeof(uint16_t) * (b.cells_cap + 1);
   if (ctx->nsys > 1)
-    hipLaunchKernelGGL(k_box_fill<true>, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx), b);
+    hipLaunchKernelGGL(k_box_fill<true>, dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, make_args(ctx),
+                       b);
   else
-    hipLaunchKernelGGL(k_box_fill<false>, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx), b);
+    hipLaunchKernelGGL(k_box_fill<false>, dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream,
+                       make_args(ctx), b);
   return hipGetLastError();
 }
 

@@ -266,6 +266,15 @@ int ggem_util_normal(int64_t n, uint64_t seed, double* out);
 /* IF table of a HEX20 element of size h[3] with nqp_1d^3 Gauss points:
  * IF[l*nqp + qp] = JxW[qp] * phi_l(qp), libMesh HEX20 node order. */
 int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*nqp */);
+/* The FP32C pair coefficients (A, Bx) of G_reg = A I + Bx x x^T
+ * (ggem_stokes.C:304-357) as a piecewise polynomial in s = r^2, the form
+ * the FP32C kernels evaluate: n_intervals intervals of 16 floats (A's 8
+ * monomial coefficients in y = 2 (s - mid)/d, then Bx's), the last one
+ * zero (s beyond the cut-off); meta = {s1, 1/d1, 1/d2, d1, d2, n1, n2, 0}
+ * (intervals of width d1 on [0, s1), d2 after).  table = NULL queries
+ * n_intervals; cap is the table's size in floats. */
+int ggem_util_fp32c_table(double eps, double alpha, double ksi, int32_t cap, float* table, float* meta,
+                          int32_t* n_intervals);
 
 /* ---- launch-shape autotuning -------------------------------------------- */
 /* enable != 0: the first launch of the pair-tensor, pair-sum and mobility
