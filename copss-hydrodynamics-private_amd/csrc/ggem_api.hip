@@ -169,7 +169,7 @@ int prepare_box(ggem_ctx* ctx) {
   const size_t ne = size_t(std::max<int64_t>(total[1], 1));
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
   if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
-  if ((e = launch_box_fill(ctx, mx[1])) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  if ((e = launch_box_fill(ctx, mx[0], mx[1])) != hipSuccess) return hip_fail(ctx, e, "box_fill");
   ctx->box_max = mx[0];
   ctx->box_valid = true;
   return GGEM_OK;

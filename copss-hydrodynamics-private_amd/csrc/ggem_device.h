@@ -68,6 +68,193 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
   return x * q + (x < rmd ? x : rmd) + i;
 }
 
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// x mod n for x a few periods at most outside [0, n) (no integer division)
+__device__ __forceinline__ int wrap_axis(int x, int n) {
+  while (x < 0) x += n;
+  while (x >= n) x -= n;
+  return x;
+}
+
+// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
+// away from an integer, far more than the FP32 rounding of the product
+__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
+
+constexpr int kBrickCells = 1 << (3 * kBrickBits);
+constexpr int kBoxMaxCells = 8192;  // box cells whose tables a brick kernel keeps in LDS
+constexpr int kBoxBits = 6;         // box coordinates packed 6 bits per axis
+
+// Box of one brick: the brick widened by the stencil reach on every side,
+// origin (ox, oy, oz) and extent (ex, ey, ez) in cells of the brick's grid,
+// periodic images unwrapped; box cells are numbered x-major through the
+// box.  The grid fields are copied (no pointer into the kernel arguments,
+// which would force a private copy of them).
+struct BoxGeom {
+  int64_t coff;
+  int n[3], per[3], nb0, nb1;
+  float h[3];
+  int ox, oy, oz, ex, ey, ez, nbc;
+  int bx0, by0, bz0;  // the brick's first cell
+  float inv_ex, inv_ey;
+
+  // coff: first brick-order cell of the grid's system
+  __device__ __forceinline__ void init(const Grid& g, int64_t cell_off, const int32_t* reach, int64_t brick) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      n[d] = g.n[d];
+      per[d] = g.periodic[d];
+      h[d] = float(g.h[d]);
+    }
+    nb0 = g.nb[0];
+    nb1 = g.nb[1];
+    coff = cell_off;
+    const int rx = reach[0], ry = reach[1], rz = reach[2];
+    const int64_t lb = brick - (coff >> (3 * kBrickBits));
+    const int64_t bxy = int64_t(nb0) * nb1;
+    const int bk = int(lb / bxy);
+    const int64_t rem = lb - int64_t(bk) * bxy;
+    const int bj = int(rem / nb0);
+    const int bi = int(rem - int64_t(bj) * nb0);
+    bx0 = bi << kBrickBits;
+    by0 = bj << kBrickBits;
+    bz0 = bk << kBrickBits;
+    ox = bx0 - rx;
+    oy = by0 - ry;
+    oz = bz0 - rz;
+    ex = (1 << kBrickBits) + 2 * rx;
+    ey = (1 << kBrickBits) + 2 * ry;
+    ez = (1 << kBrickBits) + 2 * rz;
+    nbc = ex * ey * ez;
+    inv_ex = 1.0f / float(ex);
+    inv_ey = 1.0f / float(ey);
+  }
+  __device__ __forceinline__ bool fits() const {
+    return nbc <= kBoxMaxCells && ex <= (1 << kBoxBits) && ey <= (1 << kBoxBits) && ez <= (1 << kBoxBits);
+  }
+  __device__ __forceinline__ void xyz(int q, int& x, int& y, int& z) const {
+    const int yz = div_small(q, inv_ex);
+    x = q - yz * ex;
+    z = div_small(yz, inv_ey);
+    y = yz - z * ey;
+  }
+  // brick-order index of grid cell (i, j, k) of the system
+  __device__ __forceinline__ int64_t index(int i, int j, int k) const {
+    return coff + cell_index_nb(nb0, nb1, i, j, k);
+  }
+  // global cell of box cell q; false outside a non-periodic domain
+  __device__ __forceinline__ bool cell(int q, int64_t& c) const {
+    int x, y, z;
+    xyz(q, x, y, z);
+    int X = ox + x, Y = oy + y, Z = oz + z;
+    if (X < 0 || X >= n[0]) {
+      if (!per[0]) return false;
+      X = wrap_axis(X, n[0]);
+    }
+    if (Y < 0 || Y >= n[1]) {
+      if (!per[1]) return false;
+      Y = wrap_axis(Y, n[1]);
+    }
+    if (Z < 0 || Z >= n[2]) {
+      if (!per[2]) return false;
+      Z = wrap_axis(Z, n[2]);
+    }
+    c = index(X, Y, Z);
+    return true;
+  }
+};
+
+// The box cells of a brick in LDS, whole block of NT threads: first[q] =
+// first slot of box cell q's grid cell (when first is not null), cst[q] =
+// particles of the box cells before q (cst[nbc] = all of them).  Cells
+// outside a non-periodic domain are empty.  s_w holds NT/64 ints.  Ends
+// with a barrier.
+template <int NT>
+__device__ __forceinline__ void stage_box_cells(const BoxGeom& B, const int32_t* __restrict__ cs,
+                                                int32_t* first, int32_t* cst, int32_t* s_w) {
+  const int per = (B.nbc + NT - 1) / NT;
+  const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
+  int mine = 0;
+  for (int q = q0; q < q1; ++q) {
+    int64_t c;
+    int k = 0, f = 0;
+    if (B.cell(q, c)) {
+      f = cs[c];
+      k = cs[c + 1] - f;
+    }
+    cst[q] = k;
+    if (first) first[q] = f;
+    mine += k;
+  }
+  int incl = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if ((threadIdx.x & 63) >= o) incl += y;
+  }
+  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int run = incl - mine;
+  for (int w = 0; w < int(threadIdx.x >> 6); ++w) run += s_w[w];
+  for (int q = q0; q < q1; ++q) {
+    const int k = cst[q];
+    cst[q] = run;
+    run += k;
+  }
+  if (threadIdx.x == NT - 1) cst[B.nbc] = run;  // the last thread ends at the total
+  __syncthreads();
+}
+
+// The stencil of the element in grid cell (ci, cj, ck) of the brick as runs
+// of box cells: calls run(q0, q1, code) for box cells q0..q1 of one row
+// with the periodic shift code of that run, in the order for_each_segment
+// visits the same cells (rows in stencil order; within a row the part past
+// the low face, then the part past the high face, then the rest, x
+// ascending), so the particles come out in the same order.
+template <class Run>
+__device__ __forceinline__ void box_walk(const BoxGeom& B, const StencilRow* __restrict__ rows, int nrows,
+                                         int ci, int cj, int ck, Run&& run) {
+  for (int r = 0; r < nrows; ++r) {
+    const StencilRow R = rows[r];
+    const int jj = cj + R.dj, kk = ck + R.dk;
+    uint32_t cy = 0, cz = 0;
+    if (jj < 0 || jj >= B.n[1]) {
+      if (!B.per[1]) continue;
+      cy = jj < 0 ? 2u : 1u;
+    }
+    if (kk < 0 || kk >= B.n[2]) {
+      if (!B.per[2]) continue;
+      cz = kk < 0 ? 2u : 1u;
+    }
+    const uint32_t cyz = (cy << 2) | (cz << 4);
+    const int rowq = ((kk - B.oz) * B.ey + (jj - B.oy)) * B.ex - B.ox;  // box cell of unwrapped x = rowq + x
+    int lo = ci + R.di_lo, hi = ci + R.di_hi;
+    if (B.per[0]) {
+      if (lo < 0) {
+        run(rowq + lo, rowq - 1, cyz | 2u);
+        lo = 0;
+      }
+      if (hi >= B.n[0]) {
+        run(rowq + B.n[0], rowq + hi, cyz | 1u);
+        hi = B.n[0] - 1;
+      }
+    } else {
+      lo = lo < 0 ? 0 : lo;
+      hi = hi >= B.n[0] ? B.n[0] - 1 : hi;
+    }
+    if (lo <= hi) run(rowq + lo, rowq + hi, cyz);
+  }
+}
+
 // Walk the particle slots of every element of the stencil of element
 // (ci, cj, ck): calls seg(a, b, code) for each contiguous slot range [a, b)
 // with the periodic shift code of that range (a row's x-run is one range
@@ -186,17 +373,6 @@ __device__ __forceinline__ void green_acc_f64(const GGEMConst& k, double dx, dou
 }
 
 // Kahan-compensated FP32 accumulator.
-// x mod n for x a few periods at most outside [0, n) (no integer division)
-__device__ __forceinline__ int wrap_axis(int x, int n) {
-  while (x < 0) x += n;
-  while (x >= n) x -= n;
-  return x;
-}
-
-// q / d for 0 <= q < 2^14, 1 <= d <= 64: (q + 1/2) / d is at least 1/128
-// away from an integer, far more than the FP32 rounding of the product
-__device__ __forceinline__ int div_small(int q, float inv_d) { return int((float(q) + 0.5f) * inv_d); }
-
 struct KahanF {
   float s = 0.f, c = 0.f;
   __device__ __forceinline__ void add(float v) {

@@ -330,7 +330,7 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
 hipError_t launch_box_count(ggem_ctx* ctx);
 // records, target layout and record numbers of every brick, with box_off
 // and box_eoff scanned from the counts
-hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_targets);
+hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_records, int32_t max_targets);
 // whether the context's FP32C lists can be staged per brick
 bool box_supported(const ggem_ctx* ctx);
 hipError_t launch_pair_beads(ggem_ctx* ctx, int precision, int epi, const double* u_glob,

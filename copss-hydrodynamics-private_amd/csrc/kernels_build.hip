@@ -272,26 +272,102 @@ __global__ void k_gather_forces(int64_t n, const int32_t* __restrict__ perm,
   fg64[s] = make_double4(fx, fy, fz, 0.0);
 }
 
-// Neighbour-list lengths by slot (own element + stencil, minus itself), the
-// slice lengths (64 x longest list of each wave of 64 slots) and the total.
+// ---- per-brick list builds: the brick's box cells staged in LDS ----
+//
+// One workgroup per brick of 8 x 8 x 8 cells.  The cell starts of the
+// brick's box (the brick widened by the stencil reach) go to LDS once:
+// per box cell the first slot of its grid cell and the particles of the
+// box cells before it.  Each target (slot or element) of the brick then
+// walks its stencil rows through LDS (box_walk), in for_each_segment's
+// order, so the lists and counts equal the per-slot walks over the global
+// cell starts, which serve bricks whose box does not fit the LDS tables.
+constexpr int kBrickThreads = 256;
+
+struct BrickArgs {
+  Grid g;
+  const StencilRow* rows;
+  int nrows;
+  int32_t reach[3];
+  const int32_t* cs;         // cell starts, brick order, all systems
+  const int32_t* slot_cell;  // brick-order cell of each slot
+  const SysGrid* sys;        // batched contexts (null: one system)
+  const int32_t* sys_slot;
+  int32_t cells_cap;         // box cells the LDS tables hold
+};
+
+// grid, rows and box of the brick whose first slot is sb
 template <bool BATCH>
-__global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int nrows,
-                              const int32_t* __restrict__ cell_start,
-                              const int32_t* __restrict__ slot_cell, const SysGrid* __restrict__ sys,
-                              const int32_t* __restrict__ sys_slot, int64_t n,
-                              int32_t* __restrict__ cnt, int64_t* __restrict__ slice_len,
-                              unsigned long long* __restrict__ total) {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  int64_t c = 0;
-  if (s < n) {
-    const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
-    int ci, cj, ck;
-    cell_ijk(v.g, slot_cell[s] - v.coff, ci, cj, ck);
-    for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck,
-                     [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
-    c -= 1;
-    cnt[s] = int32_t(c);
+__device__ __forceinline__ SysView brick_view(const BrickArgs& a, int64_t brick, int64_t sb, BoxGeom& B) {
+  const SysView v = sys_view<BATCH>(a.g, a.rows, a.nrows, a.cs, a.sys, a.sys_slot, sb);
+  if constexpr (BATCH) B.init(v.g, v.coff, a.sys[a.sys_slot[sb]].reach, brick);
+  else B.init(v.g, 0, a.reach, brick);
+  return v;
+}
+
+__device__ __forceinline__ bool brick_staged(const BrickArgs& a, const BoxGeom& B) {
+  return B.nbc <= a.cells_cap && B.fits();
+}
+
+// Slot of every record of the box (record = particle of a box cell, in box
+// order) after stage_box_cells: a stencil run of box cells q0..q1 is then
+// records cst[q0] .. cst[q1 + 1] - 1.  False (nothing written) when the
+// box holds more than cap records.  Uniform over the block; ends with a
+// barrier when it writes.
+constexpr int kBrickRecCap = 2048;
+
+__device__ __forceinline__ bool stage_box_slots(const BoxGeom& B, const int32_t* first, const int32_t* cst,
+                                                int32_t* rslot, int cap) {
+  if (cst[B.nbc] > cap) return false;
+  const int per = (B.nbc + kBrickThreads - 1) / kBrickThreads;
+  const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
+  for (int q = q0; q < q1; ++q) {
+    const int f = first[q], r0 = cst[q];
+    for (int r = r0, e = cst[q + 1]; r < e; ++r) rslot[r] = f + (r - r0);
   }
+  __syncthreads();
+  return true;
+}
+
+// List lengths of the brick's slots (own element + stencil, minus itself).
+template <bool BATCH>
+__global__ void __launch_bounds__(kBrickThreads) k_brick_count(BrickArgs a, int32_t* __restrict__ cnt) {
+  extern __shared__ int32_t cst[];  // [cells_cap + 1]
+  __shared__ int32_t s_w[kBrickThreads / 64];
+  const int64_t brick = blockIdx.x;
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = a.cs[c0], se = a.cs[c0 + kBrickCells];
+  if (sb >= se) return;  // uniform over the block
+  BoxGeom B;
+  const SysView v = brick_view<BATCH>(a, brick, sb, B);
+  if (brick_staged(a, B)) {
+    stage_box_cells<kBrickThreads>(B, a.cs, nullptr, cst, s_w);
+    for (int32_t s = sb + int32_t(threadIdx.x); s < se; s += kBrickThreads) {
+      int ci, cj, ck;
+      cell_ijk(v.g, a.slot_cell[s] - v.coff, ci, cj, ck);
+      int c = -1;
+      box_walk(B, v.rows, v.nrows, ci, cj, ck, [&](int q0, int q1, uint32_t) { c += cst[q1 + 1] - cst[q0]; });
+      cnt[s] = c;
+    }
+  } else {
+    for (int32_t s = sb + int32_t(threadIdx.x); s < se; s += kBrickThreads) {
+      int ci, cj, ck;
+      cell_ijk(v.g, a.slot_cell[s] - v.coff, ci, cj, ck);
+      int c = -1;
+      for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck, [&](int32_t p, int32_t q, uint32_t) { c += q - p; });
+      cnt[s] = c;
+    }
+  }
+}
+
+// Slice lengths (64 x the longest list of each wave of 64 slots) and the
+// pair total from the list lengths (one atomic per block of 1024).
+constexpr int kSliceBlock = 1024;
+__global__ void __launch_bounds__(kSliceBlock) k_slice_len(const int32_t* __restrict__ cnt, int64_t n,
+                                                           int64_t* __restrict__ slice_len,
+                                                           unsigned long long* __restrict__ total) {
+  __shared__ int64_t s_sum[kSliceBlock / 64];
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t c = s < n ? cnt[s] : 0;
   int64_t mx = c, sum = c;
   for (int off = 32; off > 0; off >>= 1) {
     mx = max(mx, (int64_t)__shfl_down(mx, off, 64));
@@ -300,45 +376,100 @@ __global__ void k_count_lists(Grid g, const StencilRow* __restrict__ rows, int n
   if ((threadIdx.x & 63) == 0) {
     const int64_t q = s >> 6;
     if (q * 64 < n) slice_len[q] = mx * 64;
-    if (sum) atomicAdd(total, (unsigned long long)sum);
+    s_sum[threadIdx.x >> 6] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t b = 0;
+    for (int w = 0; w < kSliceBlock / 64; ++w) b += s_sum[w];
+    if (b) atomicAdd(total, (unsigned long long)b);
   }
 }
 
+// The lists of the brick's slots in the sliced layout.
 template <bool BATCH>
-__global__ void k_fill_ell(Grid g, const StencilRow* __restrict__ rows, int nrows,
-                           const int32_t* __restrict__ cell_start,
-                           const int32_t* __restrict__ slot_cell, const SysGrid* __restrict__ sys,
-                           const int32_t* __restrict__ sys_slot, int64_t n,
-                           const int64_t* __restrict__ slice_off, uint32_t* __restrict__ ell) {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const SysView v = sys_view<BATCH>(g, rows, nrows, cell_start, sys, sys_slot, s);
-  int ci, cj, ck;
-  cell_ijk(v.g, slot_cell[s] - v.coff, ci, cj, ck);
-  uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
-  int64_t k = 0;
-  for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck, [&](int32_t a, int32_t b, uint32_t code) {
-    const uint32_t hi = code << kSlotBits;
-    for (int32_t t = a; t < b; ++t) {
-      if (t == s) continue;
-      out[k * kSlice] = uint32_t(t) | hi;
-      ++k;
+__global__ void __launch_bounds__(kBrickThreads) k_brick_fill_ell(BrickArgs a, const int64_t* __restrict__ slice_off,
+                                                                  uint32_t* __restrict__ ell) {
+  extern __shared__ int32_t lds[];
+  __shared__ int32_t s_w[kBrickThreads / 64];
+  int32_t* first = lds;                  // [cells_cap]
+  int32_t* cst = lds + a.cells_cap;      // [cells_cap + 1]
+  int32_t* rslot = cst + a.cells_cap + 1;  // [kBrickRecCap]
+  const int64_t brick = blockIdx.x;
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = a.cs[c0], se = a.cs[c0 + kBrickCells];
+  if (sb >= se) return;
+  BoxGeom B;
+  const SysView v = brick_view<BATCH>(a, brick, sb, B);
+  const bool staged = brick_staged(a, B);
+  bool recs = false;
+  if (staged) {
+    stage_box_cells<kBrickThreads>(B, a.cs, first, cst, s_w);
+    recs = stage_box_slots(B, first, cst, rslot, kBrickRecCap);
+  }
+  for (int32_t s = sb + int32_t(threadIdx.x); s < se; s += kBrickThreads) {
+    int ci, cj, ck;
+    cell_ijk(v.g, a.slot_cell[s] - v.coff, ci, cj, ck);
+    uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
+    int64_t k = 0;
+    auto emit = [&](int32_t p, int32_t q, uint32_t code) {
+      const uint32_t hi = code << kSlotBits;
+      for (int32_t t = p; t < q; ++t) {
+        if (t == s) continue;
+        out[k * kSlice] = uint32_t(t) | hi;
+        ++k;
+      }
+    };
+    if (recs) {
+      box_walk(B, v.rows, v.nrows, ci, cj, ck, [&](int q0, int q1, uint32_t code) {
+        const uint32_t hi = code << kSlotBits;
+        for (int r = cst[q0], e = cst[q1 + 1]; r < e; ++r) {
+          const int32_t t = rslot[r];
+          if (t == s) continue;
+          out[k * kSlice] = uint32_t(t) | hi;
+          ++k;
+        }
+      });
+    } else if (staged) {
+      box_walk(B, v.rows, v.nrows, ci, cj, ck, [&](int q0, int q1, uint32_t code) {
+        for (int q = q0; q <= q1; ++q) emit(first[q], first[q] + (cst[q + 1] - cst[q]), code);
+      });
+    } else {
+      for_each_segment(v.g, v.rows, v.nrows, v.cs, ci, cj, ck, emit);
     }
-  });
+  }
 }
 
-// Element particle lists for the spreading: per element, the slots of its
-// stencil cells (own element included) with their periodic shift codes.
-__global__ void k_elist_count(Grid g, const StencilRow* __restrict__ rows, int nrows,
-                              const int32_t* __restrict__ cell_start, int64_t* __restrict__ cnt) {
-  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= g.n_elem) return;
-  int ci, cj, ck;
-  elem_ijk(g, e, ci, cj, ck);
-  int64_t c = 0;
-  for_each_segment(g, rows, nrows, cell_start, ci, cj, ck,
-                   [&](int32_t a, int32_t b, uint32_t) { c += b - a; });
-  cnt[e] = c;
+// Element lists of the spreading by brick (one system): element = cell,
+// the own element is part of its stencil and nothing is excluded.  Every
+// brick has elements, particles or not.
+__device__ __forceinline__ bool brick_elem(const BoxGeom& B, int l, int& ci, int& cj, int& ck, int64_t& e) {
+  constexpr int M = (1 << kBrickBits) - 1;
+  ci = B.bx0 + (l & M);
+  cj = B.by0 + ((l >> kBrickBits) & M);
+  ck = B.bz0 + (l >> (2 * kBrickBits));
+  e = ci + int64_t(B.n[0]) * (cj + int64_t(B.n[1]) * ck);
+  return ci < B.n[0] && cj < B.n[1] && ck < B.n[2];  // padding cells of the last bricks are no elements
+}
+
+__global__ void __launch_bounds__(kBrickThreads) k_brick_elist_count(BrickArgs a, int64_t* __restrict__ cnt) {
+  extern __shared__ int32_t cst[];  // [cells_cap + 1]
+  __shared__ int32_t s_w[kBrickThreads / 64];
+  BoxGeom B;
+  B.init(a.g, 0, a.reach, blockIdx.x);
+  const bool staged = brick_staged(a, B);
+  if (staged) stage_box_cells<kBrickThreads>(B, a.cs, nullptr, cst, s_w);
+  for (int l = int(threadIdx.x); l < kBrickCells; l += kBrickThreads) {
+    int ci, cj, ck;
+    int64_t e;
+    if (!brick_elem(B, l, ci, cj, ck, e)) continue;
+    int64_t c = 0;
+    if (staged)
+      box_walk(B, a.rows, a.nrows, ci, cj, ck, [&](int q0, int q1, uint32_t) { c += cst[q1 + 1] - cst[q0]; });
+    else
+      for_each_segment(a.g, a.rows, a.nrows, a.cs, ci, cj, ck, [&](int32_t p, int32_t q, uint32_t) { c += q - p; });
+    cnt[e] = c;
+  }
 }
 
 // The lists of a block's consecutive elements form one contiguous output
@@ -600,19 +731,43 @@ hipError_t launch_gather_forces(ggem_ctx* ctx) {
   return hipGetLastError();
 }
 
+BrickArgs brick_args(const ggem_ctx* ctx) {
+  BrickArgs a;
+  a.g = ctx->grid;
+  a.rows = ctx->rows_d;
+  a.nrows = ctx->n_rows;
+  for (int d = 0; d < 3; ++d) a.reach[d] = ctx->reach[d];
+  a.cs = ctx->cell_start;
+  a.slot_cell = ctx->slot_cell;
+  a.sys = ctx->nsys > 1 ? ctx->sys_d : nullptr;
+  a.sys_slot = ctx->nsys > 1 ? ctx->sys_slot : nullptr;
+  // LDS tables sized for the largest box of any system
+  auto cells = [](const int32_t* r) {
+    return ((1 << kBrickBits) + 2 * r[0]) * ((1 << kBrickBits) + 2 * r[1]) * ((1 << kBrickBits) + 2 * r[2]);
+  };
+  int cap = cells(ctx->reach);
+  for (int k = 0; k < ctx->nsys && ctx->nsys > 1; ++k) cap = std::max(cap, cells(ctx->sys_h[k].reach));
+  a.cells_cap = std::min(cap, kBoxMaxCells);
+  return a;
+}
+
+unsigned brick_count(const ggem_ctx* ctx) { return unsigned(ctx->grid.n_cells >> (3 * kBrickBits)); }
+
 hipError_t launch_count_lists(ggem_ctx* ctx) {
   hipError_t e = hipMemsetAsync(ctx->pairs_d, 0, sizeof(int64_t), ctx->stream);
   if (e != hipSuccess || ctx->n == 0) return e;
   ProfScope ps(ctx, "count_lists");
-  auto* tot = reinterpret_cast<unsigned long long*>(ctx->pairs_d);
-  if (ctx->nsys > 1)
-    hipLaunchKernelGGL(k_count_lists<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->sys_d, ctx->sys_slot,
-                       ctx->n, ctx->nbr_cnt, ctx->slice_len, tot);
+  const BrickArgs a = brick_args(ctx);
+  const size_t lds = sizeof(int32_t) * (size_t(a.cells_cap) + 1);
+  if (a.sys)
+    hipLaunchKernelGGL(k_brick_count<true>, dim3(brick_count(ctx)), dim3(kBrickThreads), lds, ctx->stream, a,
+                       ctx->nbr_cnt);
   else
-    hipLaunchKernelGGL(k_count_lists<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream,
-                       ctx->grid, ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, nullptr, nullptr,
-                       ctx->n, ctx->nbr_cnt, ctx->slice_len, tot);
+    hipLaunchKernelGGL(k_brick_count<false>, dim3(brick_count(ctx)), dim3(kBrickThreads), lds, ctx->stream, a,
+                       ctx->nbr_cnt);
+  hipLaunchKernelGGL(k_slice_len, dim3(blocks_for(ctx->n, kSliceBlock)), dim3(kSliceBlock), 0, ctx->stream,
+                     ctx->nbr_cnt, ctx->n,
+                     ctx->slice_len, reinterpret_cast<unsigned long long*>(ctx->pairs_d));
   return hipGetLastError();
 }
 
@@ -633,14 +788,14 @@ hipError_t launch_scan64(ggem_ctx* ctx, const int64_t* in, int64_t len, int64_t*
 hipError_t launch_fill_ell(ggem_ctx* ctx) {
   if (ctx->n == 0) return hipSuccess;
   ProfScope ps(ctx, "fill_ell");
-  if (ctx->nsys > 1)
-    hipLaunchKernelGGL(k_fill_ell<true>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->sys_d, ctx->sys_slot,
-                       ctx->n, ctx->slice_off, ctx->ell);
-  else
-    hipLaunchKernelGGL(k_fill_ell<false>, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid,
-                       ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->slot_cell, nullptr, nullptr, ctx->n,
+  const BrickArgs a = brick_args(ctx);
+  const size_t lds = sizeof(int32_t) * (2 * size_t(a.cells_cap) + 1 + kBrickRecCap);
+  if (a.sys)
+    hipLaunchKernelGGL(k_brick_fill_ell<true>, dim3(brick_count(ctx)), dim3(kBrickThreads), lds, ctx->stream, a,
                        ctx->slice_off, ctx->ell);
+  else
+    hipLaunchKernelGGL(k_brick_fill_ell<false>, dim3(brick_count(ctx)), dim3(kBrickThreads), lds, ctx->stream,
+                       a, ctx->slice_off, ctx->ell);
   return hipGetLastError();
 }
 
@@ -653,8 +808,10 @@ hipError_t launch_part_sys(ggem_ctx* ctx) {
 
 hipError_t launch_elist_count(ggem_ctx* ctx) {
   ProfScope ps(ctx, "elist_count");
-  hipLaunchKernelGGL(k_elist_count, dim3(blocks_for(ctx->grid.n_elem)), dim3(kBlock), 0, ctx->stream,
-                     ctx->grid, ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->elist_off);
+  const BrickArgs a = brick_args(ctx);
+  const size_t lds = sizeof(int32_t) * (size_t(a.cells_cap) + 1);
+  hipLaunchKernelGGL(k_brick_elist_count, dim3(brick_count(ctx)), dim3(kBrickThreads), lds, ctx->stream, a,
+                     ctx->elist_off);
   return hipGetLastError();
 }
 

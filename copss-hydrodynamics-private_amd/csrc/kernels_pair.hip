@@ -452,10 +452,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // whose box holds more than kBoxMaxRecs records gathers from global memory
 // (the list entries and the neighbours' records).
 constexpr int kBoxThreads = 256;
-constexpr int kBoxMaxCells = 8192;  // box cells of a staged brick (16-bit starts in LDS)
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
-constexpr int kBoxBits = 6;         // box coordinates packed 6 bits per axis
-constexpr int kBrickCells = 1 << (3 * kBrickBits);
 
 struct BoxArgs {
   const int32_t* cs;     // cell starts, brick order
@@ -471,8 +468,11 @@ struct BoxArgs {
   float4* pos;           // [records] offset xyz, packed box cell
   uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
   float2* ab;            // [entries] pair tensor (A, Bx) from the coefficient table, brick layout
-  int32_t cells_cap;     // k_box_fill: box cells and waves of a brick its LDS tables hold
+  int32_t cells_cap;     // k_box_fill: box cells, waves and records of a brick its LDS tables hold
   int32_t waves_cap;
+  int32_t recs_cap;
+  const StencilRow* rows;  // stencil rows (a batched context: all systems', SysGrid::row_off)
+  int32_t nrows;
 };
 
 // Entry layout of a brick: its targets in the order tgt, in waves of 64;
@@ -483,105 +483,16 @@ struct BoxArgs {
 constexpr int kBoxMaxWaves = kBoxMaxTargets / 64;  // waves of one brick
 constexpr int kBoxSortMax = 2048;   // targets of a brick ordered in LDS
 
-// Box of one brick: origin (ox, oy, oz) and extent (ex, ey, ez) in cells of
-// the brick's system grid.  The grid fields are copied (no pointer into the
-// kernel arguments, which would force a private copy of them).
-struct BoxGeom {
-  int64_t coff;
-  int n[3], per[3], nb0, nb1;
-  float h[3];
-  int ox, oy, oz, ex, ey, ez, nbc;
-  float inv_ex, inv_ey;
-
-  __device__ __forceinline__ void grid(const Grid& g) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      n[d] = g.n[d];
-      per[d] = g.periodic[d];
-      h[d] = float(g.h[d]);
-    }
-    nb0 = g.nb[0];
-    nb1 = g.nb[1];
+// Box of the brick holding first_slot (the slot picks the system of a
+// batched context).
+__device__ __forceinline__ void box_init(BoxGeom& B, const PairArgs& a, const int32_t* reach, int64_t brick,
+                                         int64_t first_slot) {
+  if (a.sys) {
+    const SysGrid* S = a.sys + a.sys_slot[first_slot];
+    B.init(S->g, S->cell_off, S->reach, brick);
+  } else {
+    B.init(a.g, 0, reach, brick);
   }
-  // first_slot: any slot of the brick (picks the system of a batch)
-  __device__ __forceinline__ void init(const PairArgs& a, const int32_t* reach, int64_t brick,
-                                       int64_t first_slot) {
-    int rx, ry, rz;
-    if (a.sys) {
-      const SysGrid* S = a.sys + a.sys_slot[first_slot];
-      grid(S->g);
-      coff = S->cell_off;
-      rx = S->reach[0];
-      ry = S->reach[1];
-      rz = S->reach[2];
-    } else {
-      grid(a.g);
-      coff = 0;
-      rx = reach[0];
-      ry = reach[1];
-      rz = reach[2];
-    }
-    const int64_t lb = brick - (coff >> (3 * kBrickBits));
-    const int64_t bxy = int64_t(nb0) * nb1;
-    const int bk = int(lb / bxy);
-    const int64_t rem = lb - int64_t(bk) * bxy;
-    const int bj = int(rem / nb0);
-    const int bi = int(rem - int64_t(bj) * nb0);
-    ox = (bi << kBrickBits) - rx;
-    oy = (bj << kBrickBits) - ry;
-    oz = (bk << kBrickBits) - rz;
-    ex = (1 << kBrickBits) + 2 * rx;
-    ey = (1 << kBrickBits) + 2 * ry;
-    ez = (1 << kBrickBits) + 2 * rz;
-    nbc = ex * ey * ez;
-    inv_ex = 1.0f / float(ex);
-    inv_ey = 1.0f / float(ey);
-  }
-  __device__ __forceinline__ bool fits() const {
-    return nbc <= kBoxMaxCells && ex <= (1 << kBoxBits) && ey <= (1 << kBoxBits) && ez <= (1 << kBoxBits);
-  }
-  __device__ __forceinline__ void xyz(int q, int& x, int& y, int& z) const {
-    const int yz = div_small(q, inv_ex);
-    x = q - yz * ex;
-    z = div_small(yz, inv_ey);
-    y = yz - z * ey;
-  }
-  // brick-order index of grid cell (i, j, k) of the system
-  __device__ __forceinline__ int64_t index(int i, int j, int k) const {
-    return coff + cell_index_nb(nb0, nb1, i, j, k);
-  }
-  // global cell of box cell q; false outside a non-periodic domain
-  __device__ __forceinline__ bool cell(int q, int64_t& c) const {
-    int x, y, z;
-    xyz(q, x, y, z);
-    int X = ox + x, Y = oy + y, Z = oz + z;
-    if (X < 0 || X >= n[0]) {
-      if (!per[0]) return false;
-      X = wrap_axis(X, n[0]);
-    }
-    if (Y < 0 || Y >= n[1]) {
-      if (!per[1]) return false;
-      Y = wrap_axis(Y, n[1]);
-    }
-    if (Z < 0 || Z >= n[2]) {
-      if (!per[2]) return false;
-      Z = wrap_axis(Z, n[2]);
-    }
-    c = index(X, Y, Z);
-    return true;
-  }
-};
-
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
 }
 
 // Records of every brick's box (a brick is staged when it has targets, its
@@ -600,7 +511,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
   int total = 0;
   if (nt > 0) {  // uniform over the block
     BoxGeom B;
-    B.init(a, b.reach, brick, sb);
+    box_init(B, a, b.reach, brick, sb);
     if (B.fits()) {
       int mine = 0;
       for (int q = int(threadIdx.x); q < B.nbc; q += kBoxThreads) {
@@ -649,8 +560,33 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
     for (int w = 0; w < kBoxThreads / 64; ++w) e += s_e[w];
     b.ecnt[brick] = e;
     b.cnt[brick] = staged ? total : 0;
-    if (staged) atomicMax(b.maxrec, total);
-    atomicMax(b.maxrec + 1, nt);
+  }
+}
+
+// Largest staged record count and most targets of any brick (one block; a
+// reduction instead of an atomic per brick on the same two words).
+constexpr int kMaxThreads = 1024;
+__global__ void __launch_bounds__(kMaxThreads) k_box_max(BoxArgs b, int64_t nbricks) {
+  __shared__ int32_t s_m[2][kMaxThreads / 64];
+  int mr = 0, mt = 0;
+  for (int64_t k = threadIdx.x; k < nbricks; k += kMaxThreads) {
+    mr = max(mr, int(b.cnt[k]));
+    mt = max(mt, b.cs[(k + 1) * kBrickCells] - b.cs[k * kBrickCells]);
+  }
+  mr = wave_max(mr);
+  mt = wave_max(mt);
+  if ((threadIdx.x & 63) == 0) {
+    s_m[0][threadIdx.x >> 6] = mr;
+    s_m[1][threadIdx.x >> 6] = mt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kMaxThreads / 64; ++w) {
+      mr = max(mr, s_m[0][w]);
+      mt = max(mt, s_m[1][w]);
+    }
+    b.maxrec[0] = mr;
+    b.maxrec[1] = mt;
   }
 }
 
@@ -659,10 +595,11 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
 template <bool BATCH>
 __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b) {
   extern __shared__ float4 dyn[];
-  float4* s_tab = dyn;                                                   // coefficient table
-  int32_t* cfirst = reinterpret_cast<int32_t*>(dyn + 4 * a.ct.n);        // first slot of each box cell
-  int32_t* wpre = cfirst + b.cells_cap;                                  // entry rows before each wave
-  uint16_t* cst = reinterpret_cast<uint16_t*>(wpre + b.waves_cap + 1);   // record start of each box cell
+  float4* s_tab = dyn;                                                 // coefficient table
+  float4* rpos = dyn + 4 * a.ct.n;                                     // [recs_cap] records of a staged brick
+  int32_t* cfirst = reinterpret_cast<int32_t*>(rpos + b.recs_cap);     // [cells_cap] first slot of each box cell
+  int32_t* cst = cfirst + b.cells_cap;                                 // [cells_cap + 1] record start of each box cell
+  int32_t* wpre = cst + b.cells_cap + 1;                               // [waves_cap + 1] entry rows before each wave
   __shared__ int32_t s_w[kBoxThreads / 64];
   const int64_t brick = blockIdx.x;
   const int64_t c0 = brick * kBrickCells;
@@ -674,59 +611,30 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   BoxGeom B;
-  B.init(a, b.reach, brick, sb);
+  box_init(B, a, b.reach, brick, sb);
   if (nrec > 0) {
-    // (1) counts and first slots of the box cells, (2) exclusive scan of the
-    // counts in place: contiguous chunks per thread, then across the waves
-    // (the total is nrec)
-    const int per = (B.nbc + kBoxThreads - 1) / kBoxThreads;
-    const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
-    int mine = 0;
-    for (int q = q0; q < q1; ++q) {
-      int64_t c;
-      int k = 0, f = 0;
-      if (B.cell(q, c)) {
-        f = b.cs[c];
-        k = b.cs[c + 1] - f;
-      }
-      cst[q] = uint16_t(k);
-      cfirst[q] = f;
-      mine += k;
-    }
-    int incl = mine;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if ((threadIdx.x & 63) >= o) incl += y;
-    }
-    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
-    __syncthreads();
-    int run = incl - mine;
-    for (int w = 0; w < int(threadIdx.x >> 6); ++w) run += s_w[w];
-    for (int q = q0; q < q1; ++q) {
-      const int k = cst[q];
-      cst[q] = uint16_t(run);
-      run += k;
-    }
-    if (threadIdx.x == 0) cst[B.nbc] = uint16_t(nrec);
-    __syncthreads();
-    // (3) records: record i lies in the last box cell whose start is <= i
+    // (1) first slots and record starts of the box cells (the total is nrec),
+    // (2) the records: record i lies in the last box cell whose start is <= i
+    stage_box_cells<kBoxThreads>(B, b.cs, cfirst, cst, s_w);
     for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
       int lo = 0, hi = B.nbc;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (int(cst[mid]) <= i) lo = mid;
+        if (cst[mid] <= i) lo = mid;
         else hi = mid;
       }
-      const int64_t t = int64_t(cfirst[lo]) + (i - int(cst[lo]));
+      const int64_t t = int64_t(cfirst[lo]) + (i - cst[lo]);
       int x, y, z;
       B.xyz(lo, x, y, z);
       const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
       const float4 p = a.pc32[t];
+      const float4 rec = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
       b.slot[r0 + i] = uint32_t(t);
-      b.pos[r0 + i] = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
+      b.pos[r0 + i] = rec;
+      rpos[i] = rec;
     }
   }
-  // (4) the entry layout: the longest list of every wave of the order,
+  // (3) the entry layout: the longest list of every wave of the order,
   // exclusive prefix (the host keeps nt within kBoxMaxWaves waves)
   const int nw = (nt + 63) / 64;
   for (int w = wv; w < nw; w += kBoxThreads / 64) {
@@ -744,25 +652,59 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     }
   }
   __syncthreads();
-  // (5) the entries of the brick's targets: the pair tensor (A, Bx) from
+  // (4) the entries of the brick's targets: the pair tensor (A, Bx) from
   // the coefficient table at the FP32C displacement, exactly as the
   // list-order kernel forms both, and in staged bricks the neighbour's
-  // record number; the next entry's position record is loaded while the
-  // current one is evaluated
+  // record number.  Staged bricks walk the stencil through the box records
+  // in LDS (list order, box_walk); the others read their lists.
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
+  const StencilRow* rows = b.rows;
+  int nrows = b.nrows;
+  if constexpr (BATCH) {
+    const SysGrid& S = a.sys[a.sys_slot[sb]];
+    rows += S.row_off;
+    nrows = S.nrows;
+  }
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
     const int64_t bb = b.eoff[brick] + 64 * int64_t(wpre[i >> 6]) + (i & 63);
     b.base[sb + i] = bb;
     const int cnt = a.cnt[s];
     if (cnt == 0) continue;  // an empty list has no storage
-    const int64_t base = a.slice_off[s >> 6] + (s & 63);
-    const int last = cnt - 1;
     const float4 po = a.pc32[s];
     const uint32_t co = __float_as_uint(po.w);
     const int io = int(co & ((1u << kCellBitsX) - 1));
     const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
     const int ko = int(co >> (kCellBitsX + kCellBitsY));
+    if (nrec > 0) {
+      const int bx = io - B.ox, by = jo - B.oy, bz = ko - B.oz;
+      const int qs = (bz * B.ey + by) * B.ex + bx;
+      const int self = cst[qs] + int(s - cfirst[qs]);
+      int64_t o = bb;
+      box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
+        for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
+          if (r == self) continue;
+          const float4 pt = rpos[r];
+          const uint32_t bt = __float_as_uint(pt.w);
+          const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
+          const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
+          const int dk = int(bt >> (2 * kBoxBits)) - bz;
+          const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+          const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+          const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+          float A, Bx;
+          fp32c_coef(s_tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
+          b.ab[o] = make_float2(A, Bx);
+          b.idx[o] = uint16_t(r);
+          o += kSlice;
+        }
+      });
+      continue;
+    }
+    // unstaged brick: the list in list order, the neighbour's position
+    // record loaded one entry ahead
+    const int64_t base = a.slice_off[s >> 6] + (s & 63);
+    const int last = cnt - 1;
     uint32_t e1 = a.ell[base];
     float4 p1 = a.pc32[e1 & kSlotMask];
     for (int k = 0; k < cnt; ++k) {
@@ -770,7 +712,6 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       const float4 pt = p1;
       e1 = a.ell[base + int64_t(min(k + 1, last)) * kSlice];
       p1 = a.pc32[e1 & kSlotMask];
-      const uint32_t t = e & kSlotMask;
       const uint32_t code = e >> kSlotBits;
       // the neighbour's unwrapped cell: its cell plus the entry's periodic shift
       const uint32_t ct = __float_as_uint(pt.w);
@@ -786,15 +727,6 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       float A, Bx;
       fp32c_coef(s_tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
       b.ab[bb + int64_t(k) * kSlice] = make_float2(A, Bx);
-      if (nrec > 0) {  // its box cell, and its rank in the cell, give the record
-        const int x = ui - B.ox, y = uj - B.oy, z = uk - B.oz;
-        int r = 0;
-        if (x >= 0 && x < B.ex && y >= 0 && y < B.ey && z >= 0 && z < B.ez) {
-          const int q = (z * B.ey + y) * B.ex + x;
-          r = int(cst[q]) + int(int64_t(t) - cfirst[q]);
-        }
-        b.idx[bb + int64_t(k) * kSlice] = uint16_t(min(max(r, 0), nrec - 1));  // LDS reads stay in the records
-      }
     }
   }
 }
@@ -895,7 +827,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   }
   __syncthreads();
   BoxGeom B;
-  B.init(a, b.reach, brick, sb);
+  box_init(B, a, b.reach, brick, sb);
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
   constexpr int U = 8;  // entries per group, the next group loaded one ahead
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
@@ -1080,6 +1012,9 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   BoxArgs b;
   b.cs = ctx->cell_start;
   for (int d = 0; d < 3; ++d) b.reach[d] = ctx->reach[d];
+  b.rows = ctx->rows_d;
+  b.nrows = ctx->n_rows;
+  b.cells_cap = b.waves_cap = b.recs_cap = 0;
   b.cnt = ctx->box_cnt;
   b.ecnt = ctx->box_ecnt;
   b.maxrec = ctx->box_max_d;
@@ -1190,15 +1125,14 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
 bool box_supported(const ggem_ctx* ctx) { return ctx->n > 0 && !ctx->exact_minimg; }
 
 hipError_t launch_box_count(ggem_ctx* ctx) {
-  hipError_t e = hipMemsetAsync(ctx->box_max_d, 0, 2 * sizeof(int32_t), ctx->stream);
-  if (e != hipSuccess) return e;
   ProfScope ps(ctx, "box_count");
-  hipLaunchKernelGGL(k_box_count, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx),
-                     box_args(ctx));
+  const BoxArgs b = box_args(ctx);
+  hipLaunchKernelGGL(k_box_count, dim3(n_bricks(ctx)), dim3(kBoxThreads), 0, ctx->stream, make_args(ctx), b);
+  hipLaunchKernelGGL(k_box_max, dim3(1), dim3(kMaxThreads), 0, ctx->stream, b, int64_t(n_bricks(ctx)));
   return hipGetLastError();
 }
 
-hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_targets) {
+hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_records, int32_t max_targets) {
   ProfScope ps(ctx, "box_fill");
   BoxArgs b = box_args(ctx);
   // LDS tables sized for the largest box and the most crowded brick
@@ -1209,9 +1143,10 @@ hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_targets) {
   for (int k = 0; k < ctx->nsys && ctx->nsys > 1; ++k) cap = std::max(cap, cells(ctx->sys_h[k].reach));
   b.cells_cap = std::min(cap, kBoxMaxCells);
   b.waves_cap = (max_targets + 63) / 64;
-  if (b.waves_cap > kBoxMaxWaves) return hipErrorInvalidValue;
-  const size_t lds = sizeof(float4) * 4 * size_t(ctx->tab.n) + sizeof(int32_t) * (b.cells_cap + b.waves_cap + 1) +
-                     sizeof(uint16_t) * (b.cells_cap + 1);
+  b.recs_cap = max_records;
+  if (b.waves_cap > kBoxMaxWaves || max_records > kBoxMaxRecs) return hipErrorInvalidValue;
+  const size_t lds = sizeof(float4) * (4 * size_t(ctx->tab.n) + size_t(b.recs_cap)) +
+                     sizeof(int32_t) * (2 * size_t(b.cells_cap) + b.waves_cap + 2);
   if (ctx->nsys > 1)
     hipLaunchKernelGGL(k_box_fill<true>, dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, make_args(ctx),
                        b);

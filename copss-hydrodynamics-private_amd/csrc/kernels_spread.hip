@@ -20,7 +20,7 @@
 // NQ1^2 x 3 accumulators.  The NQ1 threads of an element read the same
 // list entries (broadcast loads).  rho goes to LDS for the IF contraction
 // (FP64 arithmetic in both precisions).  FP32C: factors and kernel values
-// in FP32 after the FP64 displacement, partial sums merged every 16
+// in FP32 after the FP64 displacement, partial sums merged every 32
 // particles into Kahan accumulators, rho staged in LDS as FP32.
 #include "ggem_device.h"
 
@@ -28,8 +28,7 @@ namespace ggem {
 
 namespace {
 
-constexpr int kMaxNu = 27;
-constexpr int kMergeEvery = 16;
+constexpr int kMergeEvery = 32;
 // elements per block, candidates of the autotuner (index 0 is the default);
 // the per-element arithmetic does not depend on the choice
 constexpr int kElemChoices[] = {64, 32, 16};
@@ -143,11 +142,21 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
         ey[g] = ex * exp_neg(sy[g]);
         ez[g] = exp_neg(sz[g]);
       }
+      // g = E_y E_z (T0 - S_y - S_z) = E_z P_b - E_y Q_c with
+      // T0 = 2.5 - S_x, P_b = E_y (T0 - S_y), Q_c = E_z S_z: two
+      // operations per Gauss point
+      const T t0 = T(2.5) - sx;
+      T pb[NQ1], qc[NQ1];
+#pragma unroll
+      for (int g = 0; g < NQ1; ++g) {
+        pb[g] = ey[g] * (t0 - sy[g]);
+        qc[g] = ez[g] * sz[g];
+      }
 #pragma unroll
       for (int c = 0; c < NQ1; ++c)
 #pragma unroll
         for (int b = 0; b < NQ1; ++b) {
-          const T gv = (ey[b] * ez[c]) * ((T(2.5) - sx) - (sy[b] + sz[c]));
+          const T gv = fma(ez[c], pb[b], -(ey[b] * qc[c]));
           p[c * NQ1 + b][0] += gv * T(f.x);
           p[c * NQ1 + b][1] += gv * T(f.y);
           p[c * NQ1 + b][2] += gv * T(f.z);
@@ -200,7 +209,7 @@ __global__ void __launch_bounds__(kElems * NQ1) k_spread(SpreadArgs<T> a, const 
   }
   __syncthreads();
   double* stage = reinterpret_cast<double*>(s_rho);
-  constexpr int kStage = int(sizeof(s_rho) / sizeof(double));
+  constexpr int kStage = int(sizeof(s_rho) / (sizeof(double)));  // doubles the buffer holds
   const int step = kStage / per;  // elements per pass (n_u <= 27 keeps it >= 1)
   const int64_t e_first = blk * kElems;
   const int ne = int(min<int64_t>(kElems, a.g.n_elem - e_first));
