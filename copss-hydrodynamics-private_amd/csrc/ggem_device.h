@@ -185,16 +185,31 @@ __device__ __forceinline__ void stage_box_cells(const BoxGeom& B, const int32_t*
   const int per = (B.nbc + NT - 1) / NT;
   const int q0 = min(B.nbc, int(threadIdx.x) * per), q1 = min(B.nbc, q0 + per);
   int mine = 0;
-  for (int q = q0; q < q1; ++q) {
-    int64_t c;
-    int k = 0, f = 0;
-    if (B.cell(q, c)) {
-      f = cs[c];
-      k = cs[c + 1] - f;
+  // eight cells at a time, their cell-start loads all in flight (cells
+  // outside the domain or past the chunk read cell 0 and count nothing)
+  constexpr int G = 8;
+  for (int q = q0; q < q1; q += G) {
+    int64_t c[G];
+    bool in[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      in[u] = q + u < q1 && B.cell(q + u, c[u]);
+      if (!in[u]) c[u] = 0;
     }
-    cst[q] = k;
-    if (first) first[q] = f;
-    mine += k;
+    int f[G], e[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      f[u] = cs[c[u]];
+      e[u] = cs[c[u] + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (q + u >= q1) break;
+      const int k = in[u] ? e[u] - f[u] : 0;
+      cst[q + u] = k;
+      if (first) first[q + u] = in[u] ? f[u] : 0;
+      mine += k;
+    }
   }
   int incl = mine;
   for (int o = 1; o < 64; o <<= 1) {

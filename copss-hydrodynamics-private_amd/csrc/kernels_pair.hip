@@ -514,9 +514,24 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
     box_init(B, a, b.reach, brick, sb);
     if (B.fits()) {
       int mine = 0;
-      for (int q = int(threadIdx.x); q < B.nbc; q += kBoxThreads) {
-        int64_t c;
-        if (B.cell(q, c)) mine += b.cs[c + 1] - b.cs[c];
+      constexpr int G = 8;  // cells per thread and step, their loads in flight together
+      for (int q0 = int(threadIdx.x); q0 < B.nbc; q0 += G * kBoxThreads) {
+        int64_t c[G];
+        bool in[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int q = q0 + u * kBoxThreads;
+          in[u] = q < B.nbc && B.cell(q, c[u]);
+          if (!in[u]) c[u] = 0;
+        }
+        int f[G], e[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          f[u] = b.cs[c[u]];
+          e[u] = b.cs[c[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) mine += in[u] ? e[u] - f[u] : 0;
       }
       mine = wave_sum(mine);
       if (lane == 0) s_w[wv] = mine;
@@ -616,22 +631,41 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     // (1) first slots and record starts of the box cells (the total is nrec),
     // (2) the records: record i lies in the last box cell whose start is <= i
     stage_box_cells<kBoxThreads>(B, b.cs, cfirst, cst, s_w);
-    for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
-      int lo = 0, hi = B.nbc;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (cst[mid] <= i) lo = mid;
-        else hi = mid;
+    int top = 1;  // largest power of two below nbc
+    while (2 * top < B.nbc) top *= 2;
+    constexpr int G = 4;  // records per thread and step, their searches and loads interleaved
+    for (int i0 = int(threadIdx.x); i0 < nrec; i0 += G * kBoxThreads) {
+      int lo[G], ii[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        ii[u] = min(i0 + u * kBoxThreads, nrec - 1);
+        lo[u] = 0;
       }
-      const int64_t t = int64_t(cfirst[lo]) + (i - cst[lo]);
-      int x, y, z;
-      B.xyz(lo, x, y, z);
-      const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
-      const float4 p = a.pc32[t];
-      const float4 rec = make_float4(p.x, p.y, p.z, __uint_as_float(packed));
-      b.slot[r0 + i] = uint32_t(t);
-      b.pos[r0 + i] = rec;
-      rpos[i] = rec;
+      for (int step = top; step > 0; step >>= 1)
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int m = lo[u] + step;
+          if (m < B.nbc && cst[m] <= ii[u]) lo[u] = m;
+        }
+      int64_t t[G];
+      float4 p[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        t[u] = int64_t(cfirst[lo[u]]) + (ii[u] - cst[lo[u]]);
+        p[u] = a.pc32[t[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int i = i0 + u * kBoxThreads;
+        if (i >= nrec) break;
+        int x, y, z;
+        B.xyz(lo[u], x, y, z);
+        const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
+        const float4 rec = make_float4(p[u].x, p[u].y, p[u].z, __uint_as_float(packed));
+        b.slot[r0 + i] = uint32_t(t[u]);
+        b.pos[r0 + i] = rec;
+        rpos[i] = rec;
+      }
     }
   }
   // (3) the entry layout: the longest list of every wave of the order,
