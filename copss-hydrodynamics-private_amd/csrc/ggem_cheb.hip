@@ -102,7 +102,11 @@ struct Applier {
       } else if (e == hipSuccess && epi.kind == 2) {
         e = launch_axpby(ctx, len, 2. * epi.db, xr.soa, 2. * epi.da, mx);  // 2db x + 2da M x
         if (e == hipSuccess) e = launch_axpby(ctx, len, -1.0, epi.xm1.soa + r * len, 1.0, mx);
-        if (e == hipSuccess) e = launch_axpby(ctx, len, epi.c, mx, 1.0, epi.y + r * len);
+        if (e == hipSuccess && (epi.ymask & 1))
+          e = launch_axpby(ctx, len, epi.cm1, epi.xm1.soa + r * len, 1.0, epi.y + r * len);
+        if (e == hipSuccess && (epi.ymask & 2))
+          e = launch_axpby(ctx, len, epi.c0, xr.soa, 1.0, epi.y + r * len);
+        if (e == hipSuccess && (epi.ymask & 4)) e = launch_axpby(ctx, len, epi.c, mx, 1.0, epi.y + r * len);
       }
       if (e == hipSuccess)
         e = hipMemcpyAsync(vr.soa, mx, sizeof(double) * len, hipMemcpyDeviceToDevice, ctx->stream);
@@ -145,13 +149,23 @@ int enqueue_fixed(ggem_ctx* ctx, Applier& apply, const FixedStep& s) {
                          s.sys);
   if (e != hipSuccess) return hip_fail(ctx, e, "init y");
   SlotVec a = s.x0, b = s.x1;
+  int added = 1;  // y holds c_0 x_0 + c_1 x_1
   for (int i = 1; i < s.N; ++i) {
-    // x_{k+1} into x_{k-1}'s storage
+    // x_{k+1} into x_{k-1}'s storage; y gains the terms up to x_{k+1} every
+    // third step and at the last one (x_{k-1}, x_k and x_{k+1} are all at
+    // hand then), so two steps in three neither read nor write y
     ChebEpi st{2, s.par[0], s.par[1], s.par[2 + i + 1], a, s.yv.soa};
     st.dpar = s.dpar;
     st.pstride = s.N + 3;
     st.term = i + 1;
     st.sys = s.sys;
+    st.ymask = 0;
+    if (i % 3 == 0 || i == s.N - 1) {
+      for (int j = added + 1; j <= i + 1; ++j) st.ymask |= 1 << (j - (i - 1));
+      st.cm1 = s.par[2 + i - 1];
+      st.c0 = s.par[2 + i];
+      added = i + 1;
+    }
     if ((rc = apply(s.nrhs, b, a, st))) return rc;
     std::swap(a, b);
   }

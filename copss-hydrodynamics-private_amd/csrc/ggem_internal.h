@@ -313,16 +313,22 @@ enum Epi : int {
 struct ChebEpi {
   int kind;            // 0: v = M x; 1: v = db x + da M x; 2: v = 2db x + 2da M x - x_{k-1}, y += c v
   double da, db;
-  double c;            // coefficient of kind 2
+  double c;            // coefficient of v (kind 2)
   SlotVec xm1;         // x_{k-1} (kind 2); may alias v
   double* y;           // [nrhs][3][n] (kind 2)
   // Device-side parameters (used when dpar != nullptr, e.g. in a captured
-  // graph): per system [da, db, c_0 .. c_N] with stride pstride; term picks
-  // c; sys gives the system of each slot (nullptr: system 0).
+  // graph): per system [da, db, c_0 .. c_N] with stride pstride; term is
+  // the index of v's coefficient; sys gives the system of each slot
+  // (nullptr: system 0).
   const double* dpar = nullptr;
   int pstride = 0;
   int term = 0;
   const int32_t* sys = nullptr;
+  // kind 2: terms added to y, bit 0 x_{k-1}, bit 1 x_k, bit 2 v = x_{k+1}
+  // (0: y untouched), with coefficients cm1, c0 and c (dpar: the terms
+  // before term)
+  int ymask = 4;
+  double cm1 = 0.0, c0 = 0.0;
 };
 // pair tensors of every list entry for the given precision
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);

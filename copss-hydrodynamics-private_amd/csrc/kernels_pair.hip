@@ -205,12 +205,14 @@ __device__ __forceinline__ void apply_epilogue(const PairArgs& a, int64_t s, con
 #pragma unroll
     for (int q = 0; q < 3; ++q) out[3 * p + q] = u[q];
   } else {
-    double da = epi.da, db = epi.db, ck = epi.c;
+    double da = epi.da, db = epi.db, ck = epi.c, cm1 = epi.cm1, c0 = epi.c0;
     if (epi.dpar) {
       const double* pp = epi.dpar + int64_t(epi.sys ? epi.sys[s] : 0) * epi.pstride;
       da = pp[0];
       db = pp[1];
       ck = pp[2 + epi.term];
+      if (epi.ymask & 1) cm1 = pp[epi.term];
+      if (epi.ymask & 2) c0 = pp[1 + epi.term];
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -228,10 +230,19 @@ __device__ __forceinline__ void apply_epilogue(const PairArgs& a, int64_t s, con
         } else if (epi.kind == 1) {
           nv[q] = db * xv + da * m;
         } else {
+          const double xm = epi.xm1.soa[idx];
           double tv = 2. * db * xv + 2. * da * m;
-          tv += -1.0 * epi.xm1.soa[idx];
+          tv += -1.0 * xm;
           nv[q] = tv;
-          epi.y[idx] += ck * tv;
+          if (epi.ymask == 4) {
+            epi.y[idx] += ck * tv;
+          } else if (epi.ymask) {  // x_{k-1}, x_k and x_{k+1} terms in one update of y
+            double add = 0.0;
+            if (epi.ymask & 1) add += cm1 * xm;
+            if (epi.ymask & 2) add += c0 * xv;
+            if (epi.ymask & 4) add += ck * tv;
+            epi.y[idx] += add;
+          }
         }
         v.soa[idx] = nv[q];
       }
