@@ -464,6 +464,8 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // (the list entries and the neighbours' records).
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
+constexpr int kBaseBits = 48;       // entry offsets below 2^48
+constexpr int64_t kBaseMask = (int64_t(1) << kBaseBits) - 1;
 
 struct BoxArgs {
   const int32_t* cs;     // cell starts, brick order
@@ -474,7 +476,8 @@ struct BoxArgs {
   const int64_t* off;    // [n_bricks + 1] first record of each brick
   const int64_t* eoff;   // [n_bricks + 1] first entry of each brick
   int32_t* tgt;          // [n] slots of each brick, longest lists first (staged bricks)
-  int64_t* base;         // [n] entry 0 of the target at each position of tgt
+  int64_t* base;         // [n] entry 0 of the target at each position of tgt (low kBaseBits),
+                         // its own record in a staged brick (above)
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
   uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
@@ -713,18 +716,22 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
     const int64_t bb = b.eoff[brick] + 64 * int64_t(wpre[i >> 6]) + (i & 63);
-    b.base[sb + i] = bb;
-    const int cnt = a.cnt[s];
-    if (cnt == 0) continue;  // an empty list has no storage
     const float4 po = a.pc32[s];
     const uint32_t co = __float_as_uint(po.w);
     const int io = int(co & ((1u << kCellBitsX) - 1));
     const int jo = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1));
     const int ko = int(co >> (kCellBitsX + kCellBitsY));
+    // staged bricks: the target's own record, packed above the entry offset
+    const int bx = io - B.ox, by = jo - B.oy, bz = ko - B.oz;
+    int self = 0;
     if (nrec > 0) {
-      const int bx = io - B.ox, by = jo - B.oy, bz = ko - B.oz;
       const int qs = (bz * B.ey + by) * B.ex + bx;
-      const int self = cst[qs] + int(s - cfirst[qs]);
+      self = cst[qs] + int(s - cfirst[qs]);
+    }
+    b.base[sb + i] = bb | (int64_t(self) << kBaseBits);
+    const int cnt = a.cnt[s];
+    if (cnt == 0) continue;  // an empty list has no storage
+    if (nrec > 0) {
       int64_t o = bb;
       box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
         for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
@@ -842,7 +849,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
       const int64_t s = b.tgt[sb + i];
       double acc[1][3];
-      fp32c_pairs_global(a, s, b.ab, b.base[sb + i], xg, acc);
+      fp32c_pairs_global(a, s, b.ab, b.base[sb + i] & kBaseMask, xg, acc);
       apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
     }
     return;
@@ -877,12 +884,13 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   constexpr int U = 8;  // entries per group, the next group loaded one ahead
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
-    const int64_t base = b.base[sb + i];
-    const float4 po = a.pc32[s];
+    const int64_t bpk = b.base[sb + i];
+    const int64_t base = bpk & kBaseMask;
+    const float4 po = rpos[int(bpk >> kBaseBits)];  // the target's own record
     const uint32_t co = __float_as_uint(po.w);
-    const int bx = int(co & ((1u << kCellBitsX) - 1)) - B.ox;
-    const int by = int((co >> kCellBitsX) & ((1u << kCellBitsY) - 1)) - B.oy;
-    const int bz = int(co >> (kCellBitsX + kCellBitsY)) - B.oz;
+    const int bx = int(co & ((1u << kBoxBits) - 1));
+    const int by = int((co >> kBoxBits) & ((1u << kBoxBits) - 1));
+    const int bz = int(co >> (2 * kBoxBits));
     const int cnt = a.cnt[s];
     float ps[3] = {0.f, 0.f, 0.f};
     KahanF kh[3];
