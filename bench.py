@@ -56,12 +56,12 @@ def parse():
 # one right-hand side): per ordered pair the 2-byte neighbour record number
 # and the 8-byte pair tensor (A, Bx); per slot the operand and position
 # records read once
-# (16 + 16), x, x_{k-1} (24 each), y read and written (48), v and its
-# record written (24 + 16), the list length (4).  The staged neighbour
-# records are read once per brick box (~5x per slot at 1M beads): traffic
-# above the algorithmic bytes.
+# (16 + 16), x, x_{k-1} (24 each), y read and written on one step in three
+# (48 / 3), v and its record written (24 + 16), the list length (4).  The
+# staged neighbour records are read once per brick box (~3.4x per slot at
+# 1M beads): traffic above the algorithmic bytes.
 B_PAIR_APPLY32 = 2 + 8
-B_SLOT_APPLY = 16 + 16 + 24 + 24 + 48 + 24 + 16 + 4
+B_SLOT_APPLY = 16 + 16 + 24 + 24 + 48 // 3 + 24 + 16 + 4
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_beads1m.txt")
 
 
