@@ -132,6 +132,7 @@ int tune_pick(ggem_ctx* ctx, const std::string& key, int ncand, const std::funct
 // layout and record numbers.
 int prepare_box(ggem_ctx* ctx) {
   ctx->box_valid = false;
+  ctx->box_unstaged = false;
   ctx->box_max = 0;
   if (!box_supported(ctx)) return GGEM_OK;
   const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
@@ -145,7 +146,7 @@ int prepare_box(ggem_ctx* ctx) {
   ctx->box_off = ctx->box_cnt + ctx->box_brick_cap;
   ctx->box_ecnt = ctx->box_cnt + 2 * ctx->box_brick_cap;
   ctx->box_eoff = ctx->box_cnt + 3 * ctx->box_brick_cap;
-  if (!ctx->box_max_d) GGEM_HIP(ctx, hipMalloc(&ctx->box_max_d, 2 * sizeof(int32_t)));
+  if (!ctx->box_max_d) GGEM_HIP(ctx, hipMalloc(&ctx->box_max_d, 3 * sizeof(int32_t)));
   int rc;
   if ((rc = ensure(ctx, &ctx->box_tgt, &ctx->box_tgt_cap, size_t(ctx->n)))) return rc;
   if ((rc = ensure(ctx, &ctx->box_base, &ctx->box_base_cap, size_t(ctx->n)))) return rc;
@@ -156,10 +157,10 @@ int prepare_box(ggem_ctx* ctx) {
       (e = launch_scan64(ctx, ctx->box_ecnt, int64_t(nb), ctx->box_eoff)) != hipSuccess)
     return hip_fail(ctx, e, "box scan");
   int64_t total[2] = {0, 0};
-  int32_t mx[2] = {0, 0};
+  int32_t mx[3] = {0, 0, 0};
   GGEM_HIP(ctx, hipMemcpyAsync(&total[0], ctx->box_off + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
   GGEM_HIP(ctx, hipMemcpyAsync(&total[1], ctx->box_eoff + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(mx, ctx->box_max_d, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(mx, ctx->box_max_d, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   // no brick staged, or a brick too crowded for the entry layout: the
   // list-order kernels serve
@@ -169,15 +170,27 @@ int prepare_box(ggem_ctx* ctx) {
   const size_t ne = size_t(std::max<int64_t>(total[1], 1));
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
   if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
+  // bricks that are not staged fill their tensors from the lists
+  ctx->box_unstaged = mx[2] > 0;
+  if (ctx->box_unstaged && (rc = ensure_ell(ctx))) return rc;
   if ((e = launch_box_fill(ctx, mx[0], mx[1])) != hipSuccess) return hip_fail(ctx, e, "box_fill");
   ctx->box_max = mx[0];
   ctx->box_valid = true;
   return GGEM_OK;
 }
 
-int ensure_tensors(ggem_ctx* ctx, int) {
+int ensure_ell(ggem_ctx* ctx) {
+  if (ctx->ell_valid) return GGEM_OK;
+  hipError_t e = launch_fill_ell(ctx);
+  if (e != hipSuccess) return hip_fail(ctx, e, "fill_ell");
+  ctx->ell_valid = true;
+  return GGEM_OK;
+}
+
+int ensure_tensors(ggem_ctx* ctx, int nrhs) {
   int rc;
   if (ctx->p.precision == GGEM_FP64) {
+    if ((rc = ensure_ell(ctx))) return rc;
     if (ctx->coef64_valid) return GGEM_OK;
     size_t cap = size_t(ctx->coef64_cap);
     if ((rc = ensure(ctx, &ctx->coef64, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1))))) return rc;
@@ -193,6 +206,9 @@ int ensure_tensors(ggem_ctx* ctx, int) {
     if ((rc = prepare_box(ctx))) return rc;
     ctx->box_built = true;
   }
+  // the list-order kernels read the lists: several operands, no staged
+  // bricks (or some unstaged), and the autotuner, which times every shape
+  if (nrhs > 1 || !ctx->box_valid || ctx->box_unstaged || ctx->tune.on) return ensure_ell(ctx);
   return GGEM_OK;
 }
 
@@ -813,7 +829,7 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   size_t cap = size_t(ctx->ell_cap);
   if ((rc = ensure(ctx, &ctx->ell, &cap, size_t(std::max<int64_t>(tot[1], 1))))) return rc;
   ctx->ell_cap = int64_t(cap);
-  if ((e = launch_fill_ell(ctx)) != hipSuccess) return hip_fail(ctx, e, "fill_ell");
+  ctx->ell_valid = false;  // filled by ensure_ell when a kernel that reads them runs
   ctx->elem_valid = true;
   ctx->built = true;
   return GGEM_OK;

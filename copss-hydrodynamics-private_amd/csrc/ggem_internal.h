@@ -153,6 +153,7 @@ struct ggem_ctx {
   int64_t* pairs_d = nullptr;     // pair count reduction
   int64_t pairs = -1;
   bool built = false;
+  bool ell_valid = false;  // the sliced lists are filled (on demand: the staged FP32C sums do not read them)
 
   // particle neighbour lists, sliced ELL: slice q holds targets 64q..64q+63,
   // entry k of lane l at ell[slice_off[q] + 64 k + l]
@@ -194,10 +195,11 @@ struct ggem_ctx {
   size_t box_ab_cap = 0;
   uint16_t* box_idx = nullptr;    // [entries] the neighbour's record number (staged bricks)
   size_t box_idx_cap = 0;
-  int32_t* box_max_d = nullptr;   // [2] largest record count of a staged brick, most targets of a brick
+  int32_t* box_max_d = nullptr;   // [3] largest record count of a staged brick, most targets of a brick, unstaged bricks with targets
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
   bool box_valid = false;         // ... and some brick is staged
+  bool box_unstaged = false;      // ... and some brick with targets is not
   int bsign = 1;  // sign of Bx: +1 when xi > alpha
 
   // element particle lists (spreading), CSR over elements
@@ -332,7 +334,7 @@ struct ChebEpi {
 };
 // pair tensors of every list entry for the given precision
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);
-// FP32C staged records: box_cnt (and box_max_d, cleared first) per brick
+// FP32C staged records: box_cnt per brick (and box_max_d by a reduction)
 hipError_t launch_box_count(ggem_ctx* ctx);
 // records, target layout and record numbers of every brick, with box_off
 // and box_eoff scanned from the counts

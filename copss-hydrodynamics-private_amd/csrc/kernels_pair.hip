@@ -472,7 +472,7 @@ struct BoxArgs {
   int32_t reach[3];      // stencil reach of a plain context
   int64_t* cnt;          // [n_bricks] records of each staged brick (k_box_count)
   int64_t* ecnt;         // [n_bricks] entries of each brick's layout (k_box_count)
-  int32_t* maxrec;       // [2] largest staged record count, most targets of a brick
+  int32_t* maxrec;       // [3] largest staged record count, most targets of a brick, unstaged bricks
   const int64_t* off;    // [n_bricks + 1] first record of each brick
   const int64_t* eoff;   // [n_bricks + 1] first entry of each brick
   int32_t* tgt;          // [n] slots of each brick, longest lists first (staged bricks)
@@ -596,26 +596,32 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
 // reduction instead of an atomic per brick on the same two words).
 constexpr int kMaxThreads = 1024;
 __global__ void __launch_bounds__(kMaxThreads) k_box_max(BoxArgs b, int64_t nbricks) {
-  __shared__ int32_t s_m[2][kMaxThreads / 64];
-  int mr = 0, mt = 0;
+  __shared__ int32_t s_m[3][kMaxThreads / 64];
+  int mr = 0, mt = 0, nu = 0;  // nu: bricks with targets that are not staged
   for (int64_t k = threadIdx.x; k < nbricks; k += kMaxThreads) {
+    const int nt = b.cs[(k + 1) * kBrickCells] - b.cs[k * kBrickCells];
     mr = max(mr, int(b.cnt[k]));
-    mt = max(mt, b.cs[(k + 1) * kBrickCells] - b.cs[k * kBrickCells]);
+    mt = max(mt, nt);
+    nu += nt > 0 && b.cnt[k] == 0 ? 1 : 0;
   }
   mr = wave_max(mr);
   mt = wave_max(mt);
+  nu = wave_sum(nu);
   if ((threadIdx.x & 63) == 0) {
     s_m[0][threadIdx.x >> 6] = mr;
     s_m[1][threadIdx.x >> 6] = mt;
+    s_m[2][threadIdx.x >> 6] = nu;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kMaxThreads / 64; ++w) {
       mr = max(mr, s_m[0][w]);
       mt = max(mt, s_m[1][w]);
+      nu += s_m[2][w];
     }
     b.maxrec[0] = mr;
     b.maxrec[1] = mt;
+    b.maxrec[2] = nu;
   }
 }
 
