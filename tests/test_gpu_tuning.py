@@ -151,3 +151,33 @@ def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks():
     m.reinit(x, f)
     u_ref = m.local_velocity_beads()
     assert rel_l2(r0[0], u_ref) < 1e-6
+
+
+def test_lists_filled_on_demand_between_one_and_several_operands():
+    """FP32C with one operand runs the staged kernels, which never read the
+    sliced lists, so a build skips their fill; a later step with several
+    operands (list-order kernels) fills them then.  Its results equal those
+    of the same step on a context whose first use needed the lists."""
+    cfg, x, f = W.make_inputs("beads20k")
+    cfg["precision"] = G.FP32C
+    dw = G.normal(3 * 3 * len(x), 23).reshape(3, -1, 3)
+    a = G.Context(cfg)
+    b = G.Context(cfg)
+    try:
+        a.reinit(x, f)
+        b.reinit(x, f)
+        u1 = a.local_velocity_beads()            # staged kernels only
+        y1, _, _ = a.cheb_brownian(8, 0.05, 3.0, dw[:1], mode=G.CHEB_FIXED, want_disp=False)
+        ya, da, _ = a.cheb_brownian(8, 0.05, 3.0, dw, mode=G.CHEB_FIXED)  # lists filled here
+        yb, db, _ = b.cheb_brownian(8, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+        assert np.array_equal(ya, yb) and np.array_equal(da, db)
+        assert np.array_equal(u1, b.local_velocity_beads())
+        # a rebuild invalidates the lists: same results again after it
+        a.build_neighbors()
+        yc, dc, _ = a.cheb_brownian(8, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+        assert np.array_equal(ya, yc) and np.array_equal(da, dc)
+        y1b, _, _ = b.cheb_brownian(8, 0.05, 3.0, dw[:1], mode=G.CHEB_FIXED, want_disp=False)
+        assert np.array_equal(y1, y1b)
+    finally:
+        a.close()
+        b.close()
