@@ -1,6 +1,8 @@
 // kernels_build.hip -- (a) the neighbour build: element binning, counting
-// sort of the particles into cell order (stable by particle id), and the
-// CSR emission of the reference's lists for export.
+// sort of the particles into cell order (stable by particle id), the
+// per-slot list lengths and sliced lists and the element lists of the
+// spreading (built brick by brick from LDS-staged cell starts), and the CSR
+// emission of the reference's lists for export.
 //
 // Reference: PointMesh<3>::reinit (point_mesh.C:245-289) with
 // build_elem_point_containing_list (:573-648), build_elem_point_neighbor_list
