@@ -181,8 +181,19 @@ int prepare_box(ggem_ctx* ctx) {
 
 int ensure_ell(ggem_ctx* ctx) {
   if (ctx->ell_valid) return GGEM_OK;
-  hipError_t e = launch_fill_ell(ctx);
-  if (e != hipSuccess) return hip_fail(ctx, e, "fill_ell");
+  // slice offsets (the scan of the slice lengths of the count pass), one
+  // synchronisation for the size, then the lists
+  const int64_t nsl = (ctx->n + kSlice - 1) / kSlice;
+  hipError_t e = launch_scan64(ctx, ctx->slice_len, nsl, ctx->slice_off);
+  if (e != hipSuccess) return hip_fail(ctx, e, "scan64");
+  int64_t len = 0;
+  int rc = d2h(ctx, &len, ctx->slice_off + nsl, sizeof(int64_t));
+  if (rc) return rc;
+  ctx->ell_len = len;
+  size_t cap = size_t(ctx->ell_cap);
+  if ((rc = ensure(ctx, &ctx->ell, &cap, size_t(std::max<int64_t>(len, 1))))) return rc;
+  ctx->ell_cap = int64_t(cap);
+  if ((e = launch_fill_ell(ctx)) != hipSuccess) return hip_fail(ctx, e, "fill_ell");
   ctx->ell_valid = true;
   return GGEM_OK;
 }
@@ -385,6 +396,7 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->box_idx);
   f(ctx->box_ab);
   f(ctx->box_max_d);
+  f(ctx->brick_recs);
   f(ctx->pc32);
   f(ctx->coef64);
   f(ctx->noise);
@@ -806,17 +818,19 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   if ((e = launch_scan(ctx)) != hipSuccess) return hip_fail(ctx, e, "scan");
   if ((e = launch_place(ctx)) != hipSuccess) return hip_fail(ctx, e, "place");
   if ((e = launch_gather_forces(ctx)) != hipSuccess) return hip_fail(ctx, e, "gather_forces");
-  const int64_t nsl = (ctx->n + kSlice - 1) / kSlice;
+  {
+    int rc0 = ensure(ctx, &ctx->brick_recs, &ctx->brick_recs_cap,
+                     size_t(std::max<int64_t>(ctx->grid.n_cells >> (3 * kBrickBits), 1)));
+    if (rc0) return rc0;
+  }
   if ((e = launch_count_lists(ctx)) != hipSuccess) return hip_fail(ctx, e, "count");
-  if ((e = launch_scan64(ctx, ctx->slice_len, nsl, ctx->slice_off)) != hipSuccess)
-    return hip_fail(ctx, e, "scan64");
-  // one synchronisation: the out-of-domain flag, the pair count and the list size
+  // one synchronisation: the out-of-domain flag and the pair count (the
+  // sliced layout is laid out and filled by ensure_ell when a kernel that
+  // reads it is about to run)
   int32_t flag = 0;
-  int64_t tot[2] = {0, 0};
+  int64_t pairs = 0;
   GGEM_HIP(ctx, hipMemcpyAsync(&flag, ctx->flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&tot[0], ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&tot[1], ctx->slice_off + nsl, sizeof(int64_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&pairs, ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
   int rc = sync(ctx);
   if (rc) return rc;
   if (flag) {
@@ -824,12 +838,9 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
     // point_mesh.C:602-612: a particle no element contains is fatal
     return fail(ctx, GGEM_ERR_OUT_OF_DOMAIN, "a particle lies outside the mesh");
   }
-  ctx->pairs = tot[0];
-  ctx->ell_len = tot[1];
-  size_t cap = size_t(ctx->ell_cap);
-  if ((rc = ensure(ctx, &ctx->ell, &cap, size_t(std::max<int64_t>(tot[1], 1))))) return rc;
-  ctx->ell_cap = int64_t(cap);
-  ctx->ell_valid = false;  // filled by ensure_ell when a kernel that reads them runs
+  ctx->pairs = pairs;
+  ctx->ell_len = -1;
+  ctx->ell_valid = false;
   ctx->elem_valid = true;
   ctx->built = true;
   return GGEM_OK;

@@ -195,6 +195,8 @@ struct ggem_ctx {
   size_t box_ab_cap = 0;
   uint16_t* box_idx = nullptr;    // [entries] the neighbour's record number (staged bricks)
   size_t box_idx_cap = 0;
+  int32_t* brick_recs = nullptr;  // [n_bricks] particles in each brick's box (list count pass)
+  size_t brick_recs_cap = 0;
   int32_t* box_max_d = nullptr;   // [3] largest record count of a staged brick, most targets of a brick, unstaged bricks with targets
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists

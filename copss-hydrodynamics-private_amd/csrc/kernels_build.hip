@@ -295,6 +295,7 @@ struct BrickArgs {
   const SysGrid* sys;        // batched contexts (null: one system)
   const int32_t* sys_slot;
   int32_t cells_cap;         // box cells the LDS tables hold
+  int32_t* recs;             // [n_bricks] particles in the box of each brick the count pass staged
 };
 
 // grid, rows and box of the brick whose first slot is sb
@@ -343,6 +344,7 @@ __global__ void __launch_bounds__(kBrickThreads) k_brick_count(BrickArgs a, int3
   const SysView v = brick_view<BATCH>(a, brick, sb, B);
   if (brick_staged(a, B)) {
     stage_box_cells<kBrickThreads>(B, a.cs, nullptr, cst, s_w);
+    if (threadIdx.x == 0) a.recs[brick] = cst[B.nbc];  // (the staged-record census reuses it)
     for (int32_t s = sb + int32_t(threadIdx.x); s < se; s += kBrickThreads) {
       int ci, cj, ck;
       cell_ijk(v.g, a.slot_cell[s] - v.coff, ci, cj, ck);
@@ -750,6 +752,7 @@ BrickArgs brick_args(const ggem_ctx* ctx) {
   int cap = cells(ctx->reach);
   for (int k = 0; k < ctx->nsys && ctx->nsys > 1; ++k) cap = std::max(cap, cells(ctx->sys_h[k].reach));
   a.cells_cap = std::min(cap, kBoxMaxCells);
+  a.recs = ctx->brick_recs;
   return a;
 }
 

@@ -473,6 +473,7 @@ struct BoxArgs {
   int64_t* cnt;          // [n_bricks] records of each staged brick (k_box_count)
   int64_t* ecnt;         // [n_bricks] entries of each brick's layout (k_box_count)
   int32_t* maxrec;       // [3] largest staged record count, most targets of a brick, unstaged bricks
+  const int32_t* recs;   // [n_bricks] particles in each brick's box (from the list count pass)
   const int64_t* off;    // [n_bricks + 1] first record of each brick
   const int64_t* eoff;   // [n_bricks + 1] first entry of each brick
   int32_t* tgt;          // [n] slots of each brick, longest lists first (staged bricks)
@@ -513,7 +514,6 @@ __device__ __forceinline__ void box_init(BoxGeom& B, const PairArgs& a, const in
 // box fits the packing and holds at most kBoxMaxRecs records), the order of
 // its targets and the size of its entry layout.
 __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b) {
-  __shared__ int32_t s_w[kBoxThreads / 64];
   __shared__ int32_t keys[kBoxSortMax];   // list lengths in slot order
   __shared__ int32_t skeys[kBoxSortMax];  // the same, longest first
   __shared__ int64_t s_e[kBoxThreads / 64];
@@ -526,33 +526,9 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_count(PairArgs a, BoxArgs b
   if (nt > 0) {  // uniform over the block
     BoxGeom B;
     box_init(B, a, b.reach, brick, sb);
-    if (B.fits()) {
-      int mine = 0;
-      constexpr int G = 8;  // cells per thread and step, their loads in flight together
-      for (int q0 = int(threadIdx.x); q0 < B.nbc; q0 += G * kBoxThreads) {
-        int64_t c[G];
-        bool in[G];
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-          const int q = q0 + u * kBoxThreads;
-          in[u] = q < B.nbc && B.cell(q, c[u]);
-          if (!in[u]) c[u] = 0;
-        }
-        int f[G], e[G];
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-          f[u] = b.cs[c[u]];
-          e[u] = b.cs[c[u] + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < G; ++u) mine += in[u] ? e[u] - f[u] : 0;
-      }
-      mine = wave_sum(mine);
-      if (lane == 0) s_w[wv] = mine;
-      __syncthreads();
-#pragma unroll
-      for (int w = 0; w < kBoxThreads / 64; ++w) total += s_w[w];
-    }
+    // the list count pass staged exactly the boxes that fit and counted
+    // their particles
+    if (B.fits()) total = b.recs[brick];
   }
   const bool staged = total > 0 && total <= kBoxMaxRecs;
   const bool sorted = staged && nt <= kBoxSortMax;
@@ -1077,6 +1053,7 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.cnt = ctx->box_cnt;
   b.ecnt = ctx->box_ecnt;
   b.maxrec = ctx->box_max_d;
+  b.recs = ctx->brick_recs;
   b.off = ctx->box_off;
   b.eoff = ctx->box_eoff;
   b.tgt = ctx->box_tgt;
