@@ -11,19 +11,23 @@
 // The targets' neighbour lists are stored sliced by wave (entry k of lane l
 // of slice q at slice_off[q] + 64k + l): every list access of a wave is one
 // coalesced transaction.  Positions do not move between neighbour builds,
-// so the pair tensor of every list entry is evaluated once per build
-// (k_tensors: the transcendental work, compute bound) and kept in the same
-// layout as G = A I + sgn w w^T, w = sqrt(|Bx|) x_ij: 16 bytes per entry in
-// FP32, 32 in FP64.  Every later application (the local velocity, each
-// Chebyshev term, M y) streams the tensors and gathers one operand record
-// per neighbour (k_apply: a block-sparse product, HBM bound).  Blocks are
-// renumbered per XCD so a chip partition keeps a contiguous stretch of cells
-// and their neighbours in its L2.
+// so pair tensors are evaluated once per build and streamed by every later
+// application (the local velocity, each Chebyshev term, M y).  FP64:
+// k_tensors stores G = A I + sgn w w^T, w = sqrt(|Bx|) x_ij (32 bytes per
+// entry) in list order and k_apply streams them, gathering one operand
+// record per neighbour.  FP32C, one operand: per brick of cells the
+// neighbours' records are staged in LDS and the entries stream a 2-byte
+// record number and the 8-byte (A, Bx) (k_box_fill, k_apply_box, below).
+// FP32C, several operands or the explicit minimum image: k_apply evaluates
+// (A, Bx) per pair from the coefficient table in LDS.  Blocks are
+// renumbered per XCD so a chip partition keeps a contiguous stretch of
+// cells and their neighbours in its L2.
 //
-// FP64: the reference's formula with libm-class erf/exp.  FP32C:
-// displacement in FP64, tensor in FP32 (erfc form, shared exponentials),
-// partial sums in FP32 merged every 8 neighbours into a Kahan accumulator;
-// the epilogue (combination, recurrence) runs in FP64.
+// FP64: the reference's formula with libm-class erf/exp.  FP32C: x_ij from
+// the cell-relative FP32 position records ((cell difference) h + offset
+// difference), (A, Bx) from a piecewise polynomial in r^2, partial sums in
+// FP32 merged every 8 neighbours into a Kahan accumulator; the epilogue
+// (combination, recurrence) runs in FP64.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -450,18 +454,19 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // One workgroup per brick of 8 x 8 x 8 cells.  The brick's box is the brick
 // widened by the stencil reach (periodic images unwrapped); each particle of
 // the box is one record, in box order: box cells x-major through the box,
-// particles of a cell in slot order.  Once per list build k_box_fill writes
-// every brick's records (slot, cell-relative FP32 position with the packed
-// box cell) and, in staged bricks, each list entry's neighbour record
-// number in its target's box (16 bits).  k_apply_box copies the
-// coefficient table, the records and their operands to LDS and streams the
-// record numbers in list order (2 bytes per pair), reading the neighbour
-// from LDS and (A, Bx) from the table.  The summation order (list order)
+// particles of a cell in slot order, so a stencil run of box cells is one
+// range of records.  Once per list build k_box_fill writes every staged
+// brick's records (slot, cell-relative FP32 position with the packed box
+// cell) and walks each target's stencil through them in list order
+// (box_walk): per entry the neighbour's record number (16 bits) and (A, Bx)
+// from the coefficient table.  k_apply_box copies the records and their
+// operands to LDS and streams record numbers and tensors (10 bytes per
+// pair), reading the neighbour from LDS.  The summation order (list order)
 // and the arithmetic are k_apply's; the displacement (box_t - box_s) h +
 // (offset_t - offset_s) has the same integer cell difference as the list
 // entry's periodic shift gives, so the results are bit-identical.  A brick
-// whose box holds more than kBoxMaxRecs records gathers from global memory
-// (the list entries and the neighbours' records).
+// whose box holds more than kBoxMaxRecs records reads its list and gathers
+// the neighbours' records from global memory.
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
 constexpr int kBaseBits = 48;       // entry offsets below 2^48
