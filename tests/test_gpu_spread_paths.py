@@ -1,4 +1,5 @@
-"""Spreading against the FP64 oracle (FP64 <= 1e-12, FP32C <= 1e-6
+"""Spreading, neighbour lists and local velocity against the FP64 oracle
+(FP64 <= 1e-12, FP32C <= 1e-6
 relative L2) on periodic, fine-grid, slit, wall-bounded, wrapping and
 crowded geometries, with the default and with autotuned launch shapes.
 The wrapping box exercises the explicit minimum image, taken per Gauss
@@ -80,3 +81,25 @@ def test_spread_paths_match_oracle(name, nq):
         # launch shapes do not change the arithmetic
         assert np.array_equal(out[prec, False][0], out[prec, True][0])
         assert np.array_equal(out[prec, False][1], out[prec, True][1])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_lists_and_local_velocity_paths_match_oracle(name):
+    """The per-brick list builds on the same geometries: neighbour lists
+    equal the oracle's, and the local velocity (FP64 through the sliced
+    lists, FP32C through the staged brick records or, in crowded or
+    self-overlapping boxes, the list-order fallback) matches it."""
+    cfg, x, f = CASES[name]()
+    m = O.Mesh(ora_cfg(cfg))
+    m.reinit(x, f)
+    u_o = m.local_velocity_beads()
+    off_o, ids_o = m.csr()
+    for prec in (G.FP64, G.FP32C):
+        ctx = G.Context(dict(cfg, precision=prec))
+        try:
+            ctx.reinit(x, f)
+            off, ids = ctx.neighbor_csr()
+            assert np.array_equal(off, off_o) and np.array_equal(ids, ids_o), (name, prec)
+            assert rel_l2(ctx.local_velocity_beads(), u_o) < TOL[prec], (name, prec)
+        finally:
+            ctx.close()
