@@ -868,7 +868,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   BoxGeom B;
   box_init(B, a, b.reach, brick, sb);
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
-  constexpr int U = 8;  // entries per group, the next group loaded one ahead
+  constexpr int U = 5;  // entries per group, the next group loaded one ahead (measured best of 3..10)
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
     const int64_t bpk = b.base[sb + i];
