@@ -103,6 +103,37 @@ def run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, steps, profile):
     return (t1 - t0) * 1e3 / max(steps, 1)
 
 
+def bench_fe_velocity(G, ctx, cfg, n):
+    """U_glob at every bead from a random nodal velocity on the workload's
+    HEX20 box mesh (ggem_global_velocity_beads).  Algorithmic HBM bytes per
+    launch: 64 per bead (32-byte position record, slot -> particle and cell
+    numbers, 24-byte result) + 80 per element holding a bead (its 20 node
+    numbers) + 24 per node of those elements (its velocity)."""
+    node_xyz, conn = G.hex20_box_mesh(cfg)
+    nn = node_xyz.shape[0]
+    ctx.set_fe_mesh(nn, conn)
+    node_u = G.normal(3 * nn, 4001).reshape(nn, 3)
+    ctx.global_velocity_beads(node_u)
+    reps = 20
+    ctx.sync()
+    ctx.profile_reset()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.global_velocity_beads(node_u)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    k_ms, k_n = ctx.profile_get("interp_hex20")
+    k_ms /= max(k_n, 1)
+    occ = np.unique(ctx.elem_ids())
+    traffic = 64 * n + 80 * occ.size + 24 * np.unique(conn[occ]).size
+    return {"kernel": "k_interp_hex20", "nodes": nn, "elements_with_beads": int(occ.size), "avg_launch_ms": k_ms,
+            "call_ms_with_copies": (t1 - t0) * 1e3 / reps,
+            "algorithmic_bytes": int(traffic), "achieved_gbs": traffic / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None,
+            "frac_hbm_peak": traffic / (k_ms * 1e-3) / PEAK_HBM if k_ms > 0 else None}
+
+
 def bench_batch(G, W, args):
     """Batched workload: 256 independent periodic suspensions (N_k uniform in
     [500, 8000], 5% volume fraction) with 16 noise vectors each; one step =
@@ -293,6 +324,7 @@ def main():
         out["fp32c_speedup_over_fp64"] = r64["ms"] / r32["ms"]
         out["fp32c_rel_l2_vs_fp64"] = {"u_local": rel(r32["u"], r64["u"]), "Fe": rel(r32["Fe"], r64["Fe"]),
                                        "brownian_disp": rel(r32["disp"], r64["disp"])}
+    out["fe_velocity"] = bench_fe_velocity(G, ctx, cfg, n)
     ctx.close()
     if not args.no_batch:
         out["batch256"] = bench_batch(G, W, args)

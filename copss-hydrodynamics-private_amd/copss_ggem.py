@@ -42,7 +42,8 @@ EXPORTS = [
     "ggem_sync", "ggem_set_particles", "ggem_set_forces", "ggem_build_neighbors",
     "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
-    "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_spread", "ggem_mobility_apply",
+    "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_set_fe_mesh",
+    "ggem_global_velocity_beads", "ggem_combine_velocity_fe", "ggem_spread", "ggem_mobility_apply",
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
     "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
     "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
@@ -116,6 +117,9 @@ def lib():
             "ggem_global_self_exclusion": [P, P],
             "ggem_combine_velocity": [P, P, P],
             "ggem_local_velocity_points": [P, I64, P, P, P],
+            "ggem_set_fe_mesh": [P, I64, P],
+            "ggem_global_velocity_beads": [P, P, P],
+            "ggem_combine_velocity_fe": [P, P, P],
             "ggem_spread": [P, I32, I32, P, P, P],
             "ggem_mobility_apply": [P, P, P],
             "ggem_set_noise": [P, I32, P],
@@ -221,6 +225,34 @@ def hex20_if_table(h, nqp_1d):
     IF = np.zeros(20 * nqp_1d ** 3)
     _check(lib().ggem_util_hex20_if_table(_p(h), nqp_1d, _p(IF)))
     return IF
+
+
+def hex20_box_mesh(cfg):
+    """Velocity nodes of the HEX20 box mesh of a workload: (node_xyz[n_nodes, 3],
+    elem_nodes[n_elem, 20]).  Nodes are the points of the half-cell lattice
+    with at most one odd index (vertices and edge mid-points), numbered with
+    x fastest, which is how build_cube (copss.C:995-1004) leaves them after
+    dropping the face and centre points; elements are e = i + nx (j + ny k)
+    with their nodes in HEX20 order (vertices, then the edge mid-nodes 0-1,
+    1-2, 2-3, 3-0, 0-4, 1-5, 2-6, 3-7, 4-5, 5-6, 6-7, 7-4)."""
+    n = np.array(cfg["n_elem"], np.int64)
+    lo = np.array(cfg["box_min"], float)
+    h = (np.array(cfg["box_max"], float) - lo) / n
+    m = 2 * n + 1
+    gx, gy, gz = np.meshgrid(np.arange(m[0]), np.arange(m[1]), np.arange(m[2]), indexing="ij")
+    lat = np.stack([gx.ravel(order="F"), gy.ravel(order="F"), gz.ravel(order="F")], axis=1)
+    keep = (lat % 2).sum(axis=1) <= 1
+    num = np.full(lat.shape[0], -1, np.int64)
+    num[keep] = np.arange(int(keep.sum()))
+    node_xyz = lo + 0.5 * lat[keep] * h
+    V = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2]])
+    E = [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 5), (2, 6), (3, 7), (4, 5), (5, 6), (6, 7), (7, 4)]
+    loc = np.concatenate([V, np.array([(V[a] + V[b]) // 2 for a, b in E])])  # [20, 3] on the lattice
+    e = np.arange(int(n.prod()))
+    ijk = np.stack([e % n[0], (e // n[0]) % n[1], e // (n[0] * n[1])], axis=1)
+    pts = 2 * ijk[:, None, :] + loc[None, :, :]
+    idx = pts[..., 0] + m[0] * (pts[..., 1] + m[1] * pts[..., 2])
+    return node_xyz, num[idx].astype(np.int32)
 
 
 class Context:
@@ -337,6 +369,25 @@ class Context:
         U = np.zeros((self.n, 3)) if out else None
         ug = None if u_glob is None else _f64(u_glob, (-1, 3))
         self._c(lib().ggem_combine_velocity(self._h, _p(ug), _p(U)))
+        return U
+
+    def set_fe_mesh(self, n_nodes, elem_nodes):
+        """Velocity mesh: elem_nodes[n_elem, 20] node numbers in HEX20 order."""
+        conn = np.ascontiguousarray(elem_nodes, dtype=np.int32)
+        self._c(lib().ggem_set_fe_mesh(self._h, int(n_nodes), _p(conn)))
+
+    def global_velocity_beads(self, node_u):
+        """U_glob of every bead from the nodal velocity node_u[n_nodes, 3]."""
+        nu = _f64(node_u, (-1, 3))
+        u = np.zeros((self.n, 3))
+        self._c(lib().ggem_global_velocity_beads(self._h, _p(nu), _p(u)))
+        return u
+
+    def combine_velocity_fe(self, node_u, out=True):
+        """combine_velocity with U_glob interpolated on the device from node_u."""
+        nu = _f64(node_u, (-1, 3))
+        U = np.zeros((self.n, 3)) if out else None
+        self._c(lib().ggem_combine_velocity_fe(self._h, _p(nu), _p(U)))
         return U
 
     def local_velocity_points(self, pts, elem=None):

@@ -405,6 +405,8 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->part_off_d);
   f(ctx->sys_part);
   f(ctx->sys_slot);
+  f(ctx->fe_conn);
+  f(ctx->fe_u);
   if (ctx->par_h) (void)hipHostFree(ctx->par_h);
   ctx->par_h = nullptr;
   if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
@@ -942,6 +944,70 @@ int ggem_combine_velocity(ggem_ctx* ctx, const double* u_glob, double* U) {
     ug = P.io;
   }
   hipError_t e = launch_pair_beads(ctx, ctx->p.precision, EPI_MOBILITY, ug, ctx->u_out);
+  if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
+  if (U && len > 0) return d2h(ctx, U, ctx->u_out, sizeof(double) * len);
+  return GGEM_OK;
+}
+
+int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (ctx->nsys > 1) return fail(ctx, GGEM_ERR_INVALID_ARG, "ggem_set_fe_mesh needs a single-system context");
+  const int64_t ne = ctx->grid.n_elem;
+  if (n_nodes < 1 || n_nodes > INT32_MAX || !elem_nodes)
+    return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid node count or connectivity");
+  for (int64_t k = 0; k < 20 * ne; ++k)
+    if (elem_nodes[k] < 0 || elem_nodes[k] >= n_nodes) {
+      char buf[128];
+      std::snprintf(buf, sizeof(buf), "node %d of element %lld is out of range", int(k % 20), (long long)(k / 20));
+      return fail(ctx, GGEM_ERR_INVALID_ARG, buf);
+    }
+  GGEM_HIP(ctx, hipSetDevice(ctx->device));
+  if (ctx->fe_conn) (void)hipFree(ctx->fe_conn);
+  ctx->fe_conn = nullptr;
+  ctx->fe_nodes = -1;
+  GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_conn), sizeof(int32_t) * 20 * ne));
+  int rc = h2d(ctx, ctx->fe_conn, elem_nodes, sizeof(int32_t) * 20 * ne);
+  if (rc) return rc;
+  if ((rc = ensure(ctx, &ctx->fe_u, &ctx->fe_u_cap, size_t(3 * n_nodes)))) return rc;
+  ctx->fe_nodes = n_nodes;
+  return GGEM_OK;
+}
+
+namespace {
+// U_glob of every bead into the device array out (particle order)
+int interp_global(ggem_ctx* ctx, const double* node_u, double* out) {
+  if (ctx->fe_nodes < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no FE mesh (call ggem_set_fe_mesh)");
+  if (!node_u) return fail(ctx, GGEM_ERR_INVALID_ARG, "null nodal velocity");
+  int rc = h2d(ctx, ctx->fe_u, node_u, sizeof(double) * 3 * ctx->fe_nodes);
+  if (rc) return rc;
+  hipError_t e = launch_interp_hex20(ctx, ctx->fe_conn, ctx->fe_u, out);
+  if (e != hipSuccess) return hip_fail(ctx, e, "interp_hex20");
+  return GGEM_OK;
+}
+}  // namespace
+
+int ggem_global_velocity_beads(ggem_ctx* ctx, const double* node_u, double* u_glob) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_global_velocity_beads"))) return rc;
+  if (!u_glob && ctx->n > 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "null output");
+  const int64_t len = 3 * ctx->n;
+  Pools& P = pools(ctx);
+  if ((rc = ensure(ctx, &P.io2, &P.io2_cap, size_t(len)))) return rc;
+  if ((rc = interp_global(ctx, node_u, P.io2))) return rc;
+  return len > 0 ? d2h(ctx, u_glob, P.io2, sizeof(double) * len) : GGEM_OK;
+}
+
+int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_combine_velocity_fe"))) return rc;
+  const int64_t len = 3 * ctx->n;
+  if ((rc = ensure_tensors(ctx))) return rc;
+  Pools& P = pools(ctx);
+  if ((rc = ensure(ctx, &P.io, &P.io_cap, size_t(len)))) return rc;
+  if ((rc = interp_global(ctx, node_u, P.io))) return rc;
+  hipError_t e = launch_pair_beads(ctx, ctx->p.precision, EPI_MOBILITY, P.io, ctx->u_out);
   if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
   if (U && len > 0) return d2h(ctx, U, ctx->u_out, sizeof(double) * len);
   return GGEM_OK;

@@ -238,6 +238,12 @@ struct ggem_ctx {
   size_t par_cap = 0;
   hipEvent_t par_ev = nullptr;
 
+  // FE mesh of the global velocity: 20 node numbers per element (HEX20)
+  int32_t* fe_conn = nullptr;
+  int64_t fe_nodes = -1;  // -1 until ggem_set_fe_mesh
+  double* fe_u = nullptr;  // [3 fe_nodes] nodal velocity
+  size_t fe_u_cap = 0;
+
   // generic scratch (grown on demand)
   void* scratch = nullptr;
 
@@ -298,6 +304,10 @@ hipError_t launch_fill_csr(ggem_ctx* ctx, const int64_t* offsets_d, int32_t* ids
 hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets_d,
                                  int32_t* ids_d, int pass);
 hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* elem);
+
+// ---- launchers (kernels_fe.hip) ----
+// out[3p + d] = HEX20 interpolant of node_u[3 node + d] at particle p in its element
+hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out);
 
 // ---- launchers (kernels_pair.hip) ----
 // A vector operand of the pair sum: FP64 SoA by slot for the element-wise

@@ -1,0 +1,99 @@
+// kernels_fe.hip -- the FEM part of each bead's velocity on the device: the
+// HEX20 interpolant of the nodal FEM velocity at the bead, in the element
+// the bead was binned to.
+//
+// Reference: PMSystemStokes::compute_point_velocity takes
+// Uglobal(d) = point_value(var_d, pt, *elem) (pm_system_stokes.C:441-443),
+// libMesh's System::point_value = sum_l phi_l(xi) u[dof_l] over the
+// element's 20 velocity dofs in node order, with xi the point mapped to the
+// reference cube [-1, 1]^3.  The generated meshes are axis-aligned boxes of
+// size h (build_cube, copss.C:995-1004), so the map is xi = 2 (x - c) / h
+// with c the element centre, and the second-order shape functions are the
+// serendipity ones of HEX20 (8 vertices, then the 12 edge mid-nodes
+// 0-1, 1-2, 2-3, 3-0, 0-4, 1-5, 2-6, 3-7, 4-5, 5-6, 6-7, 7-4).
+//
+// One thread per slot: the beads of one element sit in consecutive slots,
+// so a wave reads each element's 20 node numbers and 60 values once
+// through its L1, and the only scattered access is the 24-byte result in
+// particle order.
+#include "ggem_device.h"
+
+namespace ggem {
+
+namespace {
+
+constexpr int kFeBlock = 256;
+
+// phi_l(a, b, c) of the 20 HEX20 serendipity functions in libMesh's order
+__device__ __forceinline__ void hex20_phi(double a, double b, double c, double* phi) {
+  const double am = 1.0 - a, ap = 1.0 + a, bm = 1.0 - b, bp = 1.0 + b, cm = 1.0 - c, cp = 1.0 + c;
+  // vertices: 1/8 (1 + a a_l)(1 + b b_l)(1 + c c_l)(a a_l + b b_l + c c_l - 2)
+  phi[0] = 0.125 * am * bm * cm * (-a - b - c - 2.0);
+  phi[1] = 0.125 * ap * bm * cm * (a - b - c - 2.0);
+  phi[2] = 0.125 * ap * bp * cm * (a + b - c - 2.0);
+  phi[3] = 0.125 * am * bp * cm * (-a + b - c - 2.0);
+  phi[4] = 0.125 * am * bm * cp * (-a - b + c - 2.0);
+  phi[5] = 0.125 * ap * bm * cp * (a - b + c - 2.0);
+  phi[6] = 0.125 * ap * bp * cp * (a + b + c - 2.0);
+  phi[7] = 0.125 * am * bp * cp * (-a + b + c - 2.0);
+  // edge mid-nodes: 1/4 (1 - t^2) along the edge's axis times the two
+  // linear factors of its position on the other axes
+  const double qa = am * ap, qb = bm * bp, qc = cm * cp;
+  phi[8] = 0.25 * qa * bm * cm;   // 0-1
+  phi[9] = 0.25 * ap * qb * cm;   // 1-2
+  phi[10] = 0.25 * qa * bp * cm;  // 2-3
+  phi[11] = 0.25 * am * qb * cm;  // 3-0
+  phi[12] = 0.25 * am * bm * qc;  // 0-4
+  phi[13] = 0.25 * ap * bm * qc;  // 1-5
+  phi[14] = 0.25 * ap * bp * qc;  // 2-6
+  phi[15] = 0.25 * am * bp * qc;  // 3-7
+  phi[16] = 0.25 * qa * bm * cp;  // 4-5
+  phi[17] = 0.25 * ap * qb * cp;  // 5-6
+  phi[18] = 0.25 * qa * bp * cp;  // 6-7
+  phi[19] = 0.25 * am * qb * cp;  // 7-4
+}
+
+__global__ __launch_bounds__(kFeBlock) void k_interp_hex20(Grid g, int64_t n, const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ slot_cell,
+                                                           const double4* __restrict__ x4,
+                                                           const int32_t* __restrict__ conn,
+                                                           const double* __restrict__ node_u,
+                                                           double* __restrict__ out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int ci, cj, ck;
+  cell_ijk(g, slot_cell[s], ci, cj, ck);
+  const int64_t e = ci + int64_t(g.n[0]) * (cj + int64_t(g.n[1]) * ck);
+  const double4 p = x4[s];
+  const double a = 2.0 * (p.x - (g.bmin[0] + (double(ci) + 0.5) * g.h[0])) / g.h[0];
+  const double b = 2.0 * (p.y - (g.bmin[1] + (double(cj) + 0.5) * g.h[1])) / g.h[1];
+  const double c = 2.0 * (p.z - (g.bmin[2] + (double(ck) + 0.5) * g.h[2])) / g.h[2];
+  double phi[20];
+  hex20_phi(a, b, c, phi);
+  const int32_t* cn = conn + 20 * e;
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+#pragma unroll
+  for (int l = 0; l < 20; ++l) {
+    const int64_t nd = cn[l];
+    u0 += phi[l] * node_u[3 * nd];
+    u1 += phi[l] * node_u[3 * nd + 1];
+    u2 += phi[l] * node_u[3 * nd + 2];
+  }
+  const int64_t q = perm[s];
+  out[3 * q] = u0;
+  out[3 * q + 1] = u1;
+  out[3 * q + 2] = u2;
+}
+
+}  // namespace
+
+hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out) {
+  const int64_t n = ctx->n;
+  if (n == 0) return hipSuccess;
+  ProfScope ps(ctx, "interp_hex20");
+  hipLaunchKernelGGL(k_interp_hex20, dim3(unsigned((n + kFeBlock - 1) / kFeBlock)), dim3(kFeBlock), 0,
+                     ctx->stream, ctx->grid, n, ctx->perm, ctx->slot_cell, ctx->x4, conn, node_u, out);
+  return hipGetLastError();
+}
+
+}  // namespace ggem

@@ -147,6 +147,23 @@ int ggem_combine_velocity(ggem_ctx* ctx, const double* u_glob, double* U);
 int ggem_local_velocity_points(ggem_ctx* ctx, int64_t m, const double* pts, const int32_t* elem,
                                double* u);
 
+/* ---- FEM velocity at the beads ---------------------------------------- */
+/* The velocity mesh for ggem_global_velocity_beads: n_nodes nodes and, for
+ * each element e = i + nx (j + ny k), its 20 node numbers in libMesh's HEX20
+ * order (8 vertices, then the edge mid-nodes 0-1, 1-2, 2-3, 3-0, 0-4, 1-5,
+ * 2-6, 3-7, 4-5, 5-6, 6-7, 7-4), as Elem::node_id gives them for the
+ * build_cube mesh (copss.C:995-1004).  Copied to the device once; every
+ * number must lie in [0, n_nodes).  Single-system contexts only. */
+int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes /* 20*n_elem */);
+/* U_glob of every bead: point_value(u/v/w, pt, elem) of
+ * PMSystemStokes::compute_point_velocity (pm_system_stokes.C:441-443), the
+ * HEX20 interpolant of the nodal velocity node_u[3*node + d] in the bead's
+ * element of the last neighbour build.  u_glob: 3n, particle order. */
+int ggem_global_velocity_beads(ggem_ctx* ctx, const double* node_u, double* u_glob);
+/* ggem_combine_velocity with U_glob interpolated on the device from node_u
+ * (pm_system_stokes.C:441-457 as a whole); U may be NULL. */
+int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U);
+
 /* ---- (c) spreading ---------------------------------------------------- */
 /* AssembleStokes::compute_element_rhs point-force branch for every element
  * (assemble_stokes.h:134-141, assemble_stokes.C:643-672 via

@@ -1,4 +1,5 @@
-"""ctypes binding of the FP64 CPU oracle (liboracle_ggem.so).
+"""ctypes binding of the FP64 CPU oracle (liboracle_ggem.so), and the HEX20
+interpolation of the FEM velocity at the beads in numpy.
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke() entry point and the
 cpu_baseline leg of bench.py, as the checker.  The product path never imports
@@ -303,3 +304,53 @@ class Mesh:
         if rc != 0:
             raise ValueError("cheb failed")
         return y, disp, info
+
+
+# ---- FEM velocity at a bead: HEX20 interpolation (numpy) ------------------
+# libMesh 1.1.0's FE<3, LAGRANGE> SECOND on HEX20 (fe_lagrange_shape_3D.C;
+# libMesh is not part of the reference tree) is the 20-node serendipity
+# element: with node l at (x_l, y_l, z_l) in {-1, 0, 1}^3,
+#   vertex:        1/8 (1 + a x_l)(1 + b y_l)(1 + c z_l)(a x_l + b y_l + c z_l - 2)
+#   edge (x_l=0):  1/4 (1 - a^2)(1 + b y_l)(1 + c z_l), and likewise in y, z.
+HEX20_VERTICES = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                           [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+HEX20_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 5), (2, 6), (3, 7),
+               (4, 5), (5, 6), (6, 7), (7, 4)]
+HEX20_NODES = np.concatenate([HEX20_VERTICES,
+                              np.array([0.5 * (HEX20_VERTICES[a] + HEX20_VERTICES[b])
+                                        for a, b in HEX20_EDGES])])
+
+
+def hex20_phi(xi):
+    """phi[m, 20] of the HEX20 shape functions at reference points xi[m, 3]."""
+    xi = np.asarray(xi, float).reshape(-1, 3)
+    a, b, c = xi[:, 0:1], xi[:, 1:2], xi[:, 2:3]
+    xl, yl, zl = HEX20_NODES[:, 0], HEX20_NODES[:, 1], HEX20_NODES[:, 2]
+    fa, fb, fc = 1 + a * xl, 1 + b * yl, 1 + c * zl
+    vert = 0.125 * fa * fb * fc * (a * xl + b * yl + c * zl - 2)
+    ex = 0.25 * (1 - a * a) * fb * fc
+    ey = 0.25 * fa * (1 - b * b) * fc
+    ez = 0.25 * fa * fb * (1 - c * c)
+    is_v = np.arange(20) < 8
+    return np.where(is_v, vert, np.where(xl == 0, ex, np.where(yl == 0, ey, ez)))
+
+
+def fe_point_value_hex20(cfg, xyz, elem, elem_nodes, node_u):
+    """U_glob of each bead: point_value(var, pt, *elem) for var = u, v, w
+    (pm_system_stokes.C:441-443), i.e. sum_l phi_l(xi) u[node_l] in node
+    order, with xi = 2 (x - c_e) / h in the bead's element e of the
+    axis-aligned box mesh.  xyz[m, 3], elem[m], elem_nodes[n_elem, 20],
+    node_u[n_nodes, 3]."""
+    n = np.array(cfg["n_elem"], np.int64)
+    lo = np.array(cfg["box_min"], float)
+    h = (np.array(cfg["box_max"], float) - lo) / n
+    e = np.asarray(elem, np.int64)
+    ijk = np.stack([e % n[0], (e // n[0]) % n[1], e // (n[0] * n[1])], axis=1)
+    centre = lo + (ijk + 0.5) * h
+    phi = hex20_phi(2.0 * (np.asarray(xyz, float).reshape(-1, 3) - centre) / h)
+    conn = np.asarray(elem_nodes, np.int64)[e]
+    node_u = np.asarray(node_u, float).reshape(-1, 3)
+    u = np.zeros((e.size, 3))
+    for l in range(20):
+        u += phi[:, l:l + 1] * node_u[conn[:, l]]
+    return u
