@@ -15,7 +15,9 @@
 // One thread per slot: the beads of one element sit in consecutive slots,
 // so a wave reads each element's 20 node numbers and 60 values once
 // through its L1, and the only scattered access is the 24-byte result in
-// particle order.
+// particle order.  Blocks are renumbered so that each XCD takes one
+// contiguous range of slots (whole bricks of cells): the nodes that
+// neighbouring elements share are then fetched into one L2, not several.
 #include "ggem_device.h"
 
 namespace ggem {
@@ -23,6 +25,7 @@ namespace ggem {
 namespace {
 
 constexpr int kFeBlock = 256;
+constexpr int kFeGroup = 10;  // nodal values loaded per group
 
 // phi_l(a, b, c) of the 20 HEX20 serendipity functions in libMesh's order
 __device__ __forceinline__ void hex20_phi(double a, double b, double c, double* phi) {
@@ -59,7 +62,7 @@ __global__ __launch_bounds__(kFeBlock) void k_interp_hex20(Grid g, int64_t n, co
                                                            const int32_t* __restrict__ conn,
                                                            const double* __restrict__ node_u,
                                                            double* __restrict__ out) {
-  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_block(blockIdx.x, gridDim.x) * kFeBlock + threadIdx.x;
   if (s >= n) return;
   int ci, cj, ck;
   cell_ijk(g, slot_cell[s], ci, cj, ck);
@@ -71,13 +74,30 @@ __global__ __launch_bounds__(kFeBlock) void k_interp_hex20(Grid g, int64_t n, co
   double phi[20];
   hex20_phi(a, b, c, phi);
   const int32_t* cn = conn + 20 * e;
+  int nds[20];
+#pragma unroll
+  for (int l = 0; l < 20; ++l) nds[l] = cn[l];
+  // the nodal values in two groups of ten, each group's 30 loads issued
+  // together before its sums (one load at a time left the kernel waiting
+  // on memory latency; all 20 at once cost occupancy); sums in node order
   double u0 = 0.0, u1 = 0.0, u2 = 0.0;
 #pragma unroll
-  for (int l = 0; l < 20; ++l) {
-    const int64_t nd = cn[l];
-    u0 += phi[l] * node_u[3 * nd];
-    u1 += phi[l] * node_u[3 * nd + 1];
-    u2 += phi[l] * node_u[3 * nd + 2];
+  for (int l0 = 0; l0 < 20; l0 += kFeGroup) {
+    double v[3 * kFeGroup];
+#pragma unroll
+    for (int j = 0; j < kFeGroup; ++j) {
+      const int64_t nd = nds[l0 + j];
+      v[3 * j] = node_u[3 * nd];
+      v[3 * j + 1] = node_u[3 * nd + 1];
+      v[3 * j + 2] = node_u[3 * nd + 2];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kFeGroup; ++j) {
+      u0 += phi[l0 + j] * v[3 * j];
+      u1 += phi[l0 + j] * v[3 * j + 1];
+      u2 += phi[l0 + j] * v[3 * j + 2];
+    }
   }
   const int64_t q = perm[s];
   out[3 * q] = u0;
