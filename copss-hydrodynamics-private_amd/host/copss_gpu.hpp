@@ -168,6 +168,20 @@ class GGEMStokes {
           pm->ctx());
   }
 
+  // The velocity mesh once (20 node numbers per element in HEX20 order),
+  // then compute_point_velocity with point_value (pm_system_stokes.C:441-443)
+  // evaluated on the device from the nodal velocity node_u[3 * node + d].
+  void set_velocity_mesh(PointMesh* pm, int64_t n_nodes, const std::vector<int32_t>& elem_nodes) const {
+    if (int64_t(elem_nodes.size()) != 20 * pm->n_elem())
+      throw std::runtime_error("set_velocity_mesh: elem_nodes must hold 20 node numbers per element");
+    check(ggem_set_fe_mesh(pm->ctx(), n_nodes, elem_nodes.data()), pm->ctx());
+  }
+  void compute_point_velocity_fe(PointMesh* pm, const std::vector<double>& node_u,
+                                 std::vector<double>& pv) const {
+    pv.assign(3 * pm->num_particles(), 0.0);
+    check(ggem_combine_velocity_fe(pm->ctx(), node_u.data(), pv.data()), pm->ctx());
+  }
+
   // AssembleStokes::assemble_global_F spreading part: Fe of every element
   // (compute_element_rhs, assemble_stokes.C:643-672), IF table as
   // assemble_int_force builds it.

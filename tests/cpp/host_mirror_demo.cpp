@@ -89,6 +89,14 @@ int main() {
     std::vector<double> ug(300, 0.25), pv, Fe;
     ggem.compute_point_velocity(&pm, ug, pv);
     print_vec("b100_point_velocity", pv);
+    // the same with U_glob interpolated on the device from a uniform nodal
+    // velocity of 0.25 (every element with its own 20 nodes)
+    std::vector<int32_t> conn(20 * 8000);
+    for (int32_t k = 0; k < 20 * 8000; ++k) conn[k] = k;
+    ggem.set_velocity_mesh(&pm, 20 * 8000, conn);
+    std::vector<double> node_u(3 * 20 * 8000, 0.25), pv_fe;
+    ggem.compute_point_velocity_fe(&pm, node_u, pv_fe);
+    print_vec("b100_point_velocity_fe", pv_fe);
     ggem.element_force_vectors(&pm, 3, IF, Fe);
     double s = 0;
     for (double v : Fe) s += v * v;

@@ -49,6 +49,8 @@ def test_b100_velocity_spread_brownian(demo):
     m.reinit(x, f)
     pv = np.array(demo["b100_point_velocity"]).reshape(-1, 3)
     assert rel_l2(pv, m.combine_velocity(np.full_like(x, 0.25))) < 1e-12
+    pv_fe = np.array(demo["b100_point_velocity_fe"]).reshape(-1, 3)
+    assert rel_l2(pv_fe, m.combine_velocity(np.full_like(x, 0.25))) < 1e-12
     _, Fe = m.spread(3, G.hex20_if_table([2.0, 2.0, 2.0], 3), want_rho=False)
     assert abs(demo["b100_Fe_norm"] - np.linalg.norm(Fe)) < 1e-12 * np.linalg.norm(Fe)
     w = np.linalg.eigvalsh(m.mobility_dense())
