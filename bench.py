@@ -128,10 +128,37 @@ def bench_fe_velocity(G, ctx, cfg, n):
     k_ms /= max(k_n, 1)
     occ = np.unique(ctx.elem_ids())
     traffic = 64 * n + 80 * occ.size + 24 * np.unique(conn[occ]).size
-    return {"kernel": "k_interp_hex20", "nodes": nn, "elements_with_beads": int(occ.size), "avg_launch_ms": k_ms,
-            "call_ms_with_copies": (t1 - t0) * 1e3 / reps,
-            "algorithmic_bytes": int(traffic), "achieved_gbs": traffic / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None,
-            "frac_hbm_peak": traffic / (k_ms * 1e-3) / PEAK_HBM if k_ms > 0 else None}
+    out = {"kernel": "k_interp_hex20", "nodes": nn, "elements_with_beads": int(occ.size), "avg_launch_ms": k_ms,
+           "call_ms_with_copies": (t1 - t0) * 1e3 / reps,
+           "algorithmic_bytes": int(traffic), "achieved_gbs": traffic / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None,
+           "frac_hbm_peak": traffic / (k_ms * 1e-3) / PEAK_HBM if k_ms > 0 else None}
+    # nodal force: spreading (27 Gauss points) + k_gather_nodes and a host
+    # copy of 3 doubles per node, against ggem_spread's copy of Fe per
+    # element.  Algorithmic bytes of k_gather_nodes: per incidence its
+    # 4-byte (element, node) number and its 3 Fe values (24 B); per node its
+    # offset (8 B) and the 24-byte result.
+    IF = G.hex20_if_table(ctx.consts().h, 3)
+    ctx.spread_nodes(3, IF, nn)
+    ctx.sync()
+    ctx.profile_reset()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.spread_nodes(3, IF, nn)
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    g_ms, g_n = ctx.profile_get("gather_nodes")
+    g_ms /= max(g_n, 1)
+    t2 = time.perf_counter()
+    for _ in range(3):
+        ctx.spread(3, IF, want_rho=False)
+    t3 = time.perf_counter()
+    g_bytes = (4 + 24) * 20 * conn.shape[0] + (8 + 24) * nn
+    out["nodal_force"] = {"kernel": "k_gather_nodes", "avg_launch_ms": g_ms, "algorithmic_bytes": int(g_bytes),
+                          "achieved_gbs": g_bytes / (g_ms * 1e-3) / 1e9 if g_ms > 0 else None,
+                          "spread_nodes_call_ms_with_copy": (t1 - t0) * 1e3 / 3,
+                          "spread_fe_call_ms_with_copy": (t3 - t2) * 1e3 / 3}
+    return out
 
 
 def bench_batch(G, W, args):

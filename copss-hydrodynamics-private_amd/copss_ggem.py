@@ -43,7 +43,7 @@ EXPORTS = [
     "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_set_fe_mesh",
-    "ggem_global_velocity_beads", "ggem_combine_velocity_fe", "ggem_spread", "ggem_mobility_apply",
+    "ggem_global_velocity_beads", "ggem_combine_velocity_fe", "ggem_spread_nodes", "ggem_spread", "ggem_mobility_apply",
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
     "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
     "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
@@ -120,6 +120,7 @@ def lib():
             "ggem_set_fe_mesh": [P, I64, P],
             "ggem_global_velocity_beads": [P, P, P],
             "ggem_combine_velocity_fe": [P, P, P],
+            "ggem_spread_nodes": [P, I32, P, P],
             "ggem_spread": [P, I32, I32, P, P, P],
             "ggem_mobility_apply": [P, P, P],
             "ggem_set_noise": [P, I32, P],
@@ -389,6 +390,13 @@ class Context:
         U = np.zeros((self.n, 3)) if out else None
         self._c(lib().ggem_combine_velocity_fe(self._h, _p(nu), _p(U)))
         return U
+
+    def spread_nodes(self, nqp_1d, IF, n_nodes):
+        """Spreading summed into one force per node of the velocity mesh: [n_nodes, 3]."""
+        IF = _f64(IF).ravel()
+        F = np.zeros((int(n_nodes), 3))
+        self._c(lib().ggem_spread_nodes(self._h, nqp_1d, _p(IF), _p(F)))
+        return F
 
     def local_velocity_points(self, pts, elem=None):
         pts = _f64(pts, (-1, 3))

@@ -407,6 +407,8 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->sys_slot);
   f(ctx->fe_conn);
   f(ctx->fe_u);
+  f(ctx->fe_inc_off);
+  f(ctx->fe_inc);
   if (ctx->par_h) (void)hipHostFree(ctx->par_h);
   ctx->par_h = nullptr;
   if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
@@ -962,12 +964,30 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
       return fail(ctx, GGEM_ERR_INVALID_ARG, buf);
     }
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
-  if (ctx->fe_conn) (void)hipFree(ctx->fe_conn);
+  for (void* p : {static_cast<void*>(ctx->fe_conn), static_cast<void*>(ctx->fe_inc_off),
+                  static_cast<void*>(ctx->fe_inc)})
+    if (p) (void)hipFree(p);
   ctx->fe_conn = nullptr;
+  ctx->fe_inc_off = nullptr;
+  ctx->fe_inc = nullptr;
   ctx->fe_nodes = -1;
+  // incidences of each node, (element, local node) ascending: the order in
+  // which the nodal force sums the element vectors
+  std::vector<int64_t> off(n_nodes + 1, 0);
+  for (int64_t k = 0; k < 20 * ne; ++k) ++off[elem_nodes[k] + 1];
+  for (int64_t v = 0; v < n_nodes; ++v) off[v + 1] += off[v];
+  std::vector<int32_t> inc(20 * ne);
+  {
+    std::vector<int64_t> pos(off.begin(), off.end() - 1);
+    for (int64_t k = 0; k < 20 * ne; ++k) inc[pos[elem_nodes[k]]++] = int32_t(k);
+  }
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_conn), sizeof(int32_t) * 20 * ne));
+  GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_inc), sizeof(int32_t) * 20 * ne));
+  GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_inc_off), sizeof(int64_t) * (n_nodes + 1)));
   int rc = h2d(ctx, ctx->fe_conn, elem_nodes, sizeof(int32_t) * 20 * ne);
   if (rc) return rc;
+  if ((rc = h2d(ctx, ctx->fe_inc, inc.data(), sizeof(int32_t) * 20 * ne))) return rc;
+  if ((rc = h2d(ctx, ctx->fe_inc_off, off.data(), sizeof(int64_t) * (n_nodes + 1)))) return rc;
   if ((rc = ensure(ctx, &ctx->fe_u, &ctx->fe_u_cap, size_t(3 * n_nodes)))) return rc;
   ctx->fe_nodes = n_nodes;
   return GGEM_OK;
@@ -1011,6 +1031,26 @@ int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U) {
   if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
   if (U && len > 0) return d2h(ctx, U, ctx->u_out, sizeof(double) * len);
   return GGEM_OK;
+}
+
+int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F_node) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_spread_nodes"))) return rc;
+  if (ctx->fe_nodes < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no FE mesh (call ggem_set_fe_mesh)");
+  if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
+  if (!IF || !F_node) return fail(ctx, GGEM_ERR_INVALID_ARG, "null IF table or output");
+  if ((rc = ensure_elist(ctx))) return rc;
+  const int n_u = 20, nqp = nqp_1d * nqp_1d * nqp_1d;
+  const int64_t ne = ctx->grid.n_elem;
+  Pools& P = pools(ctx);
+  if ((rc = ensure(ctx, &P.ifd, &P.ifd_cap, size_t(n_u * nqp)))) return rc;
+  if ((rc = ensure(ctx, &P.fe, &P.fe_cap, size_t(ne) * 3 * n_u))) return rc;
+  if ((rc = h2d(ctx, P.ifd, IF, sizeof(double) * n_u * nqp))) return rc;
+  hipError_t e = launch_spread(ctx, ctx->p.precision, nqp_1d, n_u, P.ifd, nullptr, P.fe, true);
+  if (e != hipSuccess) return hip_fail(ctx, e, "spread");
+  if ((e = launch_gather_nodes(ctx, P.fe, ctx->fe_u)) != hipSuccess) return hip_fail(ctx, e, "gather_nodes");
+  return d2h(ctx, F_node, ctx->fe_u, sizeof(double) * 3 * ctx->fe_nodes);
 }
 
 int ggem_local_velocity_points(ggem_ctx* ctx, int64_t m, const double* pts, const int32_t* elem,

@@ -241,8 +241,10 @@ struct ggem_ctx {
   // FE mesh of the global velocity: 20 node numbers per element (HEX20)
   int32_t* fe_conn = nullptr;
   int64_t fe_nodes = -1;  // -1 until ggem_set_fe_mesh
-  double* fe_u = nullptr;  // [3 fe_nodes] nodal velocity
+  double* fe_u = nullptr;  // [3 fe_nodes] nodal velocity, then nodal force
   size_t fe_u_cap = 0;
+  int64_t* fe_inc_off = nullptr;  // [fe_nodes + 1] incidences of each node
+  int32_t* fe_inc = nullptr;      // [20 n_elem] 20 e + l, by node, (e, l) ascending
 
   // generic scratch (grown on demand)
   void* scratch = nullptr;
@@ -308,6 +310,8 @@ hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* e
 // ---- launchers (kernels_fe.hip) ----
 // out[3p + d] = HEX20 interpolant of node_u[3 node + d] at particle p in its element
 hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out);
+// out[3 node + d] = sum over the node's incidences (e, l) of Fe[e][l][d] (n_u = 20)
+hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out);
 
 // ---- launchers (kernels_pair.hip) ----
 // A vector operand of the pair sum: FP64 SoA by slot for the element-wise
@@ -363,8 +367,9 @@ hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x
                         const ChebEpi& epi);
 
 // ---- launchers (kernels_spread.hip) ----
+// Fe[e][j][l] (node_major: Fe[e][l][j])
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
-                         double* rho, double* Fe);
+                         double* rho, double* Fe, bool node_major = false);
 
 // ---- launchers (kernels_vec.hip) ----
 hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in_interleaved, double* out_slots);

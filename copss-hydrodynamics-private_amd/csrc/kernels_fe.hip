@@ -1,6 +1,13 @@
-// kernels_fe.hip -- the FEM part of each bead's velocity on the device: the
-// HEX20 interpolant of the nodal FEM velocity at the bead, in the element
-// the bead was binned to.
+// kernels_fe.hip -- the FEM side of the particle path on the device: the
+// HEX20 interpolant of the nodal FEM velocity at each bead, in the element
+// the bead was binned to, and the element force vectors of the spreading
+// summed into one force per node.
+//
+// Reference for the nodal force: AssembleStokes::assemble_global_F adds each
+// element's Fe into the global RHS by its dof indices
+// (rhs->add_vector(Fe, _dof_indices[elem_id]), assemble_stokes.C:508-513);
+// with one velocity dof per node and component, the point-force part is
+// F[node_l][d] += Fe[e][d][l].
 //
 // Reference: PMSystemStokes::compute_point_velocity takes
 // Uglobal(d) = point_value(var_d, pt, *elem) (pm_system_stokes.C:441-443),
@@ -105,7 +112,41 @@ __global__ __launch_bounds__(kFeBlock) void k_interp_hex20(Grid g, int64_t n, co
   out[3 * q + 2] = u2;
 }
 
+// Nodal force: each node sums the element vectors Fe[e][l][d] (the
+// spreading's node-major layout: one 24-byte read per incidence) of its
+// incidences (e, l) in ascending order.  A gather, so no atomics and the
+// same sums on every run.
+__global__ __launch_bounds__(kFeBlock) void k_gather_nodes(int64_t nn, const int64_t* __restrict__ off,
+                                                           const int32_t* __restrict__ inc,
+                                                           const double* __restrict__ Fe,
+                                                           double* __restrict__ out) {
+  const int64_t v = xcd_block(blockIdx.x, gridDim.x) * kFeBlock + threadIdx.x;
+  if (v >= nn) return;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  const int64_t k1 = off[v + 1];
+  for (int64_t k = off[v]; k < k1; ++k) {
+    const int64_t el = inc[k];
+    const int64_t e = el / 20, l = el - 20 * e;
+    const double* fe = Fe + 60 * e + 3 * l;
+    f0 += fe[0];
+    f1 += fe[1];
+    f2 += fe[2];
+  }
+  out[3 * v] = f0;
+  out[3 * v + 1] = f1;
+  out[3 * v + 2] = f2;
+}
+
 }  // namespace
+
+hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out) {
+  const int64_t nn = ctx->fe_nodes;
+  if (nn <= 0) return hipSuccess;
+  ProfScope ps(ctx, "gather_nodes");
+  hipLaunchKernelGGL(k_gather_nodes, dim3(unsigned((nn + kFeBlock - 1) / kFeBlock)), dim3(kFeBlock), 0,
+                     ctx->stream, nn, ctx->fe_inc_off, ctx->fe_inc, Fe, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out) {
   const int64_t n = ctx->n;

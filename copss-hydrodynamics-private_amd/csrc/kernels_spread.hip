@@ -48,6 +48,7 @@ struct SpreadArgs {
   const float4* pc32;  // FP32C: position relative to the cell centre, packed cell
   const float4* fg32;
   const double4* fg64;
+  int fe_node_major;  // Fe[e][l][j] (3 values per shape function) instead of Fe[e][j][l]
 };
 
 // e^{-a} for a >= 0 in FP32 from the hardware exp2: the rounding of a log2(e)
@@ -297,13 +298,14 @@ __global__ void __launch_bounds__(WAVES * 64) k_spread(SpreadArgs<T> a, const do
       const int l2 = idx / 3;
       const int j = idx - 3 * l2;
       if (l2 < p0 || l2 >= p1) continue;
-      double* o = stage + (l2 - p0) * per + j * n_u;
+      double* o = stage + (l2 - p0) * per + (a.fe_node_major ? j : j * n_u);
+      const int ls = a.fe_node_major ? 3 : 1;
       for (int l = 0; l < n_u; ++l) {
         const double* __restrict__ row = IF + l * NQP;
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NQP; ++q) s = __builtin_fma(row[q], r[m][q], s);
-        o[l] = s;
+        o[l * ls] = s;
       }
     }
     __syncthreads();
@@ -327,7 +329,7 @@ void launch_spk(ggem_ctx* ctx, const SpreadArgs<T>& a, const double* IF_d, int n
 
 template <class T, int NQ1>
 hipError_t launch_sp(ggem_ctx* ctx, int c, const double* gp, const double* IF_d, int n_u, double* rho,
-                     double* Fe) {
+                     double* Fe, bool node_major) {
   SpreadArgs<T> a;
   a.g = ctx->grid;
   a.alpha2 = T(ctx->gc.alpha2);
@@ -340,6 +342,7 @@ hipError_t launch_sp(ggem_ctx* ctx, int c, const double* gp, const double* IF_d,
   a.pc32 = ctx->pc32;
   a.fg32 = ctx->fg32;
   a.fg64 = ctx->fg64;
+  a.fe_node_major = node_major ? 1 : 0;
   if (c < 0 || c >= kNElem) return hipErrorInvalidValue;
   switch (kWaveChoices[c]) {
     case 2: launch_spk<T, NQ1, 2>(ctx, a, IF_d, n_u, rho, Fe); break;
@@ -352,7 +355,7 @@ hipError_t launch_sp(ggem_ctx* ctx, int c, const double* gp, const double* IF_d,
 }  // namespace
 
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
-                         double* rho, double* Fe) {
+                         double* rho, double* Fe, bool node_major) {
   double gp[3];
   if (nqp_1d == 2) {
     gp[0] = -std::sqrt(1. / 3.);
@@ -365,10 +368,10 @@ hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, cons
   }
   auto run = [&](int c) {
     if (nqp_1d == 2)
-      return precision == GGEM_FP64 ? launch_sp<double, 2>(ctx, c, gp, IF_d, n_u, rho, Fe)
-                                    : launch_sp<float, 2>(ctx, c, gp, IF_d, n_u, rho, Fe);
-    return precision == GGEM_FP64 ? launch_sp<double, 3>(ctx, c, gp, IF_d, n_u, rho, Fe)
-                                  : launch_sp<float, 3>(ctx, c, gp, IF_d, n_u, rho, Fe);
+      return precision == GGEM_FP64 ? launch_sp<double, 2>(ctx, c, gp, IF_d, n_u, rho, Fe, node_major)
+                                    : launch_sp<float, 2>(ctx, c, gp, IF_d, n_u, rho, Fe, node_major);
+    return precision == GGEM_FP64 ? launch_sp<double, 3>(ctx, c, gp, IF_d, n_u, rho, Fe, node_major)
+                                  : launch_sp<float, 3>(ctx, c, gp, IF_d, n_u, rho, Fe, node_major);
   };
   // rho and Fe are outputs only: idempotent
   const int c = tune_pick(ctx, tune_key("spread", precision, nqp_1d, ctx->grid.n_elem), kNElem, run);

@@ -163,6 +163,14 @@ int ggem_global_velocity_beads(ggem_ctx* ctx, const double* node_u, double* u_gl
 /* ggem_combine_velocity with U_glob interpolated on the device from node_u
  * (pm_system_stokes.C:441-457 as a whole); U may be NULL. */
 int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U);
+/* Spreading (as ggem_spread with n_u = 20) with the element vectors summed
+ * into one force per node of the ggem_set_fe_mesh mesh:
+ * F_node[3*node + d] = sum over the node's (element, local node l) of
+ * Fe[e][d][l], elements ascending: the point-force part of
+ * rhs->add_vector(Fe, dof_indices) in AssembleStokes::assemble_global_F
+ * (assemble_stokes.C:508-513), in node order.  Periodic images that the
+ * caller numbers as one node are summed into it. */
+int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F_node /* 3*n_nodes */);
 
 /* ---- (c) spreading ---------------------------------------------------- */
 /* AssembleStokes::compute_element_rhs point-force branch for every element

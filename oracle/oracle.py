@@ -354,3 +354,15 @@ def fe_point_value_hex20(cfg, xyz, elem, elem_nodes, node_u):
     for l in range(20):
         u += phi[:, l:l + 1] * node_u[conn[:, l]]
     return u
+
+
+def fe_nodal_force(elem_nodes, Fe, n_nodes):
+    """Element force vectors Fe[n_elem, 3, 20] summed into F[n_nodes, 3] by
+    the element's node numbers: the point-force part of
+    rhs->add_vector(Fe, dof_indices) (assemble_stokes.C:508-513), elements
+    and local nodes in ascending order."""
+    conn = np.asarray(elem_nodes, np.int64)
+    vals = np.transpose(np.asarray(Fe, float), (0, 2, 1)).reshape(-1, 3)  # [(e, l), d]
+    F = np.zeros((int(n_nodes), 3))
+    np.add.at(F, conn.ravel(), vals)
+    return F

@@ -4,7 +4,8 @@ values at the 20 nodes, partition of unity, exact reproduction of every
 field in the serendipity space (all quadratics plus x^2 y, ..., xyz, x^2 yz,
 ...) on anisotropic box meshes, and agreement with the library's
 quadrature table at the Gauss points.  Also checks the box-mesh builder's
-node coordinates and numbering."""
+node coordinates and numbering, and the oracle's sum of element force
+vectors into nodal forces (rhs->add_vector, assemble_stokes.C:508-513)."""
 import numpy as np
 import pytest
 
@@ -111,3 +112,27 @@ def test_box_mesh_nodes(name):
     # x fastest: node numbers increase with z, then y, then x
     key = np.lexsort((node_xyz[:, 0], node_xyz[:, 1], node_xyz[:, 2]))
     assert np.array_equal(key, np.arange(node_xyz.shape[0]))
+
+
+@pytest.mark.parametrize("name", sorted(CFGS))
+def test_nodal_force_oracle(name):
+    """fe_nodal_force against an element-by-element scatter loop; the total
+    force is kept; interior vertices collect 8 elements, edge mid-nodes 4."""
+    cfg = CFGS[name]
+    node_xyz, conn = G.hex20_box_mesh(cfg)
+    nn, ne = node_xyz.shape[0], conn.shape[0]
+    Fe = np.random.default_rng(5).normal(size=(ne, 3, 20))
+    F = O.fe_nodal_force(conn, Fe, nn)
+    ref = np.zeros((nn, 3))
+    for e in range(ne):
+        for l in range(20):
+            ref[conn[e, l]] += Fe[e, :, l]
+    assert np.array_equal(F, ref)
+    assert np.allclose(F.sum(axis=0), Fe.sum(axis=(0, 2)), rtol=1e-12, atol=1e-12)
+    counts = np.bincount(conn.ravel(), minlength=nn)
+    assert set(np.unique(counts)) <= {1, 2, 4, 8}
+    lo = np.array(cfg["box_min"], float)
+    hi = np.array(cfg["box_max"], float)
+    inner = np.all((node_xyz > lo + 1e-9) & (node_xyz < hi - 1e-9), axis=1)
+    vert = np.isin(np.arange(nn), conn[:, :8].ravel())
+    assert np.all(counts[inner & vert] == 8) and np.all(counts[inner & ~vert] == 4)
