@@ -43,7 +43,7 @@ EXPORTS = [
     "ggem_get_elem_ids", "ggem_neighbor_pairs", "ggem_get_neighbor_csr",
     "ggem_get_elem_point_csr", "ggem_local_velocity_beads", "ggem_set_noise", "ggem_global_self_exclusion",
     "ggem_combine_velocity", "ggem_local_velocity_points", "ggem_set_fe_mesh",
-    "ggem_global_velocity_beads", "ggem_combine_velocity_fe", "ggem_spread_nodes", "ggem_spread", "ggem_mobility_apply",
+    "ggem_global_velocity_beads", "ggem_combine_velocity_fe", "ggem_spread_nodes", "ggem_fix_forces", "ggem_spread", "ggem_mobility_apply",
     "ggem_cheb_brownian", "ggem_set_graph", "ggem_lanczos_bounds", "ggem_create_batch",
     "ggem_batch_set_particles", "ggem_batch_cheb_brownian", "ggem_batch_lanczos_bounds",
     "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
@@ -75,6 +75,12 @@ class Consts(C.Structure):
         "ksi", "ksi2", "two_ksi_sqrt_pi", "coeff_2", "search_radius_e", "search_radius_p")] + [
         ("h", C.c_double * 3), ("L", C.c_double * 3),
         ("stencil_rows", C.c_int32), ("stencil_cells", C.c_int32)]
+
+
+class FixParams(C.Structure):
+    _fields_ = [("chain_beads", C.c_int32), ("walls", C.c_int32), ("fene_k", C.c_double),
+                ("fene_bmax", C.c_double), ("lj_eps", C.c_double), ("lj_sigma", C.c_double),
+                ("lj_rcut", C.c_double)]
 
 
 class ChebInfo(C.Structure):
@@ -121,6 +127,7 @@ def lib():
             "ggem_global_velocity_beads": [P, P, P],
             "ggem_combine_velocity_fe": [P, P, P],
             "ggem_spread_nodes": [P, I32, P, P],
+            "ggem_fix_forces": [P, P, P],
             "ggem_spread": [P, I32, I32, P, P, P],
             "ggem_mobility_apply": [P, P, P],
             "ggem_set_noise": [P, I32, P],
@@ -390,6 +397,14 @@ class Context:
         U = np.zeros((self.n, 3)) if out else None
         self._c(lib().ggem_combine_velocity_fe(self._h, _p(nu), _p(U)))
         return U
+
+    def fix_forces(self, chain_beads=0, k=30.0, b_max=1.5, walls=False, eps=1.0, sigma=1.0,
+                   rcut=2.0 ** (1.0 / 6.0), out=True):
+        """FENE chains + LJ slit walls from the current positions; they become the context's forces."""
+        fp = FixParams(int(chain_beads), int(bool(walls)), k, b_max, eps, sigma, rcut)
+        f = np.zeros((self.n, 3)) if out else None
+        self._c(lib().ggem_fix_forces(self._h, C.byref(fp), _p(f)))
+        return f
 
     def spread_nodes(self, nqp_1d, IF, n_nodes):
         """Spreading summed into one force per node of the velocity mesh: [n_nodes, 3]."""

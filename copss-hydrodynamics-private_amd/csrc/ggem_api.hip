@@ -1033,6 +1033,22 @@ int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U) {
   return GGEM_OK;
 }
 
+int ggem_fix_forces(ggem_ctx* ctx, const ggem_fix_params* fp, double* f) {
+  if (!ctx || !fp) return fail(ctx, GGEM_ERR_INVALID_ARG, "null argument");
+  if (ctx->nsys > 1) return fail(ctx, GGEM_ERR_INVALID_ARG, "ggem_fix_forces is not available for a batched context");
+  if (fp->chain_beads > 1 && (ctx->n % fp->chain_beads != 0 || !(fp->fene_bmax > 0)))
+    return fail(ctx, GGEM_ERR_INVALID_ARG, "chains: n must be a multiple of chain_beads and fene_bmax > 0");
+  if (fp->walls && !(fp->lj_sigma > 0 && fp->lj_rcut >= 0))
+    return fail(ctx, GGEM_ERR_INVALID_ARG, "walls: lj_sigma > 0 and lj_rcut >= 0 required");
+  if (ctx->n == 0) return GGEM_OK;
+  hipError_t e = launch_fix_forces(ctx, fp->chain_beads, fp->fene_k, fp->fene_bmax, fp->walls, fp->lj_eps,
+                                   fp->lj_sigma, fp->lj_rcut);
+  if (e != hipSuccess) return hip_fail(ctx, e, "fix_forces");
+  if (ctx->built && (e = launch_gather_forces(ctx)) != hipSuccess) return hip_fail(ctx, e, "gather_forces");
+  if (f) return d2h(ctx, f, ctx->f_in, sizeof(double) * 3 * ctx->n);
+  return GGEM_OK;
+}
+
 int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F_node) {
   int rc = check_ready(ctx);
   if (rc) return rc;

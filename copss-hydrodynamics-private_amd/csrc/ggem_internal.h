@@ -312,6 +312,10 @@ hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* e
 hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out);
 // out[3 node + d] = sum over the node's incidences (e, l) of Fe[e][l][d] (n_u = 20)
 hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out);
+// f_in = FENE forces along chains of `chain` consecutive particles (chain > 1)
+// + LJ forces of the walls of the non-periodic axes (walls != 0), from x_in
+hipError_t launch_fix_forces(ggem_ctx* ctx, int32_t chain, double k, double b, int32_t walls, double eps,
+                             double sigma, double rcut);
 
 // ---- launchers (kernels_pair.hip) ----
 // A vector operand of the pair sum: FP64 SoA by slot for the element-wise

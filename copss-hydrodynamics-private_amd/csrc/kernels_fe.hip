@@ -158,3 +158,104 @@ hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double*
 }
 
 }  // namespace ggem
+
+// ---- force fields: FENE bead-spring chains and LJ slit walls -------------
+//
+// Reference: FixBase::spring_force_fene (fix_base.C:78-97) between
+// consecutive beads of a chain, F = c1 (r/Ls) / (1 - (r/Ls)^2) r_hat with
+// c1 = k Ls, Ls = b_max, the bond vector by the minimum image on periodic
+// axes; and FixPointSlitWallLJCut's wall force, FixBase::lj_force
+// (fix_base.C:168-186) of the particle -> wall vector within rcut, for the
+// two walls of every non-periodic axis.  One thread per particle in
+// particle order: f = F(bond to the next bead) - F(bond from the previous
+// bead), then the walls axis by axis, low wall first.
+namespace ggem {
+
+namespace {
+
+__device__ __forceinline__ void fene_bond(const Grid& g, const double* x, int64_t i, double k, double b,
+                                          double* fb) {
+  double r[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = x[3 * (i + 1) + d] - x[3 * i + d];
+    if (g.periodic[d]) {
+      if (v <= -g.half_L[d]) v += g.L[d];
+      if (v > g.half_L[d]) v -= g.L[d];
+    }
+    r[d] = v;
+  }
+  const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  const double ratio = R / b;
+  double val0 = 1.0 - ratio * ratio;
+  constexpr double kTiny = 1e-10;
+  if (fabs(val0) < kTiny) val0 = double((val0 > 0.0) - (val0 < 0.0)) * kTiny * kTiny;
+  const double c = (k * b) * ratio / val0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fb[d] = c * r[d] / R;
+}
+
+struct WallPos {
+  double w[3][2];  // the box faces as given (box_min, box_max)
+};
+
+__global__ void k_fix_forces(Grid g, WallPos wp, int64_t n, const double* __restrict__ x, int32_t chain,
+                             double k, double b, int32_t walls, double eps, double sigma, double rcut,
+                             double* __restrict__ f) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fi[3] = {0.0, 0.0, 0.0};
+  if (chain > 1) {
+    const int64_t j = i % chain;
+    double fb[3];
+    if (j + 1 < chain) {
+      fene_bond(g, x, i, k, b, fb);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fi[d] += fb[d];
+    }
+    if (j > 0) {
+      fene_bond(g, x, i - 1, k, b, fb);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fi[d] -= fb[d];
+    }
+  }
+  if (walls) {
+    const double s2 = sigma * sigma;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (g.periodic[d]) continue;
+      const double xd = x[3 * i + d];
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const double rw = wp.w[d][w] - xd;
+        if (fabs(rw) <= rcut) {
+          const double r2 = rw * rw;
+          const double q = s2 / r2;
+          const double sr6 = q * q * q;
+          fi[d] += -24.0 * eps * (2.0 * sr6 * sr6 - sr6) / r2 * rw;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) f[3 * i + d] = fi[d];
+}
+
+}  // namespace
+
+hipError_t launch_fix_forces(ggem_ctx* ctx, int32_t chain, double k, double b, int32_t walls, double eps,
+                             double sigma, double rcut) {
+  if (ctx->n == 0) return hipSuccess;
+  WallPos wp;
+  for (int d = 0; d < 3; ++d) {
+    wp.w[d][0] = ctx->p.box_min[d];
+    wp.w[d][1] = ctx->p.box_max[d];
+  }
+  ProfScope ps(ctx, "fix_forces");
+  hipLaunchKernelGGL(k_fix_forces, dim3(unsigned((ctx->n + kFeBlock - 1) / kFeBlock)), dim3(kFeBlock), 0,
+                     ctx->stream, ctx->grid, wp, ctx->n, ctx->x_in, chain, k, b, walls, eps, sigma, rcut,
+                     ctx->f_in);
+  return hipGetLastError();
+}
+
+}  // namespace ggem

@@ -183,6 +183,23 @@ int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F
 int ggem_spread(ggem_ctx* ctx, int32_t nqp_1d, int32_t n_u, const double* IF, double* rho,
                 double* Fe);
 
+/* ---- force fields ----------------------------------------------------- */
+/* Bead-spring chains and slit walls, the fixes of the polymer-slit system. */
+typedef struct ggem_fix_params {
+  int32_t chain_beads;  /* beads per chain, consecutive particle ids; <= 1: no springs */
+  int32_t walls;        /* != 0: LJ walls at box_min / box_max of each non-periodic axis */
+  double fene_k;        /* FENE: F = k Ls (r/Ls) / (1 - (r/Ls)^2) r_hat, Ls = fene_bmax */
+  double fene_bmax;
+  double lj_eps, lj_sigma, lj_rcut;  /* wall force within |wall - x| <= lj_rcut */
+} ggem_fix_params;
+/* Forces of the particles from their current positions: FENE springs between
+ * consecutive beads of each chain (FixBase::spring_force_fene,
+ * fix_base.C:78-97; bond vectors by the minimum image) plus the LJ wall force
+ * of FixPointSlitWallLJCut (FixBase::lj_force, fix_base.C:168-186).  The
+ * result replaces the context's forces (as ggem_set_forces would) and is
+ * copied to f (3n) unless f is NULL.  Single-system contexts only. */
+int ggem_fix_forces(ggem_ctx* ctx, const ggem_fix_params* fp, double* f);
+
 /* ---- (d) Brownian step ------------------------------------------------ */
 /* Mobility operator u = M f: _MatMult_Stokes (brownian_system.h:54-59,
  * brownian_system.C:44-140) with the host FEM part U_glob left out, i.e.
