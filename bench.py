@@ -212,23 +212,28 @@ def bench_slit(G, W, args):
     LJ slit walls at z = 0, 100), 160x160x100 box periodic in x, y, 80x80x50
     elements; Fixman mid-point steps (copss.C:1565-1745) with the device
     mobility, Chebyshev order 30 in the reference's convergence mode, FP32-
-    compensated pair sums.  Forces and position updates on the host."""
+    compensated pair sums.  Chain and wall forces on the device
+    (ggem_fix_forces); position updates on the host.  The same steps with
+    the forces on the host are reported beside."""
     import fixman as F
-    cfg, x, _ = W.make_inputs("slit25k")
+    cfg, x0, _ = W.make_inputs("slit25k")
     cfg["precision"] = G.FP32C
-    forces = F.SlitPolymerForces(cfg, 400, 64)
-    ctx = G.Context(cfg)
-    integ = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), forces, dt=1e-3, cheb_order=30)
-    x = integ.step(x)  # warm-up, eigenvalue bounds
-    t0 = time.perf_counter()
-    for _ in range(args.slit_steps):
-        x = integ.step(x)
-    dt_s = time.perf_counter() - t0
-    ctx.close()
-    return {"beads": int(len(x)), "steps": args.slit_steps, "ms_per_step": dt_s * 1e3 / args.slit_steps,
-            "brownian_steps_per_s": args.slit_steps / dt_s, "cheb_retries": integ.stats["cheb_retries"],
-            "cheb_failures": integ.stats["cheb_failures"],
-            "note": "host forces and position update; includes host<->device copies per step"}
+    res = {}
+    for name, forces in (("device", F.DeviceSlitForces(64)), ("host", F.SlitPolymerForces(cfg, 400, 64))):
+        ctx = G.Context(cfg)
+        integ = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), forces, dt=1e-3, cheb_order=30)
+        x = integ.step(x0)  # warm-up, eigenvalue bounds
+        t0 = time.perf_counter()
+        for _ in range(args.slit_steps):
+            x = integ.step(x)
+        res[name] = (time.perf_counter() - t0, integ.stats)
+        ctx.close()
+    dt_s, stats = res["device"]
+    return {"beads": int(len(x0)), "steps": args.slit_steps, "ms_per_step": dt_s * 1e3 / args.slit_steps,
+            "brownian_steps_per_s": args.slit_steps / dt_s, "cheb_retries": stats["cheb_retries"],
+            "cheb_failures": stats["cheb_failures"],
+            "host_forces_ms_per_step": res["host"][0] * 1e3 / args.slit_steps,
+            "note": "device chain/wall forces, host position update; includes host<->device copies per step"}
 
 
 def main():

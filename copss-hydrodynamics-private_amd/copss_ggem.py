@@ -430,8 +430,9 @@ class Context:
         return rho, Fe
 
     def mobility_apply(self, f):
-        f = _f64(f, (-1, 3))
-        u = np.zeros_like(f)
+        """u = M f; f None: the context's current forces."""
+        f = None if f is None else _f64(f, (-1, 3))
+        u = np.zeros((self.n, 3))
         self._c(lib().ggem_mobility_apply(self._h, _p(f), _p(u)))
         return u
 

@@ -17,7 +17,9 @@ is any object with reinit(x, f), mobility_apply(f), cheb(N, lo, hi, tol,
 dw) -> (y, disp, converged) and bounds() -> (lmin, lmax); DeviceMobility
 wraps a copss_ggem.Context.  Forces come from a callable F(x) -> f (the
 FENE chains and slit walls of SlitPolymerForces for the polymer-slit
-workload; they are inputs of the GGEM path, computed on the host).
+workload, computed on the host), or from DeviceSlitForces, which forms the
+same forces on the device from the positions already there
+(ggem_fix_forces) so that they never cross to the host.
 """
 import numpy as np
 
@@ -35,6 +37,11 @@ class DeviceMobility:
 
     def reinit(self, x, f):
         self.ctx.reinit(x, f)
+
+    def reinit_fixes(self, x, fixes):
+        """Positions, neighbour build, then the forces of `fixes` on the device."""
+        self.ctx.reinit(x, None)
+        self.ctx.fix_forces(out=False, **fixes.params)
 
     def mobility_apply(self, f):
         return self.ctx.mobility_apply(f)
@@ -93,6 +100,15 @@ class SlitPolymerForces:
         return f
 
 
+class DeviceSlitForces:
+    """The forces of SlitPolymerForces (FENE chains of n_beads consecutive
+    beads, LJ walls on the non-periodic axes) formed by ggem_fix_forces."""
+    on_device = True
+
+    def __init__(self, n_beads, k=30.0, b_max=1.5, eps=1.0, sigma=1.0, rcut=2.0 ** (1.0 / 6.0)):
+        self.params = dict(chain_beads=n_beads, k=k, b_max=b_max, walls=True, eps=eps, sigma=sigma, rcut=rcut)
+
+
 def wrap(x, cfg):
     lo = np.array(cfg["box_min"], float)
     L = np.array(cfg["box_max"], float) - lo
@@ -117,8 +133,12 @@ class FixmanIntegrator:
     def step(self, x0, dw=None):
         x0 = np.asarray(x0, float).reshape(-1, 3)
         n = len(x0)
-        f0 = self.forces(x0)
-        self.mob.reinit(x0, f0)
+        dev = getattr(self.forces, "on_device", False)
+        f0 = None if dev else self.forces(x0)
+        if dev:
+            self.mob.reinit_fixes(x0, self.forces)
+        else:
+            self.mob.reinit(x0, f0)
         u0 = self.mob.mobility_apply(f0)
         if dw is None:
             dw = G.reference_noise(3 * n, self.seed, self.dt, self.step_no).reshape(n, 3)
@@ -134,8 +154,11 @@ class FixmanIntegrator:
         else:
             self.stats["cheb_failures"] += 1
         xm = wrap(x0 + 0.5 * self.dt * u0 + 0.5 * d0, self.cfg)
-        fm = self.forces(xm)
-        self.mob.reinit(xm, fm)
+        fm = None if dev else self.forces(xm)
+        if dev:
+            self.mob.reinit_fixes(xm, self.forces)
+        else:
+            self.mob.reinit(xm, fm)
         um = self.mob.mobility_apply(fm)
         dm = self.mob.mobility_apply(y)
         x1 = wrap(x0 + self.dt * um + dm, self.cfg)

@@ -96,3 +96,26 @@ def test_fixman_slit25k_steps():
         assert integ.stats["cheb_failures"] == 0
     finally:
         ctx.close()
+
+
+def test_device_forces_trajectory_matches_host_forces():
+    """Fixman steps with the chain and wall forces formed on the device
+    (ggem_fix_forces) follow the steps with the host forces."""
+    cfg, x0 = small_slit()
+    ctx_h, ctx_d = G.Context(cfg), G.Context(cfg)
+    try:
+        ctx_h.reinit(x0, F.SlitPolymerForces(cfg, 8, 16)(x0))
+        bounds = ctx_h.lanczos_bounds(iters=40, seed=11)
+        host = F.FixmanIntegrator(cfg, FixedBoundsDevice(ctx_h, bounds), F.SlitPolymerForces(cfg, 8, 16),
+                                  dt=1e-3, cheb_order=30)
+        dev = F.FixmanIntegrator(cfg, FixedBoundsDevice(ctx_d, bounds), F.DeviceSlitForces(16),
+                                 dt=1e-3, cheb_order=30)
+        xh, xd = x0.copy(), x0.copy()
+        for step in range(3):
+            xh = host.step(xh)
+            xd = dev.step(xd)
+            assert np.abs(xd - xh).max() < 1e-10, f"step {step}"
+        assert np.abs(xd - x0).max() > 1e-3
+    finally:
+        ctx_h.close()
+        ctx_d.close()
