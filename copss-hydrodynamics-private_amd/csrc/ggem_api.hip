@@ -409,6 +409,7 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->fe_u);
   f(ctx->fe_inc_off);
   f(ctx->fe_inc);
+  f(ctx->fe_node_order);
   if (ctx->par_h) (void)hipHostFree(ctx->par_h);
   ctx->par_h = nullptr;
   if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
@@ -965,11 +966,12 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
     }
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   for (void* p : {static_cast<void*>(ctx->fe_conn), static_cast<void*>(ctx->fe_inc_off),
-                  static_cast<void*>(ctx->fe_inc)})
+                  static_cast<void*>(ctx->fe_inc), static_cast<void*>(ctx->fe_node_order)})
     if (p) (void)hipFree(p);
   ctx->fe_conn = nullptr;
   ctx->fe_inc_off = nullptr;
   ctx->fe_inc = nullptr;
+  ctx->fe_node_order = nullptr;
   ctx->fe_nodes = -1;
   // incidences of each node, (element, local node) ascending: the order in
   // which the nodal force sums the element vectors
@@ -981,13 +983,32 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
     std::vector<int64_t> pos(off.begin(), off.end() - 1);
     for (int64_t k = 0; k < 20 * ne; ++k) inc[pos[elem_nodes[k]]++] = int32_t(k);
   }
+  // nodes in the order of the 8 x 8 x 8-element brick of their first
+  // element, so that the gather of the nodal force finds a brick's element
+  // vectors in L2 for all of its nodes
+  std::vector<int32_t> order(n_nodes);
+  {
+    const int nx = ctx->grid.n[0], ny = ctx->grid.n[1];
+    const int64_t bx = (nx + 7) / 8, by = (ny + 7) / 8, bz = (ctx->grid.n[2] + 7) / 8;
+    std::vector<int64_t> key(n_nodes), start(bx * by * bz + 1, 0);
+    for (int64_t v = 0; v < n_nodes; ++v) {
+      const int64_t e = off[v] < off[v + 1] ? inc[off[v]] / 20 : 0;
+      const int64_t i = e % nx, j = (e / nx) % ny, k = e / (int64_t(nx) * ny);
+      key[v] = (i >> 3) + bx * ((j >> 3) + by * (k >> 3));
+      ++start[key[v] + 1];
+    }
+    for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+    for (int64_t v = 0; v < n_nodes; ++v) order[start[key[v]]++] = int32_t(v);
+  }
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_conn), sizeof(int32_t) * 20 * ne));
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_inc), sizeof(int32_t) * 20 * ne));
+  GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_node_order), sizeof(int32_t) * n_nodes));
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_inc_off), sizeof(int64_t) * (n_nodes + 1)));
   int rc = h2d(ctx, ctx->fe_conn, elem_nodes, sizeof(int32_t) * 20 * ne);
   if (rc) return rc;
   if ((rc = h2d(ctx, ctx->fe_inc, inc.data(), sizeof(int32_t) * 20 * ne))) return rc;
   if ((rc = h2d(ctx, ctx->fe_inc_off, off.data(), sizeof(int64_t) * (n_nodes + 1)))) return rc;
+  if ((rc = h2d(ctx, ctx->fe_node_order, order.data(), sizeof(int32_t) * n_nodes))) return rc;
   if ((rc = ensure(ctx, &ctx->fe_u, &ctx->fe_u_cap, size_t(3 * n_nodes)))) return rc;
   ctx->fe_nodes = n_nodes;
   return GGEM_OK;

@@ -245,6 +245,7 @@ struct ggem_ctx {
   size_t fe_u_cap = 0;
   int64_t* fe_inc_off = nullptr;  // [fe_nodes + 1] incidences of each node
   int32_t* fe_inc = nullptr;      // [20 n_elem] 20 e + l, by node, (e, l) ascending
+  int32_t* fe_node_order = nullptr;  // [fe_nodes] nodes by the brick of their first element
 
   // generic scratch (grown on demand)
   void* scratch = nullptr;

@@ -115,13 +115,18 @@ __global__ __launch_bounds__(kFeBlock) void k_interp_hex20(Grid g, int64_t n, co
 // Nodal force: each node sums the element vectors Fe[e][l][d] (the
 // spreading's node-major layout: one 24-byte read per incidence) of its
 // incidences (e, l) in ascending order.  A gather, so no atomics and the
-// same sums on every run.
-__global__ __launch_bounds__(kFeBlock) void k_gather_nodes(int64_t nn, const int64_t* __restrict__ off,
+// same sums on every run.  Threads take the nodes brick by brick of their
+// first element (order), which keeps a brick's element vectors in L2 while
+// its nodes are summed; in plain node order each element was fetched once
+// per plane of nodes touching it.
+__global__ __launch_bounds__(kFeBlock) void k_gather_nodes(int64_t nn, const int32_t* __restrict__ order,
+                                                           const int64_t* __restrict__ off,
                                                            const int32_t* __restrict__ inc,
                                                            const double* __restrict__ Fe,
                                                            double* __restrict__ out) {
-  const int64_t v = xcd_block(blockIdx.x, gridDim.x) * kFeBlock + threadIdx.x;
-  if (v >= nn) return;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kFeBlock + threadIdx.x;
+  if (t >= nn) return;
+  const int64_t v = order[t];
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
   const int64_t k1 = off[v + 1];
   for (int64_t k = off[v]; k < k1; ++k) {
@@ -144,7 +149,7 @@ hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out) {
   if (nn <= 0) return hipSuccess;
   ProfScope ps(ctx, "gather_nodes");
   hipLaunchKernelGGL(k_gather_nodes, dim3(unsigned((nn + kFeBlock - 1) / kFeBlock)), dim3(kFeBlock), 0,
-                     ctx->stream, nn, ctx->fe_inc_off, ctx->fe_inc, Fe, out);
+                     ctx->stream, nn, ctx->fe_node_order, ctx->fe_inc_off, ctx->fe_inc, Fe, out);
   return hipGetLastError();
 }
 
