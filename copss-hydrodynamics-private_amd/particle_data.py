@@ -1,0 +1,69 @@
+"""Reader of the reference's particle input file (point_particle_data.in,
+a LAMMPS-style data file), PolymerChain::read_particles_data
+(polymer_chain.C:61-225): the header counts (atoms, bonds, atom types,
+bond types), the Masses, Atoms and Bonds sections.  Ids in the file are
+1-based and come back 0-based, as the reference stores them.  Where the
+reference stops with libmesh_error() this raises ValueError.
+
+The positions and constant forces feed Context.set_particles; chains of
+consecutive bead ids give chain_beads for ggem_fix_forces."""
+import numpy as np
+
+
+def _section(lines, pos, name):
+    while pos < len(lines) and lines[pos].strip() != name:
+        pos += 1
+    if pos == len(lines):
+        raise ValueError("%s are not defined correctly in the data file" % name)
+    return pos + 2  # the section title, then one empty line
+
+
+def read_particle_data(path):
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    head = " ".join(lines[1:]).split()  # line 0 is the title
+    try:
+        n_beads, w1 = int(head[0]), head[1]
+        n_bonds, w2 = int(head[2]), head[3]
+        n_types, w3, w4 = int(head[4]), head[5], head[6]
+        n_btypes, w5, w6 = int(head[7]), head[8], head[9]
+    except (IndexError, ValueError):
+        raise ValueError("malformed header of the data file")
+    if w1 != "atoms":
+        raise ValueError("# of atoms is not defined in the data file")
+    if w2 != "bonds":
+        raise ValueError("# of bonds is not defined in the data file")
+    if (w3, w4) != ("atom", "types"):
+        raise ValueError("# of atom types is not defined in the data file")
+    if (w5, w6) != ("bond", "types"):
+        raise ValueError("# of bond types is not defined in the data file")
+    box = np.zeros((3, 2))
+    for ln in lines[1:]:
+        t = ln.split()
+        if len(t) == 4 and t[2:] in (["xlo", "xhi"], ["ylo", "yhi"], ["zlo", "zhi"]):
+            box["xyz".index(t[2][0])] = [float(t[0]), float(t[1])]
+    p = _section(lines, 0, "Masses")
+    tok = " ".join(lines[p:]).split()
+    masses = np.array([float(tok[2 * i + 1]) for i in range(n_types)])
+    p = _section(lines, p, "Atoms")
+    tok = " ".join(lines[p:]).split()
+    if len(tok) < 13 * n_beads:
+        raise ValueError("too few atom records in the data file")
+    rec = np.array(tok[:13 * n_beads], dtype=float).reshape(n_beads, 13)
+    out = {
+        "box": box, "masses": masses,
+        "bead_id": rec[:, 0].astype(np.int64) - 1, "chain_id": rec[:, 1].astype(np.int64) - 1,
+        "bead_type": rec[:, 2].astype(np.int64), "xyz": rec[:, 3:6].copy(),
+        "rot_vec": rec[:, 6:9].copy(),  # the file holds (a, b, c); the reference's theta stays 0
+        "constant_force": rec[:, 9:12].copy(), "charge": rec[:, 12].copy(),
+    }
+    p = _section(lines, p, "Bonds")
+    tok = " ".join(lines[p:]).split()
+    if len(tok) < 4 * n_bonds:
+        raise ValueError("too few bond records in the data file")
+    b = np.array(tok[:4 * n_bonds], dtype=np.int64).reshape(n_bonds, 4)
+    out["bonds"] = b[:, 1:] - 1  # bond type, bead 1, bead 2
+    n_chains = int(rec[-1, 1]) if n_beads else 0  # the last bead's chain, as the reference takes it
+    out["n_chains"] = n_chains
+    out["beads_per_chain"] = np.bincount(out["chain_id"], minlength=n_chains)
+    return out
