@@ -6,7 +6,9 @@ bond types), the Masses, Atoms and Bonds sections.  Ids in the file are
 reference stops with libmesh_error() this raises ValueError.
 
 The positions and constant forces feed Context.set_particles; chains of
-consecutive bead ids give chain_beads for ggem_fix_forces."""
+consecutive bead ids give chain_beads for ggem_fix_forces.  The restart
+files (out.time, out.eigenvalue, vector_RIN/ROUT.dat) are read and written
+at the end of this module."""
 import numpy as np
 
 
@@ -67,3 +69,59 @@ def read_particle_data(path):
     out["n_chains"] = n_chains
     out["beads_per_chain"] = np.bincount(out["chain_id"], minlength=n_chains)
     return out
+
+
+# ---- restart files ---------------------------------------------------------
+def _last_line_numbers(path, count, what):
+    """The first `count` numbers of the last non-empty line, as
+    Copss::read_restart_time / read_restart_eigenvalue take them
+    (copss.C:909-968: seek back from the end to the last line break)."""
+    try:
+        with open(path) as fh:
+            lines = [ln for ln in fh.read().splitlines() if ln.strip()]
+    except OSError:
+        raise ValueError("can NOT read the %s output: %s" % (what, path))
+    if not lines:
+        raise ValueError("empty %s output: %s" % (what, path))
+    tok = lines[-1].split()
+    if len(tok) < count:
+        raise ValueError("last line of %s holds fewer than %d numbers" % (path, count))
+    return tok[:count]
+
+
+def read_restart_time(path="out.time"):
+    """(restart_step, o_step, real_time) from the last line of out.time."""
+    a, b, c = _last_line_numbers(path, 3, "time")
+    return int(a), int(b), float(c)
+
+
+def read_restart_eigenvalue(path="out.eigenvalue"):
+    """(eig_min, eig_max) from the last line of out.eigenvalue."""
+    a, b = _last_line_numbers(path, 2, "eigenvalue")
+    return float(a), float(b)
+
+
+PETSC_VEC_CLASSID = 1211214  # VEC_FILE_CLASSID of PETSc's binary viewer
+
+
+def read_petsc_vec(path):
+    """A Vec written by VecView to a PETSc binary viewer (vector_RIN.dat,
+    vector_ROUT.dat, copss.C:1392-1413): big-endian int32 class id and
+    length (32-bit PetscInt), then the values as big-endian float64."""
+    raw = open(path, "rb").read()
+    if len(raw) < 8:
+        raise ValueError("%s is too short for a PETSc Vec" % path)
+    cid, n = np.frombuffer(raw[:8], dtype=">i4")
+    if cid != PETSC_VEC_CLASSID:
+        raise ValueError("%s is not a PETSc Vec (class id %d)" % (path, cid))
+    if len(raw) < 8 + 8 * n:
+        raise ValueError("%s holds fewer than %d values" % (path, n))
+    return np.frombuffer(raw[8:8 + 8 * n], dtype=">f8").astype(np.float64)
+
+
+def write_petsc_vec(path, v):
+    """The same layout as read_petsc_vec, for VecLoad on restart."""
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    with open(path, "wb") as fh:
+        fh.write(np.array([PETSC_VEC_CLASSID, v.size], dtype=">i4").tobytes())
+        fh.write(v.astype(">f8").tobytes())

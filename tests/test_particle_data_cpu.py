@@ -1,7 +1,10 @@
 """The reader of the reference's particle input file
 (PolymerChain::read_particles_data, polymer_chain.C:61-225) on the
 reference's own polymer-chain example (tests/golden/
-polymer_chain_point_particle_data.in), and its errors on broken headers."""
+polymer_chain_point_particle_data.in) and its errors on broken headers;
+the restart files: last-line reads of out.time and out.eigenvalue
+(copss.C:909-968) and the PETSc binary Vec of vector_RIN/ROUT.dat
+(copss.C:1392-1413) byte by byte."""
 import os
 
 import numpy as np
@@ -44,3 +47,28 @@ def test_broken_files_raise(tmp_path, bad, old, new):
     p.write_text(txt)
     with pytest.raises(ValueError):
         P.read_particle_data(str(p))
+
+
+def test_restart_time_and_eigenvalue_last_line(tmp_path):
+    t = tmp_path / "out.time"
+    t.write_text("0 0 0.0\n10 1 0.5\n20 2 1.25\n")
+    assert P.read_restart_time(str(t)) == (20, 2, 1.25)
+    e = tmp_path / "out.eigenvalue"
+    e.write_text("0.1 2.0\n0.125 3.5\n\n")
+    assert P.read_restart_eigenvalue(str(e)) == (0.125, 3.5)
+    with pytest.raises(ValueError):
+        P.read_restart_time(str(tmp_path / "missing"))
+
+
+def test_petsc_vec_binary_layout(tmp_path):
+    p = tmp_path / "vector_ROUT.dat"
+    v = np.array([1.0, -2.5, 3e-300])
+    P.write_petsc_vec(str(p), v)
+    raw = p.read_bytes()
+    # class id 1211214 = 0x00127B4E and the length, big-endian, then big-endian doubles
+    assert raw[:8] == bytes([0x00, 0x12, 0x7B, 0x4E, 0, 0, 0, 3])
+    assert raw[8:16] == bytes([0x3F, 0xF0, 0, 0, 0, 0, 0, 0])  # 1.0
+    assert np.array_equal(P.read_petsc_vec(str(p)), v)
+    p.write_bytes(raw[:20])
+    with pytest.raises(ValueError):
+        P.read_petsc_vec(str(p))
