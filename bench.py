@@ -4,18 +4,20 @@ One step = one pass of the hot path over the 1M-bead workload (beads1m:
 1,000,000 beads, [0,500)^3 periodic, 128^3 elements, alpha = 0.5, cutoff
 8a): (a) neighbour build, (b) local pair sum, (c) Gaussian spreading onto
 27 Gauss points per element with the HEX20 IF contraction, (d) a Chebyshev
-Brownian step of order 40 on the device mobility (fixed order) plus the
-final M y.  Inputs (positions, forces, noise) are resident in HBM before the
-timed region.
+Brownian step of order 40 on the device mobility (fixed order) followed by
+the displacement M y (hi_ewald), kept on the device.  Inputs (positions,
+forces, noise) are resident in HBM before the timed region.
 
 Prints one JSON line.  value = pair interactions evaluated per second over
 the whole FP32-compensated step (pairs x (1 local sum + N recurrence
-applications) / wall time per step, build and spreading included); the
-pair kernel's own rate, the Brownian steps/s, the FP64 form's figures and
-the errors against FP64 ride along.  The timed steps run as a user runs
-them; the kernel times and the step breakdown come from the same number of
-steps run again with every launch bracketed by HIP events on the library's
-stream (profiled_ms_per_step).
+applications + 1 for M y) / wall time per step, build and spreading
+included); the pair kernel's own rate, the Brownian steps/s, the FP64
+form's figures, the same step at the other xi of the configurations
+(1/(a sqrt(pi))) and the errors against FP64 and against the FP64 oracle
+ride along.  The timed steps run as a user runs them; the kernel times
+and the step breakdown come from the same number of steps run again with
+every launch bracketed by HIP events on the library's stream
+(profiled_ms_per_step).
 """
 import argparse
 import json
@@ -52,16 +54,14 @@ def parse():
     return ap.parse_args()
 
 
-# Algorithmic HBM bytes of one FP32C Chebyshev application (k_apply_box,
-# one right-hand side): per ordered pair the 2-byte neighbour record number
-# and the 8-byte pair tensor (A, Bx); per slot the operand and position
-# records read once
-# (16 + 16), x, x_{k-1} (24 each), y read and written on one step in three
-# (48 / 3), v and its record written (24 + 16), the list length (4).  The
-# staged neighbour records are read once per brick box (~3.4x per slot at
-# 1M beads): traffic above the algorithmic bytes.
-B_PAIR_APPLY32 = 2 + 8
-B_SLOT_APPLY = 16 + 16 + 24 + 24 + 48 // 3 + 24 + 16 + 4
+# Algorithmic HBM bytes per bead of one Chebyshev application (the
+# dominant kernel, k_apply_box in FP32C): the local pair sum in FP32C reads
+# the position (FP64 kept in cells: 24 B) and the operand (FP32 gather
+# record: 12 B) and writes the FP64 result (12 B of the 24 are charged to the
+# recurrence below): 48 B; the recurrence's fused minimum reads M x_k, x_k,
+# x_{k-1} and y and writes x_{k+1} and y, 3 doubles each: 144 B.  Per bead
+# per application: 192 B, independent of the neighbour count.
+B_BEAD_APPLY = 48 + 144
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "hbm_traffic_beads1m.txt")
 
 
@@ -83,12 +83,13 @@ def rel(a, b):
 
 
 def run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, steps, profile):
-    """Run `steps` full steps; returns wall ms per step."""
+    """Run `steps` full steps; returns wall ms per step.  The Brownian step
+    keeps y and the displacement M y on the device (GGEM_CHEB_KEEP)."""
     def one():
         ctx.build_neighbors()
         ctx.local_velocity_beads(out=False)
         ctx.spread(nqp_1d, IF, want_rho=False, want_fe=False)
-        ctx.cheb_brownian(order, lo, hi, None, mode=G.CHEB_FIXED, want_y=False, want_disp=False)
+        ctx.cheb_brownian(order, lo, hi, None, mode=G.CHEB_FIXED, want_y=False, want_disp=False, keep=True)
     ctx.sync()
     if profile:
         ctx.profile_reset()
@@ -210,30 +211,169 @@ def bench_batch(G, W, args):
 def bench_slit(G, W, args):
     """Polymer-slit workload: 400 chains x 64 beads (FENE k=30, b_max=1.5a,
     LJ slit walls at z = 0, 100), 160x160x100 box periodic in x, y, 80x80x50
-    elements; Fixman mid-point steps (copss.C:1565-1745) with the device
+    elements; Fixman mid-point steps (copss.C:1565-1763) with the device
     mobility, Chebyshev order 30 in the reference's convergence mode, FP32-
-    compensated pair sums.  Chain and wall forces on the device
-    (ggem_fix_forces); position updates on the host.  The same steps with
-    the forces on the host are reported beside."""
+    compensated pair sums.  DeviceFixman: noise from the context's engine,
+    chain and wall forces, velocities, displacements and position updates
+    on the device.  Reported: ms per step over the first and the second half
+    of the run (equal when the per-step cost does not grow), the GPU's busy
+    share of a step (kernel time from event brackets over a few extra steps)
+    and the same steps through the host-array driver."""
     import fixman as F
     cfg, x0, _ = W.make_inputs("slit25k")
     cfg["precision"] = G.FP32C
+    ctx = G.Context(cfg)
+    dev = F.DeviceFixman(ctx, x0, F.DeviceSlitForces(64), dt=1e-3, cheb_order=30)
+    dev.step()  # warm-up, eigenvalue bounds
+    ctx.sync()
+    half = max(args.slit_steps // 2, 1)
+    t0 = time.perf_counter()
+    for _ in range(half):
+        dev.step()
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.slit_steps - half):
+        dev.step()
+    ctx.sync()
+    t2 = time.perf_counter()
+    nprof = 10
+    ctx.profile_reset()
+    ctx.profile(True)
+    t3 = time.perf_counter()
+    for _ in range(nprof):
+        dev.step()
+    ctx.sync()
+    t4 = time.perf_counter()
+    ctx.profile(False)
+    busy_ms, _ = ctx.profile_get("*")
+    stats = dict(dev.stats)
+    ctx.close()
+    # the host-array driver (positions, forces and noise cross each step)
+    ctx = G.Context(cfg)
+    host = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), F.SlitPolymerForces(cfg, 400, 64), dt=1e-3,
+                              cheb_order=30)
+    x = host.step(x0)
+    nh = max(args.slit_steps // 5, 1)
+    t5 = time.perf_counter()
+    for _ in range(nh):
+        x = host.step(x)
+    t6 = time.perf_counter()
+    ctx.close()
+    ms = (t2 - t0) * 1e3 / args.slit_steps
+    return {"beads": int(len(x0)), "steps": args.slit_steps, "ms_per_step": ms,
+            "ms_per_step_first_half": (t1 - t0) * 1e3 / half,
+            "ms_per_step_second_half": (t2 - t1) * 1e3 / max(args.slit_steps - half, 1),
+            "brownian_steps_per_s": 1e3 / ms, "gpu_busy_fraction": busy_ms / ((t4 - t3) * 1e3),
+            "cheb_retries": stats["cheb_retries"], "cheb_failures": stats["cheb_failures"],
+            "host_array_driver_ms_per_step": (t6 - t5) * 1e3 / nh,
+            "note": "reference-mode Chebyshev: host-side stopping decisions each term (one small copy each)"}
+
+
+def measure(G, ctx, args, IF, nqp_1d, order, lo, hi, dw, precisions, names):
+    """Timed steps per precision: eager, event-profiled (kernel times and the
+    step breakdown) and graph-replayed; plus one step's outputs."""
     res = {}
-    for name, forces in (("device", F.DeviceSlitForces(64)), ("host", F.SlitPolymerForces(cfg, 400, 64))):
-        ctx = G.Context(cfg)
-        integ = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), forces, dt=1e-3, cheb_order=30)
-        x = integ.step(x0)  # warm-up, eigenvalue bounds
-        t0 = time.perf_counter()
-        for _ in range(args.slit_steps):
-            x = integ.step(x)
-        res[name] = (time.perf_counter() - t0, integ.stats)
+    for prec in precisions:
+        ctx.set_precision(prec)
+        run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.warmup, False)
+        ms = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
+        ms_prof = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, True)
+        tag = "fp32c" if prec == G.FP32C else "fp64"
+        k_ms, k_n = ctx.profile_get("apply_" + tag)
+        s_ms, s_n = ctx.profile_get("spread_" + tag)
+        l_ms, l_n = ctx.profile_get("pair_beads_" + tag)
+        per_step = {}
+        for nm in names + ["tensors_" + tag, "pair_beads_" + tag, "spread_" + tag, "apply_" + tag]:
+            t_ms, _ = ctx.profile_get(nm)
+            if t_ms > 0:
+                per_step[nm] = t_ms / args.steps
+        ctx.set_graph(True)
+        run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, 1, False)
+        ms_graph = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
+        ctx.set_graph(False)
+        r = dict(ms=ms, ms_prof=ms_prof, ms_graph=ms_graph, apply_avg_ms=k_ms / max(k_n, 1),
+                 apply_launches=k_n / max(args.steps, 1), spread_avg_ms=s_ms / max(s_n, 1),
+                 local_avg_ms=l_ms / max(l_n, 1), per_step=per_step)
+        r["u"] = ctx.local_velocity_beads()
+        _, r["Fe"] = ctx.spread(nqp_1d, IF, want_rho=False)
+        ctx.cheb_brownian(order, lo, hi, None, mode=G.CHEB_FIXED, want_y=False, want_disp=False, keep=True)
+        r["disp"] = ctx.get_brownian()[1][0]
+        res[prec] = r
+    return res
+
+
+def cpu_baseline(G, W, args, order, nqp_1d, reps=5):
+    """The oracle's FP64 step (neighbour build, local pair sum, 27-point
+    spreading with the IF contraction, fixed-order Chebyshev step with its
+    vector algebra and the final M y) on a same-density sample of the
+    workload, all affinity cores and one core, one warm-up then the median
+    of `reps`; and the GPU's FP32C and FP64 results on the same sample
+    against the oracle's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    shrink = 4
+    cfg, x, f = W.make_sample(args.workload, shrink)
+    n = len(x)
+    c = None
+    dw = G.normal(3 * n, 3005).reshape(n, 3)
+    # GPU results on the sample, bounds from the device Lanczos
+    gpu = {}
+    for prec in (G.FP32C, G.FP64):
+        ctx = G.Context(dict(cfg, precision=prec))
+        ctx.reinit(x, f)
+        c = ctx.consts()
+        if prec == G.FP32C:
+            lmin, lmax = ctx.lanczos_bounds(iters=40, seed=3005)
+            lo, hi = lmin / 1.05, lmax * 1.05
+        IF = G.hex20_if_table(c.h, nqp_1d)
+        u = ctx.local_velocity_beads()
+        _, Fe = ctx.spread(nqp_1d, IF, want_rho=False)
+        _, d, _ = ctx.cheb_brownian(order, lo, hi, dw, mode=G.CHEB_FIXED, want_y=False)
+        gpu[prec] = (u, Fe, d[0])
         ctx.close()
-    dt_s, stats = res["device"]
-    return {"beads": int(len(x0)), "steps": args.slit_steps, "ms_per_step": dt_s * 1e3 / args.slit_steps,
-            "brownian_steps_per_s": args.slit_steps / dt_s, "cheb_retries": stats["cheb_retries"],
-            "cheb_failures": stats["cheb_failures"],
-            "host_forces_ms_per_step": res["host"][0] * 1e3 / args.slit_steps,
-            "note": "device chain/wall forces, host position update; includes host<->device copies per step"}
+    ocfg = {k: cfg[k] for k in ("mu", "alpha", "box_min", "box_max", "periodic", "n_elem")}
+    ocfg["keep_stale_lists"] = 0
+    if "ksi" in cfg:
+        ocfg["ksi"] = cfg["ksi"]
+
+    def step():
+        m = O.Mesh(ocfg)
+        m.reinit(x, f)
+        u = m.local_velocity_beads()
+        _, Fe = m.spread(nqp_1d, IF, want_rho=False)
+        _, d, _ = m.cheb_brownian(order, lo, hi, 0.1, 1, dw)
+        return m.total_pairs(), u, Fe, d
+
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    times = {}
+    for th in (cores, 1):
+        O.set_threads(th)
+        out = step()  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = step()
+            ts.append(time.perf_counter() - t0)
+        times[th] = float(np.median(ts))
+    pairs, u_o, Fe_o, d_o = out
+    evals = order + 2
+    errs = {}
+    for prec, tag in ((G.FP32C, "fp32c"), (G.FP64, "fp64")):
+        u, Fe, d = gpu[prec]
+        errs[tag] = {"u_local": rel(u, u_o), "Fe": rel(Fe, Fe_o), "brownian_disp": rel(d, d_o)}
+    full_n = W.WORKLOADS[args.workload]["n"]
+    return {"value": evals * pairs / times[cores], "unit": "pair-interactions/s", "cores": cores, "kind": "port",
+            "sample": "the oracle's FP64 step (build, local sum, 27-point spreading, order-%d fixed Chebyshev with "
+                      "its vector algebra and M y) on a same-density 1/%d-volume periodic sample: %d beads, "
+                      "box %.1f^3, %d^3 elements, %d pairs; median of %d after one warm-up"
+                      % (order, shrink ** 3, n, cfg["box_max"][0], cfg["n_elem"][0], pairs, reps),
+            "step_s": times[cores], "step_s_1core": times[1], "value_1core": evals * pairs / times[1],
+            "est_step_s_full_size": times[cores] * full_n / n,
+            "est_step_s_full_size_1core": times[1] * full_n / n}, errs
 
 
 def main():
@@ -264,50 +404,18 @@ def main():
     dw = G.normal(3 * n, 3005).reshape(n, 3)
     ctx.set_noise(dw)
     sp_pairs = nqp * ctx.elem_point_pairs()  # (Gauss point, particle) pairs per spread
-
-    res = {}
-    for prec in ([G.FP32C] if args.no_fp64 else [G.FP32C, G.FP64]):
-        ctx.set_precision(prec)
-        run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.warmup, False)
-        ms = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
-        # the same steps again with every launch bracketed by HIP events on
-        # the library's stream: kernel times and the step breakdown
-        ms_prof = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, True)
-        kname = "apply_fp32c" if prec == G.FP32C else "apply_fp64"
-        sname = "spread_fp32c" if prec == G.FP32C else "spread_fp64"
-        lname = "pair_beads_fp32c" if prec == G.FP32C else "pair_beads_fp64"
-        k_ms, k_n = ctx.profile_get(kname)
-        s_ms, s_n = ctx.profile_get(sname)
-        l_ms, l_n = ctx.profile_get(lname)
-        per_step = {}
-        tname = "tensors_fp32c" if prec == G.FP32C else "tensors_fp64"
-        for nm in ("bin", "scan", "place_sort", "gather_forces", "count_lists", "scan64", "fill_ell",
-                   "elist_count", "elist_fill", tname, "box_count", "box_fill", lname, sname, kname, "axpby"):
-            t_ms, _ = ctx.profile_get(nm)
-            if t_ms > 0:
-                per_step[nm] = t_ms / args.steps
-        # the same steps with the Chebyshev recurrence replayed from a
-        # captured graph, without per-launch event brackets
-        ctx.set_graph(True)
-        run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, 1, False)
-        ms_graph = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
-        ctx.set_graph(False)
-        res[prec] = dict(ms=ms, ms_prof=ms_prof, ms_graph=ms_graph, apply_avg_ms=k_ms / max(k_n, 1), apply_launches=k_n,
-                         spread_avg_ms=s_ms / max(s_n, 1), local_avg_ms=l_ms / max(l_n, 1),
-                         per_step=per_step)
-        # outputs of one step for the error report
-        res[prec]["u"] = ctx.local_velocity_beads()
-        _, Fe = ctx.spread(nqp_1d, IF, want_rho=False)
-        res[prec]["Fe"] = Fe
-        _, disp, _ = ctx.cheb_brownian(order, lo, hi, dw, mode=G.CHEB_FIXED, want_y=False)
-        res[prec]["disp"] = disp
+    names = ["bin", "scan", "place_sort", "gather_forces", "count_lists", "scan64", "fill_ell", "elist_count",
+             "elist_fill", "box_count", "box_fill", "axpby"]
+    res = measure(G, ctx, args, IF, nqp_1d, order, lo, hi, dw,
+                  [G.FP32C] if args.no_fp64 else [G.FP32C, G.FP64], names)
 
     r32 = res[G.FP32C]
     apply_s = r32["apply_avg_ms"] * 1e-3
-    # pair evaluations per step: the local velocity plus the order-N recurrence
-    evals = 1 + order
+    # pair evaluations per step: the local velocity, the order-N recurrence
+    # (N applications) and the displacement M y
+    evals = order + 2
     value = evals * pairs / (r32["ms"] * 1e-3)
-    bytes_apply = B_PAIR_APPLY32 * pairs + B_SLOT_APPLY * n
+    bytes_apply = B_BEAD_APPLY * n
     achieved = bytes_apply / apply_s
     out = {
         "metric": "pair-interactions/s + Brownian steps/s, FP32-comp vs FP64 @ <=1e-6 rel err",
@@ -323,26 +431,27 @@ def main():
         "dtype": "fp32 (compensated) pair sum + spreading; fp64 displacement, recurrence, combination",
         "data": "synthetic: seeded RSA positions (min dist 2.2a), N(0,1) forces and noise",
         "config": {"workload": args.workload, "beads": n, "box": cfg["box_max"], "elements": cfg["n_elem"],
-                   "alpha": cfg["alpha"], "gauss_points_per_elem": nqp, "cheb_order": order,
-                   "neighbor_pairs": pairs, "parallelism": "single GPU"},
+                   "alpha": cfg["alpha"], "xi": c.ksi, "xi_rule": "sqrt(pi)/(3 a) (GGEMStokes::set_ksi)",
+                   "gauss_points_per_elem": nqp, "cheb_order": order, "neighbor_pairs": pairs,
+                   "parallelism": "single GPU"},
         "brownian_steps_per_s": 1e3 / r32["ms"],
         "pair_evaluations_per_step": evals,
         "pair_kernel_interactions_per_s": pairs / apply_s,
         "graph": {"ms_per_step": r32["ms_graph"], "brownian_steps_per_s": 1e3 / r32["ms_graph"]},
-        "roofline": {"bound": "hbm", "kernel": "k_apply_box FP32C Chebyshev step (pair sum + recurrence update)",
+        "roofline": {"bound": "hbm", "kernel": "k_apply_box FP32C Chebyshev application (pair sum + recurrence update)",
                      "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM,
                      "traffic": profiled_traffic("k_apply_box<2>") if args.workload == "beads1m" else None,
-                     "algorithmic": "%d B x %d pairs + %d B x %d slots per launch"
-                                    % (B_PAIR_APPLY32, pairs, B_SLOT_APPLY, n),
-                     "valu_tflops": F_PAIR * pairs / apply_s / 1e12,
-                     "avg_launch_ms": r32["apply_avg_ms"], "launches": r32["apply_launches"]},
+                     "algorithmic": "%d B per bead x %d beads per launch (48 B pair sum in FP32C + 144 B "
+                                    "recurrence minimum)" % (B_BEAD_APPLY, n),
+                     "valu_tflops_algorithmic": F_PAIR * pairs / apply_s / 1e12,
+                     "avg_launch_ms": r32["apply_avg_ms"], "launches_per_step": r32["apply_launches"]},
         "kernels_ms": {"apply_fp32c": r32["apply_avg_ms"], "spread_fp32c": r32["spread_avg_ms"],
                        "pair_beads_fp32c": r32["local_avg_ms"]},
         "step_breakdown_ms": r32["per_step"],
         "profiled_ms_per_step": r32["ms_prof"],
         "spread": {"qp_particle_pairs": sp_pairs,
-                   "tflops": (F_SP * sp_pairs + 6 * 20 * nqp * ne) / (r32["spread_avg_ms"] * 1e-3) / 1e12
+                   "tflops_algorithmic": (F_SP * sp_pairs + 6 * 20 * nqp * ne) / (r32["spread_avg_ms"] * 1e-3) / 1e12
                    if r32["spread_avg_ms"] > 0 else None},
     }
     if G.FP64 in res:
@@ -350,56 +459,41 @@ def main():
         out["fp64"] = {"ms_per_step": r64["ms"], "pair_interactions_per_s": evals * pairs / (r64["ms"] * 1e-3),
                        "pair_kernel_interactions_per_s": pairs / (r64["apply_avg_ms"] * 1e-3),
                        "brownian_steps_per_s": 1e3 / r64["ms"], "graph_ms_per_step": r64["ms_graph"],
-                       "apply_avg_ms": r64["apply_avg_ms"],
-                       "spread_avg_ms": r64["spread_avg_ms"]}
-        out["fp64"]["step_breakdown_ms"] = r64["per_step"]
+                       "apply_avg_ms": r64["apply_avg_ms"], "spread_avg_ms": r64["spread_avg_ms"],
+                       "step_breakdown_ms": r64["per_step"]}
         out["fp32c_speedup_over_fp64"] = r64["ms"] / r32["ms"]
         out["fp32c_rel_l2_vs_fp64"] = {"u_local": rel(r32["u"], r64["u"]), "Fe": rel(r32["Fe"], r64["Fe"]),
                                        "brownian_disp": rel(r32["disp"], r64["disp"])}
     out["fe_velocity"] = bench_fe_velocity(G, ctx, cfg, n)
     ctx.close()
+    del res
+    # the same step at xi = 1/(a sqrt(pi)), the value the configurations state
+    if args.workload in ("beads1m", "beads20k"):
+        cfg_j = dict(cfg, ksi=1.0 / np.sqrt(np.pi))
+        cj = G.Context(cfg_j)
+        cj.reinit(x, f)
+        lmin_j, lmax_j = cj.lanczos_bounds(iters=40, seed=3005)
+        cj.set_noise(dw)
+        rj = measure(G, cj, args, IF, nqp_1d, order, lmin_j / 1.05, lmax_j * 1.05, dw,
+                     [G.FP32C] if args.no_fp64 else [G.FP32C, G.FP64], [])
+        out["xi_1_over_sqrt_pi"] = {"xi": cj.consts().ksi, "fp32c_ms_per_step": rj[G.FP32C]["ms"],
+                                    "pair_interactions_per_s": evals * pairs / (rj[G.FP32C]["ms"] * 1e-3)}
+        if G.FP64 in rj:
+            out["xi_1_over_sqrt_pi"]["fp64_ms_per_step"] = rj[G.FP64]["ms"]
+            out["xi_1_over_sqrt_pi"]["fp32c_rel_l2_vs_fp64"] = {
+                k: rel(rj[G.FP32C][k], rj[G.FP64][k]) for k in ("u", "Fe", "disp")}
+        cj.close()
+        del rj
     if not args.no_batch:
         out["batch256"] = bench_batch(G, W, args)
     if not args.no_slit:
         out["slit25k_fixman"] = bench_slit(G, W, args)
-
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        cores = os.cpu_count() or 1
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except Exception:
-            pass
-        cores = min(cores, 16)
-        O.set_threads(cores)
-        ocfg = {k: cfg[k] for k in ("mu", "alpha", "box_min", "box_max", "periodic", "n_elem")}
-        ocfg["keep_stale_lists"] = 0
-        t0 = time.perf_counter()
-        m = O.Mesh(ocfg)
-        m.reinit(x, f)
-        t_build = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        u_cpu = m.local_velocity_beads()
-        t_pair = time.perf_counter() - t0
-        # spreading on a bounded element sample is not separable from the
-        # oracle's whole-mesh loop, so time the whole mesh at 8 Gauss points
-        # and scale by the (qp, particle) count to 27 points.
-        t0 = time.perf_counter()
-        m.spread(2, G.hex20_if_table(c.h, 2), want_rho=False)
-        t_sp8 = time.perf_counter() - t0
-        t_sp = t_sp8 * 27.0 / 8.0
-        t_step_cpu = t_build + t_pair + t_sp + (order + 1) * t_pair
-        out["cpu_baseline"] = {"value": pairs / t_pair, "unit": "pair-interactions/s", "cores": cores,
-                               "kind": "port",
-                               "sample": "oracle FP64 pair sum over all %d beads (%d pairs), plus neighbour "
-                                         "build and 8-point spreading of the same workload" % (n, pairs),
-                               "pair_sum_s": t_pair, "build_s": t_build, "spread8_s": t_sp8,
-                               "est_step_s": t_step_cpu}
-        out["speedup_vs_cpu_step"] = t_step_cpu * 1e3 / r32["ms"]
-        out["fp32c_rel_l2_vs_oracle_u_local"] = rel(r32["u"], u_cpu)
-        if G.FP64 in res:
-            out["fp64_rel_l2_vs_oracle_u_local"] = rel(res[G.FP64]["u"], u_cpu)
+        base, errs = cpu_baseline(G, W, args, order, nqp_1d)
+        out["cpu_baseline"] = base
+        out["rel_l2_vs_oracle_on_sample"] = errs
+        out["speedup_vs_cpu"] = value / base["value"]
+        out["speedup_vs_cpu_1core"] = value / base["value_1core"]
     print(json.dumps(out))
 
 

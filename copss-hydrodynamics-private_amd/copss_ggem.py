@@ -29,6 +29,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libcopss_ggem.so")
 POLYMER_BEAD, POINT_PARTICLE, LAGRANGIAN_POINT = 0, 1, 2
 FP64, FP32C = 0, 1
 CHEB_REFERENCE, CHEB_FIXED = 0, 1
+CHEB_KEEP = 0x100
+WALL_SLIT, WALL_SPHERE = 0, 1
 COMPAT_ELEM_HYSTERESIS = 1
 
 ERRORS = {
@@ -49,6 +51,10 @@ EXPORTS = [
     "ggem_set_autotune", "ggem_tune_set", "ggem_tuning_report", "ggem_util_reference_noise",
     "ggem_util_rsa_positions", "ggem_util_normal", "ggem_util_hex20_if_table", "ggem_util_fp32c_table",
     "ggem_profile_enable", "ggem_profile_reset", "ggem_profile_get",
+    "ggem_get_brownian", "ggem_displacement_apply", "ggem_noise_seed", "ggem_noise_skip", "ggem_noise_draw",
+    "ggem_noise_philox", "ggem_save_positions", "ggem_move_particles", "ggem_get_positions",
+    "ggem_set_unwrapped", "ggem_get_noise", "ggem_util_noise_create", "ggem_util_noise_destroy",
+    "ggem_util_noise_draw", "ggem_util_noise_skip",
 ]
 
 
@@ -86,6 +92,12 @@ class FixParams(C.Structure):
 class ChebInfo(C.Structure):
     _fields_ = [("converged", C.c_int32), ("terms", C.c_int32), ("applies", C.c_int32),
                 ("pad", C.c_int32), ("err1", C.c_double), ("err2", C.c_double)]
+
+
+class Move(C.Structure):
+    _fields_ = [("c_velocity", C.c_double), ("c_disp", C.c_double), ("c_force", C.c_double),
+                ("c_noise", C.c_double), ("from_saved", C.c_int32), ("wall_type", C.c_int32),
+                ("cavity_radius", C.c_double), ("track", C.c_int32), ("pad", C.c_int32)]
 
 
 APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
@@ -149,6 +161,21 @@ def lib():
             "ggem_profile_enable": [P, I32],
             "ggem_profile_reset": [P],
             "ggem_profile_get": [P, C.c_char_p, P, P],
+            "ggem_get_brownian": [P, P, P],
+            "ggem_displacement_apply": [P, P],
+            "ggem_noise_seed": [P, U64],
+            "ggem_noise_skip": [P, I64],
+            "ggem_noise_draw": [P, D, P],
+            "ggem_noise_philox": [P, U64, U64, I32, D],
+            "ggem_save_positions": [P],
+            "ggem_move_particles": [P, P],
+            "ggem_get_positions": [P, P, P, P],
+            "ggem_set_unwrapped": [P, P],
+            "ggem_get_noise": [P, P],
+            "ggem_util_noise_create": [U64, C.POINTER(C.c_void_p)],
+            "ggem_util_noise_destroy": [P],
+            "ggem_util_noise_draw": [P, I64, D, P],
+            "ggem_util_noise_skip": [P, I64, I64],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -201,6 +228,30 @@ def reference_noise(n, seed, dt, skip_vectors=0):
     out = np.zeros(n)
     _check(lib().ggem_util_reference_noise(n, seed, dt, skip_vectors, _p(out)))
     return out
+
+
+class NoiseStream:
+    """The reference's noise generator kept across steps (one engine for the
+    run): draw(n, dt) -> sqrt(2) N(0, sqrt(dt)) values, skip(n, k)."""
+
+    def __init__(self, seed):
+        h = C.c_void_p()
+        _check(lib().ggem_util_noise_create(int(seed), C.byref(h)))
+        self._h = h
+
+    def draw(self, n, dt):
+        out = np.zeros(n)
+        _check(lib().ggem_util_noise_draw(self._h, n, dt, _p(out)))
+        return out
+
+    def skip(self, n, n_vectors):
+        _check(lib().ggem_util_noise_skip(self._h, n, n_vectors))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ggem_util_noise_destroy(h)
+            self._h = None
 
 
 def rsa_positions(cfg, n, min_dist, seed):
@@ -444,7 +495,7 @@ class Context:
         self._noise_nrhs = nrhs
 
     def cheb_brownian(self, N, eig_min, eig_max, dw=None, tol=0.1, mode=CHEB_REFERENCE, op=None,
-                      want_y=True, want_disp=True):
+                      want_y=True, want_disp=True, keep=False):
         if dw is None:
             nrhs = self._noise_nrhs
         else:
@@ -459,9 +510,71 @@ class Context:
         if op is not None:
             opp = self._make_op(op)
         self._c(lib().ggem_cheb_brownian(self._h, C.byref(opp) if opp is not None else None, N,
-                                         eig_min, eig_max, tol, mode, nrhs, _p(dw), _p(y),
-                                         _p(disp), info))
+                                         eig_min, eig_max, tol, mode | (CHEB_KEEP if keep else 0), nrhs,
+                                         _p(dw), _p(y), _p(disp), info))
+        if keep:
+            self._keep_nrhs = nrhs
         return y, disp, [info[i] for i in range(nrhs)]
+
+    def get_brownian(self):
+        """(y, disp) kept by cheb_brownian(..., keep=True) / displacement_apply: [nrhs, n, 3] each."""
+        nrhs = max(int(getattr(self, "_keep_nrhs", 1)), 1)
+        y = np.zeros((nrhs, self.n, 3))
+        d = np.zeros((nrhs, self.n, 3))
+        self._c(lib().ggem_get_brownian(self._h, _p(y), _p(d)))
+        return y, d
+
+    def displacement_apply(self, out=True):
+        """disp = M y with the kept y at the current positions (hi_ewald at the mid-point)."""
+        nrhs = max(int(getattr(self, "_keep_nrhs", 1)), 1)
+        d = np.zeros((nrhs, self.n, 3)) if out else None
+        self._c(lib().ggem_displacement_apply(self._h, _p(d)))
+        return d
+
+    # noise stream of the context (RandomGenerator's engine) and device noise
+    def noise_seed(self, seed):
+        self._c(lib().ggem_noise_seed(self._h, int(seed)))
+
+    def noise_skip(self, n_vectors):
+        self._c(lib().ggem_noise_skip(self._h, int(n_vectors)))
+
+    def noise_draw(self, dt, out=True):
+        """Next sqrt(2) N(0, sqrt(dt)) vector; it becomes the device noise (nrhs 1)."""
+        dw = np.zeros((self.n, 3)) if out else None
+        self._c(lib().ggem_noise_draw(self._h, dt, _p(dw)))
+        self._noise_nrhs = 1
+        return dw
+
+    def get_noise(self):
+        dw = np.zeros((int(getattr(self, "_noise_nrhs", 1)), self.n, 3))
+        self._c(lib().ggem_get_noise(self._h, _p(dw)))
+        return dw
+
+    def noise_philox(self, seed, counter, nrhs, dt):
+        self._c(lib().ggem_noise_philox(self._h, int(seed), int(counter), int(nrhs), dt))
+        self._noise_nrhs = int(nrhs)
+
+    # positions on the device
+    def save_positions(self):
+        self._c(lib().ggem_save_positions(self._h))
+
+    def move_particles(self, c_velocity=0.0, c_disp=0.0, c_force=0.0, c_noise=0.0, from_saved=False,
+                       wall_type=WALL_SLIT, cavity_radius=0.0, track=False):
+        mv = Move(c_velocity, c_disp, c_force, c_noise, int(bool(from_saved)), int(wall_type), cavity_radius,
+                  int(bool(track)), 0)
+        self._c(lib().ggem_move_particles(self._h, C.byref(mv)))
+
+    def get_positions(self):
+        """(positions, unwrapped positions, image counts) [n, 3] each."""
+        x = np.zeros((self.n, 3))
+        r = np.zeros((self.n, 3))
+        im = np.zeros((self.n, 3), np.int32)
+        self._c(lib().ggem_get_positions(self._h, _p(x), _p(r), _p(im)))
+        return x, r, im
+
+    def set_unwrapped(self, rout=None):
+        r = None if rout is None else _f64(rout, (-1, 3))
+        self._c(lib().ggem_set_unwrapped(self._h, _p(r)))
 
     def lanczos_bounds(self, iters=30, seed=7, op=None):
         lo, hi = C.c_double(), C.c_double()

@@ -400,6 +400,11 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->pc32);
   f(ctx->coef64);
   f(ctx->noise);
+  f(ctx->keep_y);
+  f(ctx->keep_d);
+  f(ctx->x_saved);
+  f(ctx->rout);
+  f(ctx->images);
   f(ctx->dpar);
   f(ctx->sys_d);
   f(ctx->part_off_d);
@@ -777,6 +782,11 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     ctx->cap = n;
     ctx->elem_valid = false;
   }
+  if (n != ctx->n) {
+    ctx->saved_valid = ctx->rout_valid = ctx->keep_d_valid = false;
+    ctx->keep_nrhs = 0;
+  }
+  ctx->u_out_valid = false;
   ctx->n = n;
   ctx->built = false;
   ctx->elist_valid = false;
@@ -927,6 +937,7 @@ int ggem_global_self_exclusion(ggem_ctx* ctx, double* uex) {
   int rc = check_ready(ctx);
   if (rc) return rc;
   const int64_t len = 3 * ctx->n;
+  ctx->u_out_valid = false;
   hipError_t e = launch_fill(ctx, len, 0.0, ctx->u_out);
   if (e == hipSuccess) e = launch_axpby(ctx, len, ctx->c.coeff_1, ctx->f_in, 0.0, ctx->u_out);
   if (e != hipSuccess) return hip_fail(ctx, e, "self_exclusion");
@@ -948,6 +959,7 @@ int ggem_combine_velocity(ggem_ctx* ctx, const double* u_glob, double* U) {
   }
   hipError_t e = launch_pair_beads(ctx, ctx->p.precision, EPI_MOBILITY, ug, ctx->u_out);
   if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
+  ctx->u_out_valid = true;
   if (U && len > 0) return d2h(ctx, U, ctx->u_out, sizeof(double) * len);
   return GGEM_OK;
 }
@@ -958,6 +970,10 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
   const int64_t ne = ctx->grid.n_elem;
   if (n_nodes < 1 || n_nodes > INT32_MAX || !elem_nodes)
     return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid node count or connectivity");
+  // each incidence is stored as the int32 number 20 e + l
+  if (20 * ne > int64_t(INT32_MAX))
+    return fail(ctx, GGEM_ERR_CAPACITY, "the FE mesh needs 20 * n_elem <= 2^31 - 1 (" + std::to_string(ne) +
+                                            " elements)");
   for (int64_t k = 0; k < 20 * ne; ++k)
     if (elem_nodes[k] < 0 || elem_nodes[k] >= n_nodes) {
       char buf[128];
@@ -1050,6 +1066,7 @@ int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U) {
   if ((rc = interp_global(ctx, node_u, P.io))) return rc;
   hipError_t e = launch_pair_beads(ctx, ctx->p.precision, EPI_MOBILITY, P.io, ctx->u_out);
   if (e != hipSuccess) return hip_fail(ctx, e, "pair_beads");
+  ctx->u_out_valid = true;
   if (U && len > 0) return d2h(ctx, U, ctx->u_out, sizeof(double) * len);
   return GGEM_OK;
 }
@@ -1230,10 +1247,11 @@ int ggem_profile_get(ggem_ctx* ctx, const char* kernel, double* total_ms, int64_
   prof_collect(ctx);
   double ms = 0;
   int64_t cnt = 0;
+  const bool all = std::string(kernel) == "*";
   for (auto& a : ctx->prof.acc)
-    if (a.name == kernel) {
-      ms = a.ms;
-      cnt = a.launches;
+    if (all || a.name == kernel) {
+      ms += a.ms;
+      cnt += a.launches;
     }
   if (total_ms) *total_ms = ms;
   if (launches) *launches = cnt;

@@ -65,6 +65,19 @@ int reserve(ggem_ctx* ctx, int nvec, int nrhs) {
   return rc;
 }
 
+// the kept Brownian vectors (GGEM_CHEB_KEEP) for nrhs operands
+int ensure_keep(ggem_ctx* ctx, int nrhs) {
+  const size_t len = size_t(nrhs) * 3 * size_t(ctx->n);
+  if (ctx->keep_cap >= len && ctx->keep_y) return GGEM_OK;
+  size_t c1 = ctx->keep_cap, c2 = ctx->keep_cap;
+  int rc = ensure(ctx, &ctx->keep_y, &c1, len);
+  if (!rc) rc = ensure(ctx, &ctx->keep_d, &c2, len);
+  ctx->keep_cap = rc ? 0 : len;
+  ctx->keep_nrhs = 0;
+  ctx->keep_d_valid = false;
+  return rc;
+}
+
 int refresh(ggem_ctx* ctx, int nrhs, const SlotVec& v) {
   hipError_t e = launch_make_gather(ctx, nrhs, v.soa, v.g32, v.g64);
   return e == hipSuccess ? GGEM_OK : hip_fail(ctx, e, "gather records");
@@ -297,6 +310,8 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
   int rc = check_ready(ctx);
   if (rc) return rc;
   if ((rc = single_only(ctx, "ggem_cheb_brownian (use ggem_batch_cheb_brownian)"))) return rc;
+  const bool keep = (mode & GGEM_CHEB_KEEP) != 0;
+  mode &= ~GGEM_CHEB_KEEP;
   if (N < 1 || nrhs < 1 || !(eig_max > eig_min) || (mode != GGEM_CHEB_REFERENCE && mode != GGEM_CHEB_FIXED))
     return fail(ctx, GGEM_ERR_INVALID_ARG, "invalid Chebyshev arguments");
   if (!dw && (ctx->noise_nrhs != nrhs || !ctx->noise))
@@ -337,7 +352,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     par[0] = da;
     par[1] = db;
     std::copy(X.begin(), X.end(), par.begin() + 2);
-    const FixedStep st{N, nrhs, par.data(), nullptr, dw_dev, x0, x1, yv, dwm, disp != nullptr};
+    const FixedStep st{N, nrhs, par.data(), nullptr, dw_dev, x0, x1, yv, dwm, disp != nullptr || keep};
     if (!op && ctx->graph_on) {
       if ((rc = upload_params(ctx, par.data(), par.size()))) return rc;
       rc = run_fixed_graph(ctx, apply, st);
@@ -349,7 +364,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     for (auto& i : inf) {
       i.converged = 1;
       i.terms = N;
-      i.applies = N + (disp ? 1 : 0);
+      i.applies = N + (disp || keep ? 1 : 0);
     }
   } else {
     e = launch_permute_in(ctx, nrhs, dw_dev, x0.soa);
@@ -404,24 +419,74 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
         if (err2 <= 1E-9) break;
         std::swap(a0, a1);
       }
-      inf[r] = ggem_cheb_info{conv ? 1 : 0, terms, applies + (disp ? 1 : 0), 0, err1, err2};
+      inf[r] = ggem_cheb_info{conv ? 1 : 0, terms, applies + (disp || keep ? 1 : 0), 0, err1, err2};
     }
   }
   if (y) {
     if ((e = launch_permute_out(ctx, nrhs, yv.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
     if ((rc = d2h(ctx, y, P.io, sizeof(double) * blk))) return rc;
   }
-  if (disp) {  // hi_ewald: disp = M y (copss.C:1686)
+  if (disp || keep) {  // hi_ewald: disp = M y (copss.C:1686)
     if (!disp_done) {
       if ((rc = refresh(ctx, nrhs, yv))) return rc;
       if ((rc = apply(nrhs, yv, dwm, plain))) return rc;
     }
+  }
+  if (keep) {  // y and M y stay on the device, particle order
+    if ((rc = ensure_keep(ctx, nrhs))) return rc;
+    if ((e = launch_permute_out(ctx, nrhs, yv.soa, ctx->keep_y)) != hipSuccess ||
+        (e = launch_permute_out(ctx, nrhs, dwm.soa, ctx->keep_d)) != hipSuccess)
+      return hip_fail(ctx, e, "permute");
+    ctx->keep_nrhs = nrhs;
+    ctx->keep_d_valid = true;
+  }
+  if (disp) {
     if ((e = launch_permute_out(ctx, nrhs, dwm.soa, P.io)) != hipSuccess) return hip_fail(ctx, e, "permute");
     if ((rc = d2h(ctx, disp, P.io, sizeof(double) * blk))) return rc;
   }
   if (info)
     for (int r = 0; r < nrhs; ++r) info[r] = inf[r];
   return y || disp ? sync(ctx) : GGEM_OK;
+}
+
+extern "C" int ggem_get_brownian(ggem_ctx* ctx, double* y, double* disp) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (ctx->keep_nrhs < 1) return fail(ctx, GGEM_ERR_NOT_READY, "no kept Brownian step (GGEM_CHEB_KEEP)");
+  const size_t bytes = sizeof(double) * size_t(ctx->keep_nrhs) * 3 * size_t(ctx->n);
+  int rc = GGEM_OK;
+  if (y && bytes && (rc = d2h(ctx, y, ctx->keep_y, bytes))) return rc;
+  if (disp && bytes) {
+    if (!ctx->keep_d_valid) return fail(ctx, GGEM_ERR_NOT_READY, "the kept displacement is not valid");
+    rc = d2h(ctx, disp, ctx->keep_d, bytes);
+  }
+  return rc;
+}
+
+// hi_ewald(M, dw, dw_mid) at the mid-point (copss.C:1730): M at the current
+// positions and lists applied to the kept y.
+extern "C" int ggem_displacement_apply(ggem_ctx* ctx, double* disp) {
+  int rc = check_ready(ctx);
+  if (rc) return rc;
+  if ((rc = single_only(ctx, "ggem_displacement_apply"))) return rc;
+  const int nrhs = ctx->keep_nrhs;
+  if (nrhs < 1) return fail(ctx, GGEM_ERR_NOT_READY, "no kept Brownian step (GGEM_CHEB_KEEP)");
+  const int64_t n = ctx->n;
+  if (n == 0) return GGEM_OK;
+  if ((rc = reserve(ctx, 2, nrhs))) return rc;
+  if ((rc = ensure_tensors(ctx, nrhs))) return rc;
+  Pools& P = pools(ctx);
+  const int prec = ctx->p.precision;
+  const SlotVec yv = slot_vec(P, 0, nrhs, n, prec), dv = slot_vec(P, 1, nrhs, n, prec);
+  hipError_t e = launch_permute_in(ctx, nrhs, ctx->keep_y, yv.soa);
+  if (e != hipSuccess) return hip_fail(ctx, e, "permute");
+  if ((rc = refresh(ctx, nrhs, yv))) return rc;
+  Applier apply{ctx, nullptr, prec, {}, {}};
+  const SlotVec none{nullptr, nullptr, nullptr};
+  if ((rc = apply(nrhs, yv, dv, ChebEpi{0, 0.0, 0.0, 0.0, none, nullptr}))) return rc;
+  if ((e = launch_permute_out(ctx, nrhs, dv.soa, ctx->keep_d)) != hipSuccess) return hip_fail(ctx, e, "permute");
+  ctx->keep_d_valid = true;
+  if (disp) return d2h(ctx, disp, ctx->keep_d, sizeof(double) * size_t(nrhs) * 3 * size_t(n));
+  return GGEM_OK;
 }
 
 extern "C" int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -214,6 +215,26 @@ struct ggem_ctx {
   double* noise = nullptr;
   int64_t noise_cap = 0;
   int32_t noise_nrhs = 0;
+  // the reference's noise engine, one for the whole run (host; sequential)
+  std::default_random_engine noise_gen;
+  std::vector<double> noise_h;
+
+  // Brownian vectors kept on the device (GGEM_CHEB_KEEP): y and the
+  // displacement M y, particle order [nrhs][3n]
+  double* keep_y = nullptr;
+  double* keep_d = nullptr;
+  size_t keep_cap = 0;
+  int32_t keep_nrhs = 0;
+  bool keep_d_valid = false;
+  // positions saved for a move (R0), unwrapped positions (ROUT) and the
+  // periodic image counts, particle order
+  double* x_saved = nullptr;
+  double* rout = nullptr;
+  int32_t* images = nullptr;
+  int64_t motion_cap = 0;
+  bool saved_valid = false;
+  bool rout_valid = false;
+  bool u_out_valid = false;  // u_out holds the velocity of the last combination
 
   // fixed-order Chebyshev step captured as a HIP graph; the recurrence
   // reads [da, db, c_0..c_N] from dpar, so one capture serves any bounds

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <random>
 #include <vector>
 
@@ -33,6 +34,45 @@ int ggem_util_reference_noise(int64_t n, uint64_t seed, double dt, int64_t skip_
   for (int64_t i = 0; i < n; ++i) out[i] = dist(gen);
   const double s2 = std::sqrt(2.0);
   for (int64_t i = 0; i < n; ++i) out[i] *= s2;
+  return GGEM_OK;
+}
+
+// The same generator as a stream that persists across calls, as the
+// reference keeps one RandomGenerator for the whole run
+// (brownian_system.h:292, copss.C:1389): each draw continues the engine.
+struct ggem_noise_stream {
+  std::default_random_engine gen;
+};
+
+int ggem_util_noise_create(uint64_t seed, ggem_noise_stream** out) {
+  if (!out) return GGEM_ERR_INVALID_ARG;
+  *out = new (std::nothrow) ggem_noise_stream;
+  if (!*out) return GGEM_ERR_CAPACITY;
+  (*out)->gen.seed(static_cast<std::default_random_engine::result_type>(seed));
+  return GGEM_OK;
+}
+
+int ggem_util_noise_destroy(ggem_noise_stream* s) {
+  delete s;
+  return GGEM_OK;
+}
+
+int ggem_util_noise_draw(ggem_noise_stream* s, int64_t n, double dt, double* out) {
+  if (!s || n < 0 || (n > 0 && !out) || !(dt > 0)) return GGEM_ERR_INVALID_ARG;
+  std::normal_distribution<double> dist(0.0, std::sqrt(dt));
+  dist.reset();
+  const double s2 = std::sqrt(2.0);
+  for (int64_t i = 0; i < n; ++i) out[i] = dist(s->gen) * s2;
+  return GGEM_OK;
+}
+
+int ggem_util_noise_skip(ggem_noise_stream* s, int64_t n, int64_t n_vectors) {
+  if (!s || n < 0 || n_vectors < 0) return GGEM_ERR_INVALID_ARG;
+  for (int64_t v = 0; v < n_vectors; ++v) {
+    std::normal_distribution<double> d(0.0, 1.0);
+    d.reset();
+    for (int64_t i = 0; i < n; ++i) (void)d(s->gen);
+  }
   return GGEM_OK;
 }
 

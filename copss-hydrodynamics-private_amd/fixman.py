@@ -5,21 +5,35 @@ the GGEM mobility of this package as M (the device part U_loc - coeff_1 f
 + f of _MatMult_Stokes; the host FEM part is not included):
 
   f0 = F(R0); U0 = M(R0) f0
-  dw = sqrt(2) N(0, sqrt(dt))                      (copss.C:1565-1572)
+  dw = sqrt(2) N(0, sqrt(dt)), the next vector of one noise engine kept for
+       the whole run (copss.C:1565-1572, random_generator.C:43-67)
   y  = M(R0)^{-1/2} dw  by Chebyshev; on failure the eigenvalue bounds are
-       recomputed (x eig_factor) and the step retried once (:1585-1660)
-  R_mid = R0 + dt/2 U0 + 1/2 M(R0) y               (:1676-1694)
+       recomputed (x eig_factor, checked positive and ordered) and the step
+       retried once (copss.C:1585-1660)
+  R_mid = R0 + dt/2 U0 + 1/2 M(R0) y               (:1676-1704)
   f_mid = F(R_mid); U_mid = M(R_mid) f_mid
-  R1 = R0 + dt U_mid + M(R_mid) y                  (:1718-1736)
+  R1 = R0 + dt U_mid + M(R_mid) y                  (:1718-1745)
 
-with periodic wrapping after each position update.  The mobility backend
-is any object with reinit(x, f), mobility_apply(f), cheb(N, lo, hi, tol,
-dw) -> (y, disp, converged) and bounds() -> (lmin, lmax); DeviceMobility
-wraps a copss_ggem.Context.  Forces come from a callable F(x) -> f (the
-FENE chains and slit walls of SlitPolymerForces for the polymer-slit
-workload, computed on the host), or from DeviceSlitForces, which forms the
-same forces on the device from the positions already there
-(ggem_fix_forces) so that they never cross to the host.
+each position update followed by FixPoint::check_walls (fix/fix_point.C:27-90:
+one periodic shift per axis, reflection at walls).  When the second
+Chebyshev attempt fails too, the step is replaced by n_relax_step
+free-draining steps R = R0 + dt F(R0) + dw (Copss::langevin_integrate,
+copss.C:1751-1763, :1884-1925).
+
+Two drivers:
+- FixmanIntegrator: positions on the host, any mobility backend with
+  reinit(x, f), mobility_apply(f), cheb(N, lo, hi, tol, dw) -> (y, disp,
+  converged) and bounds() -> (lmin, lmax) (DeviceMobility wraps a
+  copss_ggem.Context; the tests pass the oracle's).  Forces come from a
+  callable F(x) -> f, or from DeviceSlitForces (formed on the device by
+  ggem_fix_forces).
+- DeviceFixman: the same step with every vector on the device: the noise
+  engine of the context (ggem_noise_draw), the chain and wall forces
+  (ggem_fix_forces), U (ggem_combine_velocity), y and M y kept by the
+  Chebyshev step (GGEM_CHEB_KEEP), M(R_mid) y (ggem_displacement_apply) and
+  the position updates with their wall and periodic corrections
+  (ggem_move_particles).  Per step only the Chebyshev estimators' scalars
+  and the noise vector cross to the device.
 """
 import numpy as np
 
@@ -39,9 +53,10 @@ class DeviceMobility:
         self.ctx.reinit(x, f)
 
     def reinit_fixes(self, x, fixes):
-        """Positions, neighbour build, then the forces of `fixes` on the device."""
+        """Positions, neighbour build, then the forces of `fixes` on the device
+        (returned to the host)."""
         self.ctx.reinit(x, None)
-        self.ctx.fix_forces(out=False, **fixes.params)
+        return self.ctx.fix_forces(**fixes.params)
 
     def mobility_apply(self, f):
         return self.ctx.mobility_apply(f)
@@ -109,58 +124,174 @@ class DeviceSlitForces:
         self.params = dict(chain_beads=n_beads, k=k, b_max=b_max, walls=True, eps=eps, sigma=sigma, rcut=rcut)
 
 
-def wrap(x, cfg):
+def check_walls(x, cfg):
+    """FixPoint::check_walls, slit walls (fix/fix_point.C:33-62): on a periodic
+    axis one shift each way (x < min: += L; x >= max: -= L), on a wall axis
+    reflection (x < min: 2 min - x; x >= max: 2 max - x)."""
     lo = np.array(cfg["box_min"], float)
-    L = np.array(cfg["box_max"], float) - lo
+    hi = np.array(cfg["box_max"], float)
+    L = hi - lo
     per = np.array(cfg["periodic"], bool)
-    return np.where(per, lo + np.mod(x - lo, L), x)
+    x = np.array(x, float).reshape(-1, 3)
+    x = np.where(per & (x < lo), x + L, x)
+    x = np.where(per & (x >= hi), x - L, x)
+    x = np.where(~per & (x < lo), 2 * lo - x, x)
+    x = np.where(~per & (x >= hi), 2 * hi - x, x)
+    return x
+
+
+def valid_bounds(lo, hi):
+    """The range check of copss.C:1609-1614."""
+    return not (lo < 0 or hi < 0 or lo > hi)
 
 
 class FixmanIntegrator:
-    """copss.C:1565-1745 on the mobility backend `mob` (see module doc)."""
+    """copss.C:1565-1763 on the mobility backend `mob` (see module doc)."""
 
-    def __init__(self, cfg, mob, forces, dt, cheb_order=30, tol_cheb=0.1, eig_factor=1.05, seed=123456789):
+    def __init__(self, cfg, mob, forces, dt, cheb_order=30, tol_cheb=0.1, eig_factor=1.05, seed=123456789,
+                 n_relax_step=1, restart_step=0):
         self.cfg, self.mob, self.forces = cfg, mob, forces
-        self.dt, self.N, self.tol, self.factor, self.seed = dt, cheb_order, tol_cheb, eig_factor, seed
+        self.dt, self.N, self.tol, self.factor = dt, cheb_order, tol_cheb, eig_factor
+        self.n_relax = n_relax_step
+        self.noise = G.NoiseStream(seed)
+        self.restart_step = restart_step
         self.eig = None
-        self.step_no = 0
-        self.stats = {"cheb_retries": 0, "cheb_failures": 0}
+        self.step_no = restart_step
+        self.stats = {"cheb_retries": 0, "cheb_failures": 0, "relax_steps": 0}
 
     def _bounds(self):
         lmin, lmax = self.mob.bounds()
         self.eig = (lmin / self.factor, lmax * self.factor)
+        return valid_bounds(*self.eig)
+
+    def _forces(self, x):
+        """reinit at x and the forces there (host array)."""
+        if getattr(self.forces, "on_device", False):
+            return self.mob.reinit_fixes(x, self.forces)
+        f = self.forces(x)
+        self.mob.reinit(x, f)
+        return f
+
+    def _draw(self, n):
+        if self.step_no == self.restart_step and self.restart_step > 0 and not getattr(self, "_caught_up", False):
+            self.noise.skip(3 * n, self.restart_step)  # copss.C:1544-1555, once
+            self._caught_up = True
+        return self.noise.draw(3 * n, self.dt).reshape(n, 3)
 
     def step(self, x0, dw=None):
         x0 = np.asarray(x0, float).reshape(-1, 3)
         n = len(x0)
-        dev = getattr(self.forces, "on_device", False)
-        f0 = None if dev else self.forces(x0)
-        if dev:
-            self.mob.reinit_fixes(x0, self.forces)
-        else:
-            self.mob.reinit(x0, f0)
+        f0 = self._forces(x0)
         u0 = self.mob.mobility_apply(f0)
         if dw is None:
-            dw = G.reference_noise(3 * n, self.seed, self.dt, self.step_no).reshape(n, 3)
-        if self.eig is None:
-            self._bounds()
+            dw = self._draw(n)
+        ok = False
         y = d0 = None
+        recompute = self.eig is None
         for attempt in range(2):
+            if recompute and not self._bounds():
+                break  # invalid range: the step fails (copss.C:1609-1614)
             y, d0, ok = self.mob.cheb(self.N, self.eig[0], self.eig[1], self.tol, dw)
             if ok:
                 break
             self.stats["cheb_retries"] += 1
-            self._bounds()
-        else:
+            recompute = True
+        if not ok:
             self.stats["cheb_failures"] += 1
-        xm = wrap(x0 + 0.5 * self.dt * u0 + 0.5 * d0, self.cfg)
-        fm = None if dev else self.forces(xm)
-        if dev:
-            self.mob.reinit_fixes(xm, self.forces)
-        else:
-            self.mob.reinit(xm, fm)
+            return self._relax(x0, f0)
+        xm = check_walls(x0 + 0.5 * self.dt * u0 + 0.5 * d0, self.cfg)
+        fm = self._forces(xm)
         um = self.mob.mobility_apply(fm)
         dm = self.mob.mobility_apply(y)
-        x1 = wrap(x0 + self.dt * um + dm, self.cfg)
+        x1 = check_walls(x0 + self.dt * um + dm, self.cfg)
         self.step_no += 1
         return x1
+
+    def _relax(self, x, f):
+        """n_relax_step free-draining steps (Copss::langevin_integrate):
+        U = F(R) (drag 1), R = R + dt U + dw, then the wall corrections."""
+        for r in range(self.n_relax):
+            if r > 0:
+                f = self._forces(x)
+            dw = self._draw(len(x))
+            x = check_walls(x + self.dt * f + dw, self.cfg)
+            self.step_no += 1
+            self.stats["relax_steps"] += 1
+        return x
+
+
+class DeviceFixman:
+    """copss.C:1565-1763 with positions, forces, noise, velocities and
+    displacements on the device (see module doc).  `fixes`: a
+    DeviceSlitForces (or None for no forces); bounds: fixed (lmin, lmax), or
+    None for Lanczos on the device mobility."""
+
+    def __init__(self, ctx, x0, fixes, dt, cheb_order=30, tol_cheb=0.1, eig_factor=1.05, seed=123456789,
+                 n_relax_step=1, bounds=None, lanczos_iters=40, lanczos_seed=11, wall_type=G.WALL_SLIT,
+                 cavity_radius=0.0):
+        self.ctx, self.fixes = ctx, fixes
+        self.dt, self.N, self.tol, self.factor = dt, cheb_order, tol_cheb, eig_factor
+        self.n_relax = n_relax_step
+        self.fixed_bounds = bounds
+        self.iters, self.lseed = lanczos_iters, lanczos_seed
+        self.walls = dict(wall_type=wall_type, cavity_radius=cavity_radius)
+        self.eig = None
+        self.step_no = 0
+        self.stats = {"cheb_retries": 0, "cheb_failures": 0, "relax_steps": 0}
+        ctx.set_particles(np.asarray(x0, float).reshape(-1, 3))
+        ctx.noise_seed(seed)
+        ctx.set_unwrapped()
+
+    def _reinit(self):
+        """neighbour build at the context's positions and the forces there"""
+        self.ctx.build_neighbors()
+        if self.fixes is not None:
+            self.ctx.fix_forces(out=False, **self.fixes.params)
+
+    def _bounds(self):
+        lmin, lmax = self.fixed_bounds or self.ctx.lanczos_bounds(iters=self.iters, seed=self.lseed)
+        self.eig = (lmin / self.factor, lmax * self.factor)
+        return valid_bounds(*self.eig)
+
+    def step(self):
+        c = self.ctx
+        c.save_positions()                      # R0
+        self._reinit()
+        c.combine_velocity(out=False)           # U0 = M(R0) f0
+        c.noise_draw(self.dt, out=False)        # dw
+        ok = False
+        recompute = self.eig is None
+        for attempt in range(2):
+            if recompute and not self._bounds():
+                break
+            _, _, info = c.cheb_brownian(self.N, self.eig[0], self.eig[1], None, tol=self.tol,
+                                         mode=G.CHEB_REFERENCE, want_y=False, want_disp=False, keep=True)
+            ok = bool(info[0].converged)
+            if ok:
+                break
+            self.stats["cheb_retries"] += 1
+            recompute = True
+        if not ok:
+            self.stats["cheb_failures"] += 1
+            self._relax()
+            return
+        c.move_particles(c_velocity=0.5 * self.dt, c_disp=0.5, from_saved=True, **self.walls)  # R_mid
+        self._reinit()
+        c.combine_velocity(out=False)           # U_mid = M(R_mid) f_mid
+        c.displacement_apply(out=False)         # M(R_mid) y
+        c.move_particles(c_velocity=self.dt, c_disp=1.0, from_saved=True, track=True, **self.walls)  # R1
+        self.step_no += 1
+
+    def _relax(self):
+        """n_relax_step free-draining steps: R = R + dt F(R) + dw; the forces
+        are those of the reinit at the start of the step, then recomputed."""
+        for r in range(self.n_relax):
+            if r > 0:
+                self._reinit()
+            self.ctx.noise_draw(self.dt, out=False)
+            self.ctx.move_particles(c_force=self.dt, c_noise=1.0, track=True, **self.walls)
+            self.step_no += 1
+            self.stats["relax_steps"] += 1
+
+    def positions(self):
+        return self.ctx.get_positions()

@@ -3,7 +3,13 @@ a LAMMPS-style data file), PolymerChain::read_particles_data
 (polymer_chain.C:61-225): the header counts (atoms, bonds, atom types,
 bond types), the Masses, Atoms and Bonds sections.  Ids in the file are
 1-based and come back 0-based, as the reference stores them.  Where the
-reference stops with libmesh_error() this raises ValueError.
+reference stops with libmesh_error() (a bad header word, no Masses or
+Atoms section) this raises ValueError.  Where it reads on with a failed
+stream extraction it does not stop either: records missing at the end of
+the Atoms or Bonds section repeat the last record read (the reference's
+extraction leaves its variables unchanged), and a file without a Bonds
+section has no bonds (bonds is empty, bonds_missing True; the reference's
+values there are uninitialised).
 
 The positions and constant forces feed Context.set_particles; chains of
 consecutive bead ids give chain_beads for ggem_fix_forces.  The restart
@@ -12,12 +18,25 @@ at the end of this module."""
 import numpy as np
 
 
-def _section(lines, pos, name):
+def _section(lines, pos, name, required=True):
     while pos < len(lines) and lines[pos].strip() != name:
         pos += 1
     if pos == len(lines):
+        if not required:
+            return None
         raise ValueError("%s are not defined correctly in the data file" % name)
     return pos + 2  # the section title, then one empty line
+
+
+def _records(tok, width, count, dtype):
+    """count records of `width` numbers from the tokens; records past the
+    last complete one repeat it, as the reference's failed extractions
+    leave their variables unchanged (polymer_chain.C:148-205)."""
+    have = min(len(tok) // width, count)
+    rec = np.array(tok[:width * have], dtype=dtype).reshape(have, width)
+    if have < count and have > 0:
+        rec = np.concatenate([rec, np.repeat(rec[-1:], count - have, axis=0)])
+    return rec
 
 
 def read_particle_data(path):
@@ -49,9 +68,11 @@ def read_particle_data(path):
     masses = np.array([float(tok[2 * i + 1]) for i in range(n_types)])
     p = _section(lines, p, "Atoms")
     tok = " ".join(lines[p:]).split()
-    if len(tok) < 13 * n_beads:
-        raise ValueError("too few atom records in the data file")
-    rec = np.array(tok[:13 * n_beads], dtype=float).reshape(n_beads, 13)
+    # the Atoms section ends where the next section title starts
+    stop = next((i for i, t in enumerate(tok) if t == "Bonds"), len(tok))
+    rec = _records(tok[:stop], 13, n_beads, float)
+    if rec.shape[0] < n_beads:
+        raise ValueError("no atom records in the data file")
     out = {
         "box": box, "masses": masses,
         "bead_id": rec[:, 0].astype(np.int64) - 1, "chain_id": rec[:, 1].astype(np.int64) - 1,
@@ -59,12 +80,12 @@ def read_particle_data(path):
         "rot_vec": rec[:, 6:9].copy(),  # the file holds (a, b, c); the reference's theta stays 0
         "constant_force": rec[:, 9:12].copy(), "charge": rec[:, 12].copy(),
     }
-    p = _section(lines, p, "Bonds")
-    tok = " ".join(lines[p:]).split()
-    if len(tok) < 4 * n_bonds:
-        raise ValueError("too few bond records in the data file")
-    b = np.array(tok[:4 * n_bonds], dtype=np.int64).reshape(n_bonds, 4)
+    q = _section(lines, p, "Bonds", required=False)
+    b = np.zeros((0, 4), np.int64)
+    if q is not None and n_bonds > 0:
+        b = _records(" ".join(lines[q:]).split(), 4, n_bonds, np.int64)
     out["bonds"] = b[:, 1:] - 1  # bond type, bead 1, bead 2
+    out["bonds_missing"] = b.shape[0] < n_bonds
     n_chains = int(rec[-1, 1]) if n_beads else 0  # the last bead's chain, as the reference takes it
     out["n_chains"] = n_chains
     out["beads_per_chain"] = np.bincount(out["chain_id"], minlength=n_chains)

@@ -50,6 +50,24 @@ WORKLOADS = {
 }
 
 
+def make_sample(name, shrink=4, **kw):
+    """(cfg, positions, forces) of a periodic workload shrunk by `shrink` per
+    axis at the same bead density and element size: box / shrink,
+    elements / shrink, beads / shrink^3 (random sequential addition at 2.2a
+    and N(0,1) forces with the workload's seeds).  Every bead sees the same
+    stencil, neighbour density and Gaussian support as in the full
+    workload, so per-bead costs carry over by the bead count."""
+    w = WORKLOADS[name]
+    full = w["cfg"](**kw)
+    assert all(full["periodic"]), "samples are taken of periodic workloads"
+    cfg = dict(full, box_max=[b / shrink for b in full["box_max"]],
+               n_elem=[int(ne) // shrink for ne in full["n_elem"]])
+    n = w["n"] // shrink ** 3
+    x = G.rsa_positions(cfg, n, 2.2, 1000 + w["id"])
+    f = G.normal(3 * n, 2000 + w["id"]).reshape(n, 3)
+    return cfg, x, f
+
+
 def ggem3_cfg(**kw):
     return _cfg([-15, -15, -15], [15, 15, 15], [0, 0, 0], [15, 15, 15], alpha=0.35, **kw)
 

@@ -227,6 +227,11 @@ typedef struct ggem_cheb_info {
 } ggem_cheb_info;
 
 enum { GGEM_CHEB_REFERENCE = 0, GGEM_CHEB_FIXED = 1 };
+/* OR'ed into the mode of ggem_cheb_brownian: y and the displacement M y are
+ * computed on the device and kept there, in particle order, for
+ * ggem_move_particles, ggem_displacement_apply and ggem_get_brownian, also
+ * when the y and disp pointers are NULL. */
+#define GGEM_CHEB_KEEP 0x100
 
 /* BrownianSystem::chebyshev_polynomial_approximation (brownian_system.h:239-243,
  * brownian_system.C:967-1178) followed by hi_ewald (copss.C:1686):
@@ -259,6 +264,74 @@ int ggem_set_graph(ggem_ctx* ctx, int32_t enable);
  * eig_factor as copss.C:1607 does. */
 int ggem_lanczos_bounds(ggem_ctx* ctx, const ggem_op* op, int32_t iters, uint64_t seed,
                         double* lmin, double* lmax);
+
+/* ---- Brownian integration on the device -------------------------------- */
+/* Copies of the vectors kept by the last ggem_cheb_brownian with
+ * GGEM_CHEB_KEEP (or updated by ggem_displacement_apply): y = M^{-1/2} dw
+ * and disp = M y, nrhs blocks of 3n in particle order; either may be NULL. */
+int ggem_get_brownian(ggem_ctx* ctx, double* y, double* disp);
+/* disp = M y with the kept y, at the context's current positions and
+ * neighbour lists: hi_ewald(M, dw, dw_mid) at the mid-point of
+ * Copss::fixman_integrate (copss.C:1730).  The result replaces the kept
+ * displacement; it is copied to disp (3n x nrhs) unless disp is NULL. */
+int ggem_displacement_apply(ggem_ctx* ctx, double* disp);
+
+/* Noise stream kept in the context: the engine of RandomGenerator
+ * (random_generator.C:43-67, one std::default_random_engine for the whole
+ * run, seeded by set_seed :96-99; copss.C:1389).  ggem_noise_seed restarts
+ * the stream; ggem_noise_skip draws and discards n_vectors vectors of 3n
+ * N(0,1) values (the restart catch-up of copss.C:1544-1555);
+ * ggem_noise_draw draws the next vector dw = sqrt(2) N(0, sqrt(dt)) over
+ * 3n values (std_random_vector + VecScale, copss.C:1565-1572), which
+ * becomes the device noise (nrhs = 1, as ggem_set_noise) and is copied to
+ * dw unless dw is NULL.  The draws run on the host (the engine is
+ * sequential); each costs O(n). */
+int ggem_noise_seed(ggem_ctx* ctx, uint64_t seed);
+int ggem_noise_skip(ggem_ctx* ctx, int64_t n_vectors);
+int ggem_noise_draw(ggem_ctx* ctx, double dt, double* dw);
+/* Device noise from a counter-based generator: nrhs vectors of 3n values
+ * sqrt(2) N(0, sqrt(dt)) (Philox4x32-10 keyed by seed, then the Box-Muller
+ * transform; values 2j and 2j + 1 of vector r come from the counter words
+ * (counter low, counter high, r, j)), written
+ * straight to the device noise.  No counterpart in the reference, whose
+ * noise is the host engine above; for throughput runs with many noise
+ * vectors. */
+int ggem_noise_philox(ggem_ctx* ctx, uint64_t seed, uint64_t counter, int32_t nrhs, double dt);
+/* A copy of the device noise (noise_nrhs blocks of 3n, particle order). */
+int ggem_get_noise(ggem_ctx* ctx, double* dw);
+
+/* Positions updated on the device, Copss::fixman_integrate /
+ * langevin_integrate (copss.C:1676-1745, :1916-1925):
+ *   x = base + c_velocity U + c_disp D + c_force f + c_noise dw
+ * with base the positions saved by ggem_save_positions (R0) when from_saved
+ * is set, else the current positions; U the velocity of the last
+ * ggem_combine_velocity / ggem_combine_velocity_fe, D the kept displacement,
+ * f the context's forces (the free-draining velocity of langevin_integrate)
+ * and dw the device noise.  Then FixPoint::check_walls (fix_point.C:27-90):
+ * wall_type 0 (slit): a periodic axis wraps once, x < min: x += L,
+ * x >= max: x -= L (counting the image crossings), and a wall axis
+ * reflects, x < min: 2 min - x, x >= max: 2 max - x; wall_type 1: a point
+ * outside the spherical cavity of cavity_radius about the origin is
+ * reflected to (2 R - |x|) x/|x|.  With track != 0 the increment (without
+ * the wrap) is added to the unwrapped positions ROUT (copss.C:1740-1743).
+ * The new positions become the context's; a neighbour build or
+ * ggem_update_positions follows before the next pair sum. */
+typedef struct ggem_move {
+  double c_velocity, c_disp, c_force, c_noise;
+  int32_t from_saved;
+  int32_t wall_type;     /* 0 slit (box faces), 1 spherical cavity */
+  double cavity_radius;
+  int32_t track;
+  int32_t pad;
+} ggem_move;
+int ggem_save_positions(ggem_ctx* ctx);
+int ggem_move_particles(ggem_ctx* ctx, const ggem_move* mv);
+/* Current positions (3n, wrapped), the unwrapped positions ROUT and the
+ * periodic image counts (PointParticle::counter); any output may be NULL. */
+int ggem_get_positions(ggem_ctx* ctx, double* xyz, double* unwrapped, int32_t* images);
+/* ROUT and image counts restart from rout (3n; NULL: the current
+ * positions, zero counts), as Copss::run reads vector_RIN (copss.C:1392-1413). */
+int ggem_set_unwrapped(ggem_ctx* ctx, const double* rout);
 
 /* ---- batched suspensions ---------------------------------------------- */
 /* Many independent suspensions in one context: the particles, cells and
@@ -298,6 +371,17 @@ int ggem_batch_lanczos_bounds(ggem_ctx* ctx, int32_t iters, uint64_t seed, doubl
  * vectors of length n first (copss.C:1544-1555). */
 int ggem_util_reference_noise(int64_t n, uint64_t seed, double dt, int64_t skip_vectors,
                               double* out);
+/* The same generator as a stream that continues across calls (one
+ * RandomGenerator for the whole run, brownian_system.h:292, copss.C:1389):
+ * ggem_util_noise_draw gives the next sqrt(2) N(0, sqrt(dt)) vector of n
+ * values (copss.C:1565-1572), ggem_util_noise_skip discards n_vectors
+ * vectors of n N(0,1) values (the restart catch-up, copss.C:1544-1555).
+ * Draw k after seeding equals ggem_util_reference_noise(n, seed, dt, k). */
+typedef struct ggem_noise_stream ggem_noise_stream;
+int ggem_util_noise_create(uint64_t seed, ggem_noise_stream** out);
+int ggem_util_noise_destroy(ggem_noise_stream* s);
+int ggem_util_noise_draw(ggem_noise_stream* s, int64_t n, double dt, double* out);
+int ggem_util_noise_skip(ggem_noise_stream* s, int64_t n, int64_t n_vectors);
 /* Random sequential addition of n points uniform in the box, rejecting any
  * pair closer than min_dist (minimum image on periodic axes);
  * std::mt19937_64(seed), uniform_real_distribution per axis x, y, z. */
@@ -338,7 +422,8 @@ int ggem_tuning_report(ggem_ctx* ctx, char* buf, int64_t cap, int64_t* len);
 
 /* ---- profiling ---------------------------------------------------------- */
 /* When enabled, every kernel launch is bracketed by HIP events on the
- * context's stream.  ggem_profile_get sums the durations per kernel name. */
+ * context's stream.  ggem_profile_get sums the durations per kernel name
+ * ("*": over all kernels). */
 int ggem_profile_enable(ggem_ctx* ctx, int32_t enable);
 int ggem_profile_reset(ggem_ctx* ctx);
 int ggem_profile_get(ggem_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);

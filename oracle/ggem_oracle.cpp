@@ -326,6 +326,38 @@ void orc_correct_position(const orc_params* p, double* x) {
   }
 }
 
+// FixPoint::check_walls (fix/fix_point.C:27-90) for one point: slit walls
+// (wall_type 0: a periodic axis wraps once each way and counts the image,
+// a wall axis reflects) or a spherical cavity of radius R about the origin
+// (wall_type 1: a point outside is reflected to pt_unit 2R - pt).
+void orc_check_walls(const orc_params* p, int wall_type, double radius, double* x, int32_t* count) {
+  if (wall_type == 0) {
+    for (int i = 0; i < 3; ++i) {
+      const double L = p->box_max[i] - p->box_min[i];
+      if (p->periodic[i]) {
+        if (x[i] < p->box_min[i]) {
+          x[i] += L;
+          count[i] -= 1;
+        }
+        if (x[i] >= p->box_max[i]) {
+          x[i] -= L;
+          count[i] += 1;
+        }
+      } else {
+        if (x[i] < p->box_min[i]) x[i] = 2. * p->box_min[i] - x[i];
+        if (x[i] >= p->box_max[i]) x[i] = 2. * p->box_max[i] - x[i];
+      }
+    }
+  } else {
+    const double nrm = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (nrm > radius) {
+      double u[3];
+      for (int i = 0; i < 3; ++i) u[i] = x[i] / nrm;
+      for (int i = 0; i < 3; ++i) x[i] = u[i] * (2. * radius) - x[i];
+    }
+  }
+}
+
 void orc_green_local_regularized(const orc_params* p, const double* x, double* G9) {
   orc_consts c;
   derive(p, &c);

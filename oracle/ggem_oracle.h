@@ -50,6 +50,8 @@ typedef struct orc_consts {
 int orc_consts_get(const orc_params* p, orc_consts* c);
 void orc_point_vector(const orc_params* p, const double* p0, const double* p1, double* d);
 void orc_correct_position(const orc_params* p, double* x);
+/* FixPoint::check_walls for one point: wall_type 0 slit, 1 spherical cavity */
+void orc_check_walls(const orc_params* p, int wall_type, double radius, double* x, int32_t* count);
 void orc_green_local_regularized(const orc_params* p, const double* x, double* G9);
 void orc_green_unbounded_smoothed(const orc_params* p, const double* x, double a_or_k, double* G9);
 double orc_smoothed_force_exp(const orc_params* p, const double* r);

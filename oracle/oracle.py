@@ -51,6 +51,7 @@ def lib():
             "orc_consts_get": (C.c_int, [P, P]),
             "orc_point_vector": (None, [P, P, P, P]),
             "orc_correct_position": (None, [P, P]),
+            "orc_check_walls": (None, [P, C.c_int, D, P, P]),
             "orc_green_local_regularized": (None, [P, P, P]),
             "orc_green_unbounded_smoothed": (None, [P, P, D, P]),
             "orc_smoothed_force_exp": (D, [P, P]),
@@ -154,6 +155,26 @@ def point_vector(cfg, p0, p1):
     d = np.zeros(3)
     lib().orc_point_vector(C.byref(p), _ptr(a), _ptr(b), _ptr(d))
     return d
+
+
+def correct_position(cfg, x):
+    """PMPeriodicBoundary::correct_position of one point (a copy)."""
+    p = make_params(cfg)
+    a = np.array(x, dtype=np.float64)
+    lib().orc_correct_position(C.byref(p), _ptr(a))
+    return a
+
+
+def check_walls(cfg, x, wall_type=0, radius=0.0, count=None):
+    """FixPoint::check_walls of the points x[m, 3]: (new points, image counts)."""
+    p = make_params(cfg)
+    a = np.array(x, dtype=np.float64).reshape(-1, 3)
+    cnt = np.zeros(a.shape, np.int32) if count is None else np.array(count, np.int32).reshape(-1, 3)
+    for i in range(a.shape[0]):
+        row, crow = a[i].copy(), cnt[i].copy()
+        lib().orc_check_walls(C.byref(p), int(wall_type), float(radius), _ptr(row), _ptr(crow))
+        a[i], cnt[i] = row, crow
+    return a, cnt
 
 
 def chebyshev_coefficients(N, da, db):

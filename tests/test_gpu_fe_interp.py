@@ -126,6 +126,20 @@ def test_argument_errors():
         ctx.close()
 
 
+def test_fe_mesh_rejects_incidence_numbers_beyond_int32():
+    """20 n_elem incidences numbered 20 e + l in int32: a 480^3 grid
+    (110.6M elements) is refused before the connectivity is read or any
+    device memory is taken for it."""
+    cfg = dict(W.WORKLOADS["beads1m"]["cfg"](), n_elem=[480, 480, 480], precision=G.FP64)
+    ctx = G.Context(cfg)
+    try:
+        with pytest.raises(G.GGEMError) as ei:
+            ctx.set_fe_mesh(1000, np.zeros((1, 20), np.int32))  # only its pointer is passed
+        assert ei.value.code == -8  # GGEM_ERR_CAPACITY
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("prec", [G.FP64, G.FP32C])
 @pytest.mark.parametrize("case", ["periodic", "walls"])
 def test_nodal_force_matches_oracle(case, prec):

@@ -157,3 +157,43 @@ def test_every_host_kernel_stub_has_device_code():
     assert stubs and dev
     missing = sorted(stubs - dev)
     assert not missing, missing[:3]
+
+
+def test_noise_stream_continues_the_engine():
+    """One engine for the run (brownian_system.h:292): draw k of a stream
+    equals the stateless reference noise with k vectors skipped, and the
+    restart catch-up skips whole vectors."""
+    n, seed, dt = 3 * 257, 123456789, 1e-3
+    s = G.NoiseStream(seed)
+    for k in range(4):
+        assert np.array_equal(s.draw(n, dt), G.reference_noise(n, seed, dt, skip_vectors=k)), k
+    s2 = G.NoiseStream(seed)
+    s2.skip(n, 3)
+    assert np.array_equal(s2.draw(n, dt), G.reference_noise(n, seed, dt, skip_vectors=3))
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of Salmon et al. (SC 2011) in numpy uint64 arithmetic:
+    ctr [m, 4], key [2] (uint32 values)."""
+    M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+    c = np.array(ctr, dtype=np.uint64).reshape(-1, 4)
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    mask = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(M0) * c[:, 0]
+        p1 = np.uint64(M1) * c[:, 2]
+        c = np.stack([((p1 >> np.uint64(32)) ^ c[:, 1] ^ k0) & mask, p1 & mask,
+                      ((p0 >> np.uint64(32)) ^ c[:, 3] ^ k1) & mask, p0 & mask], axis=1)
+        k0 = (k0 + np.uint64(W0)) & mask
+        k1 = (k1 + np.uint64(W1)) & mask
+    return c
+
+
+def test_philox_known_answers():
+    """The Random123 known-answer vectors of Philox4x32-10."""
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+           ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+           ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+            [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+    for ctr, key, want in kat:
+        assert [int(v) for v in philox4x32_10(ctr, key)[0]] == want

@@ -72,3 +72,32 @@ def test_petsc_vec_binary_layout(tmp_path):
     p.write_bytes(raw[:20])
     with pytest.raises(ValueError):
         P.read_petsc_vec(str(p))
+
+
+def test_ggem_validation_file_without_bonds():
+    """The input of the golden GGEM profiles: 3 beads, 2 bonds declared and
+    no Bonds section; the reference reads it on (polymer_chain.C:188-202)."""
+    d = P.read_particle_data(os.path.join(GOLDEN, "ggem_validation_point_particle_data.in"))
+    assert np.array_equal(d["xyz"], [[-5.0, 0, 0], [0, 0, 0], [5.0, 0, 0]])
+    assert np.array_equal(d["constant_force"], [[0.3333, 0, 0]] * 3)
+    assert d["bonds"].shape == (0, 3) and d["bonds_missing"]
+    assert d["n_chains"] == 1 and list(d["beads_per_chain"]) == [3]
+
+
+def test_short_sections_repeat_the_last_record(tmp_path):
+    """Records missing at the end of a section repeat the last one read (the
+    reference's failed extractions keep their values): the
+    PointParticle_Polymer_BD_HI input cut after its first bond, and
+    Poisson_NeumannBC, which declares 2 atoms and lists one."""
+    src = os.path.join(GOLDEN, "polymer_bd_hi_point_particle_data.in")
+    d = P.read_particle_data(src)
+    assert np.array_equal(d["bonds"][:, 1:], np.stack([np.arange(10), np.arange(1, 11)], 1))
+    txt = open(src).read()
+    cut = tmp_path / "cut.in"
+    cut.write_text(txt[:txt.index("2 1 2 3")])
+    d = P.read_particle_data(str(cut))
+    assert d["bonds"].shape == (10, 3) and not d["bonds_missing"]
+    assert np.array_equal(d["bonds"], [[0, 0, 1]] * 10)
+    d = P.read_particle_data(os.path.join(GOLDEN, "poisson_neumann_point_particle_data.in"))
+    assert np.array_equal(d["xyz"], [[0, 0, -3.9]] * 2)
+    assert np.array_equal(d["charge"], [1.0, 1.0]) and np.array_equal(d["bead_id"], [0, 0])
