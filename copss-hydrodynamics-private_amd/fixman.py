@@ -124,17 +124,23 @@ class DeviceSlitForces:
         self.params = dict(chain_beads=n_beads, k=k, b_max=b_max, walls=True, eps=eps, sigma=sigma, rcut=rcut)
 
 
-def check_walls(x, cfg):
+def check_walls(x, cfg, count=None):
     """FixPoint::check_walls, slit walls (fix/fix_point.C:33-62): on a periodic
     axis one shift each way (x < min: += L; x >= max: -= L), on a wall axis
-    reflection (x < min: 2 min - x; x >= max: 2 max - x)."""
+    reflection (x < min: 2 min - x; x >= max: 2 max - x).  count (int [n, 3],
+    PointParticle::counter) is updated in place by the periodic shifts, on
+    every call as in the reference, the Fixman mid-point's included."""
     lo = np.array(cfg["box_min"], float)
     hi = np.array(cfg["box_max"], float)
     L = hi - lo
     per = np.array(cfg["periodic"], bool)
     x = np.array(x, float).reshape(-1, 3)
-    x = np.where(per & (x < lo), x + L, x)
-    x = np.where(per & (x >= hi), x - L, x)
+    below = per & (x < lo)
+    x = np.where(below, x + L, x)
+    above = per & (x >= hi)
+    x = np.where(above, x - L, x)
+    if count is not None:
+        count += above.astype(count.dtype) - below.astype(count.dtype)
     x = np.where(~per & (x < lo), 2 * lo - x, x)
     x = np.where(~per & (x >= hi), 2 * hi - x, x)
     return x
@@ -158,6 +164,8 @@ class FixmanIntegrator:
         self.eig = None
         self.step_no = restart_step
         self.stats = {"cheb_retries": 0, "cheb_failures": 0, "relax_steps": 0}
+        self.rout = None    # unwrapped positions (ROUT), set on the first step
+        self.images = None  # periodic image counts
 
     def _bounds(self):
         lmin, lmax = self.mob.bounds()
@@ -181,6 +189,9 @@ class FixmanIntegrator:
     def step(self, x0, dw=None):
         x0 = np.asarray(x0, float).reshape(-1, 3)
         n = len(x0)
+        if self.rout is None or self.rout.shape != x0.shape:
+            self.rout = x0.copy()
+            self.images = np.zeros(x0.shape, np.int32)
         f0 = self._forces(x0)
         u0 = self.mob.mobility_apply(f0)
         if dw is None:
@@ -199,11 +210,13 @@ class FixmanIntegrator:
         if not ok:
             self.stats["cheb_failures"] += 1
             return self._relax(x0, f0)
-        xm = check_walls(x0 + 0.5 * self.dt * u0 + 0.5 * d0, self.cfg)
+        xm = check_walls(x0 + 0.5 * self.dt * u0 + 0.5 * d0, self.cfg, self.images)
         fm = self._forces(xm)
         um = self.mob.mobility_apply(fm)
         dm = self.mob.mobility_apply(y)
-        x1 = check_walls(x0 + self.dt * um + dm, self.cfg)
+        inc = self.dt * um + dm
+        x1 = check_walls(x0 + inc, self.cfg, self.images)
+        self.rout += inc  # copss.C:1740-1743
         self.step_no += 1
         return x1
 
@@ -214,7 +227,9 @@ class FixmanIntegrator:
             if r > 0:
                 f = self._forces(x)
             dw = self._draw(len(x))
-            x = check_walls(x + self.dt * f + dw, self.cfg)
+            inc = self.dt * f + dw
+            x = check_walls(x + inc, self.cfg, self.images)
+            self.rout += inc  # copss.C:1918-1922
             self.step_no += 1
             self.stats["relax_steps"] += 1
         return x
