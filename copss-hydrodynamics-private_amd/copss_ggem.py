@@ -32,6 +32,7 @@ CHEB_REFERENCE, CHEB_FIXED = 0, 1
 CHEB_KEEP = 0x100
 WALL_SLIT, WALL_SPHERE = 0, 1
 COMPAT_ELEM_HYSTERESIS = 1
+COMPAT_STALE_LISTS = 2
 
 ERRORS = {
     -1: "GGEM_ERR_INVALID_ARG", -2: "GGEM_ERR_OUT_OF_DOMAIN", -3: "GGEM_ERR_INVALID_ELEM",
@@ -54,7 +55,7 @@ EXPORTS = [
     "ggem_get_brownian", "ggem_displacement_apply", "ggem_noise_seed", "ggem_noise_skip", "ggem_noise_draw",
     "ggem_noise_philox", "ggem_save_positions", "ggem_move_particles", "ggem_get_positions",
     "ggem_set_unwrapped", "ggem_get_noise", "ggem_util_noise_create", "ggem_util_noise_destroy",
-    "ggem_util_noise_draw", "ggem_util_noise_skip",
+    "ggem_util_noise_draw", "ggem_util_noise_skip", "ggem_update_positions",
 ]
 
 
@@ -172,6 +173,7 @@ def lib():
             "ggem_get_positions": [P, P, P, P],
             "ggem_set_unwrapped": [P, P],
             "ggem_get_noise": [P, P],
+            "ggem_update_positions": [P, P, P],
             "ggem_util_noise_create": [U64, C.POINTER(C.c_void_p)],
             "ggem_util_noise_destroy": [P],
             "ggem_util_noise_draw": [P, I64, D, P],
@@ -378,10 +380,20 @@ class Context:
     def build_neighbors(self):
         self._c(lib().ggem_build_neighbors(self._h))
 
-    def reinit(self, xyz, f=None):
-        """PointMesh<3>::reinit with a neighbour-list update."""
-        self.set_particles(xyz, f)
-        self.build_neighbors()
+    def reinit(self, xyz, f=None, update_lists=True):
+        """PointMesh<3>::reinit(update_lists): with a neighbour-list update,
+        or new positions on the kept lists (ggem_update_positions)."""
+        if update_lists:
+            self.set_particles(xyz, f)
+            self.build_neighbors()
+        else:
+            self.update_positions(xyz, f)
+
+    def update_positions(self, xyz=None, f=None):
+        """New positions (None: the device's current ones) on the kept lists."""
+        x = None if xyz is None else _f64(xyz, (-1, 3))
+        ff = None if f is None else _f64(f, (-1, 3))
+        self._c(lib().ggem_update_positions(self._h, _p(x), _p(ff)))
 
     def elem_ids(self):
         e = np.zeros(self.n, np.int32)

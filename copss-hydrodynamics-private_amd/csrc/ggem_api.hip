@@ -402,6 +402,12 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->noise);
   f(ctx->keep_y);
   f(ctx->keep_d);
+  f(ctx->keep_off);
+  f(ctx->keep_ids);
+  f(ctx->keep_off2);
+  f(ctx->keep_ids2);
+  f(ctx->keep_cnt);
+  f(ctx->stale_slot);
   f(ctx->x_saved);
   f(ctx->rout);
   f(ctx->images);
@@ -502,7 +508,8 @@ int create_impl(const ggem_params* params, int32_t nsys, ggem_ctx** out) {
     if (rc) return fail(nullptr, rc, nsys > 1 ? "system " + std::to_string(k) + ": " + why : why);
     if (k > 0 && (pk->mu != params->mu || pk->alpha != params->alpha || cs[k].ksi != cs[0].ksi ||
                   pk->point_type != params->point_type || pk->precision != params->precision ||
-                  pk->device != params->device || pk->compat_flags != params->compat_flags))
+                  pk->device != params->device || pk->compat_flags != params->compat_flags ||
+                  (pk->compat_flags & GGEM_COMPAT_STALE_LISTS)))
       return fail(nullptr, GGEM_ERR_INVALID_ARG,
                   "system " + std::to_string(k) + ": a batch shares mu, alpha, xi, point type, precision, "
                   "flags and device");
@@ -789,6 +796,7 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
   ctx->u_out_valid = false;
   ctx->n = n;
   ctx->built = false;
+  ctx->slots_current = false;
   ctx->elist_valid = false;
   ctx->coef64_valid = false;
   ctx->box_built = false;
@@ -801,6 +809,44 @@ int set_particles_impl(ggem_ctx* ctx, int64_t n, const double* xyz, const double
     if (rc) return rc;
   }
   return sync(ctx);
+}
+
+}  // namespace
+
+namespace {
+
+// GGEM_COMPAT_STALE_LISTS after a build: a particle whose new list is
+// empty keeps its previous one (point_particle.h:189-194); then every
+// particle's effective list is stored by particle id in the reference's
+// order (the lists the next build keeps, and the CSR export).
+int keep_lists(ggem_ctx* ctx) {
+  const int64_t n = ctx->n;
+  int rc = ensure(ctx, &ctx->stale_slot, &ctx->stale_slot_cap, size_t(std::max<int64_t>(n, 1)));
+  if (rc) return rc;
+  const bool have = ctx->keep_n == n && ctx->keep_off != nullptr;
+  hipError_t e = launch_stale_patch(ctx, have);
+  if (e != hipSuccess) return hip_fail(ctx, e, "stale lists");
+  int32_t flag = 0;
+  int64_t pairs = 0;
+  GGEM_HIP(ctx, hipMemcpyAsync(&flag, ctx->flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  GGEM_HIP(ctx, hipMemcpyAsync(&pairs, ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  if ((rc = sync(ctx))) return rc;
+  ctx->stale_lists = flag != 0;
+  ctx->pairs = pairs;
+  if ((rc = ensure(ctx, &ctx->keep_cnt, &ctx->keep_cnt_cap, size_t(std::max<int64_t>(n, 1))))) return rc;
+  if ((rc = ensure(ctx, &ctx->keep_off2, &ctx->keep_off2_cap, size_t(n + 1)))) return rc;
+  if ((rc = ensure(ctx, &ctx->keep_ids2, &ctx->keep_ids2_cap, size_t(std::max<int64_t>(pairs, 1))))) return rc;
+  if ((rc = grow_scan64(ctx, n + 1))) return rc;
+  if ((e = launch_counts_by_particle(ctx, ctx->keep_cnt)) != hipSuccess ||
+      (e = launch_scan64(ctx, ctx->keep_cnt, n, ctx->keep_off2)) != hipSuccess ||
+      (e = launch_fill_keep(ctx, ctx->keep_off, ctx->keep_ids, ctx->keep_off2, ctx->keep_ids2)) != hipSuccess)
+    return hip_fail(ctx, e, "kept lists");
+  std::swap(ctx->keep_off, ctx->keep_off2);
+  std::swap(ctx->keep_off_cap, ctx->keep_off2_cap);
+  std::swap(ctx->keep_ids, ctx->keep_ids2);
+  std::swap(ctx->keep_ids_cap, ctx->keep_ids2_cap);
+  ctx->keep_n = n;
+  return GGEM_OK;
 }
 
 }  // namespace
@@ -858,7 +904,44 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   ctx->ell_valid = false;
   ctx->elem_valid = true;
   ctx->built = true;
+  ctx->slots_current = true;
+  ctx->stale_pos = false;
+  ctx->stale_lists = false;
+  if ((ctx->p.compat_flags & GGEM_COMPAT_STALE_LISTS) && (rc = keep_lists(ctx))) {
+    ctx->built = false;
+    return rc;
+  }
   return GGEM_OK;
+}
+
+// PointMesh<3>::reinit(false) (point_mesh.C:245-289): new positions (and
+// forces, zero when NULL: the reinit zeroes them, :249-251) with the lists,
+// the element ids and the element lists of the last build kept; every pair
+// then takes the explicit minimum image, as reinit_neighbor_vector
+// recomputes r_ij = point_vector(x_i, x_j) (:220-241).  xyz NULL: the
+// context's current positions (after ggem_move_particles).
+int ggem_update_positions(ggem_ctx* ctx, const double* xyz, const double* f) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (!ctx->built) return fail(ctx, GGEM_ERR_NOT_READY, "no lists: call ggem_build_neighbors first");
+  int rc = single_only(ctx, "ggem_update_positions");
+  if (rc) return rc;
+  GGEM_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = sizeof(double) * 3 * size_t(ctx->n);
+  if (bytes) {
+    if (xyz && (rc = h2d(ctx, ctx->x_in, xyz, bytes))) return rc;
+    if (f) rc = h2d(ctx, ctx->f_in, f, bytes);
+    else GGEM_HIP(ctx, hipMemsetAsync(ctx->f_in, 0, bytes, ctx->stream));
+    if (rc) return rc;
+  }
+  hipError_t e = launch_update_slots(ctx);
+  if (e == hipSuccess) e = launch_gather_forces(ctx);
+  if (e != hipSuccess) return hip_fail(ctx, e, "update positions");
+  ctx->slots_current = true;
+  ctx->stale_pos = true;
+  ctx->coef64_valid = false;
+  ctx->box_built = false;
+  ctx->u_out_valid = false;
+  return xyz || f ? sync(ctx) : GGEM_OK;  // the host arrays may be reused on return
 }
 
 int ggem_get_elem_ids(ggem_ctx* ctx, int32_t* elem) {
@@ -883,6 +966,11 @@ int ggem_get_neighbor_csr(ggem_ctx* ctx, int64_t* offsets, int32_t* ids) {
   if (!offsets) return fail(ctx, GGEM_ERR_INVALID_ARG, "null offsets");
   Pools& P = pools(ctx);
   const int64_t n = ctx->n;
+  if ((ctx->p.compat_flags & GGEM_COMPAT_STALE_LISTS) && ctx->keep_n == n) {  // the effective lists
+    if ((rc = d2h(ctx, offsets, ctx->keep_off, sizeof(int64_t) * (n + 1)))) return rc;
+    if (ids && offsets[n] > 0) return d2h(ctx, ids, ctx->keep_ids, sizeof(int32_t) * offsets[n]);
+    return GGEM_OK;
+  }
   if ((rc = ensure(ctx, &P.offs, &P.offs_cap, size_t(n + 1)))) return rc;
   hipError_t e = launch_count_pairs(ctx, P.offs + 1);
   if (e != hipSuccess) return hip_fail(ctx, e, "count");

@@ -61,6 +61,8 @@ int ensure_ell(ggem_ctx* ctx);
 inline int check_ready(ggem_ctx* ctx) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   if (!ctx->built) return fail(ctx, GGEM_ERR_NOT_READY, "call ggem_build_neighbors after setting particles");
+  if (!ctx->slots_current)
+    return fail(ctx, GGEM_ERR_NOT_READY, "the particles moved: call ggem_build_neighbors or ggem_update_positions");
   return GGEM_OK;
 }
 

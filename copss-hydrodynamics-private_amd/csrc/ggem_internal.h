@@ -154,6 +154,31 @@ struct ggem_ctx {
   int64_t* pairs_d = nullptr;     // pair count reduction
   int64_t pairs = -1;
   bool built = false;
+  // the slot-ordered positions (xs, x4, pc32) hold the particles' current
+  // positions; cleared by ggem_move_particles until a build or a
+  // ggem_update_positions copies them in
+  bool slots_current = false;
+  // positions changed since the lists were built (ggem_update_positions,
+  // PointMesh::reinit(false)): the lists' periodic shifts no longer apply,
+  // every pair takes the explicit minimum image
+  bool stale_pos = false;
+  // GGEM_COMPAT_STALE_LISTS: some particle kept its previous list this
+  // build (point_particle.h:189-194), so the lists are not stencil lists
+  bool stale_lists = false;
+  // GGEM_COMPAT_STALE_LISTS: each particle's list of the last build as CSR
+  // by particle id in the reference's order (the list it keeps when its new
+  // one is empty), and the slots that kept theirs
+  int64_t* keep_off = nullptr;   // [keep_n + 1]
+  int32_t* keep_ids = nullptr;
+  size_t keep_off_cap = 0, keep_ids_cap = 0;
+  int64_t keep_n = -1;            // particles of the kept lists (-1: none)
+  int64_t* keep_off2 = nullptr;   // the next build's lists, swapped in when complete
+  int32_t* keep_ids2 = nullptr;
+  size_t keep_off2_cap = 0, keep_ids2_cap = 0;
+  int64_t* keep_cnt = nullptr;    // [n] list lengths by particle
+  size_t keep_cnt_cap = 0;
+  uint8_t* stale_slot = nullptr;  // [n] 1: the slot's list is its kept one
+  size_t stale_slot_cap = 0;
   bool ell_valid = false;  // the sliced lists are filled (on demand: the staged FP32C sums do not read them)
 
   // particle neighbour lists, sliced ELL: slice q holds targets 64q..64q+63,
@@ -277,6 +302,11 @@ struct ggem_ctx {
 
 namespace ggem {
 
+// Every pair takes the explicit minimum-image test instead of the lists'
+// periodic shifts: the stencil reaches half a box, the positions moved
+// since the build, or some lists are kept ones.
+inline bool explicit_minimg(const ggem_ctx* c) { return c->exact_minimg || c->stale_pos || c->stale_lists; }
+
 // error helpers
 int fail(ggem_ctx* ctx, int code, const std::string& msg);
 int hip_fail(ggem_ctx* ctx, hipError_t e, const char* what);
@@ -328,6 +358,15 @@ hipError_t launch_fill_csr(ggem_ctx* ctx, const int64_t* offsets_d, int32_t* ids
 hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t* offsets_d,
                                  int32_t* ids_d, int pass);
 hipError_t launch_locate(ggem_ctx* ctx, int64_t m, const double* pts, int32_t* elem);
+// slot positions and FP32C records from x_in, lists kept (PointMesh::reinit(false))
+hipError_t launch_update_slots(ggem_ctx* ctx);
+// GGEM_COMPAT_STALE_LISTS: slots with an empty new list take their kept
+// one's length (stale_slot, flag), then the slice lengths and pair total
+hipError_t launch_stale_patch(ggem_ctx* ctx, bool have_kept);
+hipError_t launch_counts_by_particle(ggem_ctx* ctx, int64_t* out);
+// every particle's effective list (kept or new) in the reference's order
+hipError_t launch_fill_keep(ggem_ctx* ctx, const int64_t* old_off, const int32_t* old_ids, const int64_t* off,
+                            int32_t* ids);
 
 // ---- launchers (kernels_fe.hip) ----
 // out[3p + d] = HEX20 interpolant of node_u[3 node + d] at particle p in its element

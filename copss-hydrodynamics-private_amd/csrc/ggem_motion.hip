@@ -287,8 +287,9 @@ int ggem_move_particles(ggem_ctx* ctx, const ggem_move* mv) {
                        m.cn != 0.0 ? ctx->noise : nullptr, ctx->x_in, ctx->rout, ctx->images);
     GGEM_HIP(ctx, hipGetLastError());
   }
-  // the positions moved: lists, tensors and velocities describe the old ones
-  ctx->built = false;
+  // the positions moved: the slot copies, tensors and velocities describe
+  // the old ones (the lists stay for ggem_update_positions)
+  ctx->slots_current = false;
   ctx->u_out_valid = false;
   return GGEM_OK;
 }

@@ -296,6 +296,12 @@ struct BrickArgs {
   const int32_t* sys_slot;
   int32_t cells_cap;         // box cells the LDS tables hold
   int32_t* recs;             // [n_bricks] particles in the box of each brick the count pass staged
+  // slots whose list is the one their particle kept (GGEM_COMPAT_STALE_LISTS)
+  const uint8_t* stale;      // [n] or null
+  const int64_t* keep_off;   // kept lists by particle (reference order)
+  const int32_t* keep_ids;
+  const int32_t* slot_of;
+  const int32_t* perm;
 };
 
 // grid, rows and box of the brick whose first slot is sb
@@ -416,6 +422,12 @@ __global__ void __launch_bounds__(kBrickThreads) k_brick_fill_ell(BrickArgs a, c
     cell_ijk(v.g, a.slot_cell[s] - v.coff, ci, cj, ck);
     uint32_t* out = ell + slice_off[s >> 6] + (s & 63);
     int64_t k = 0;
+    if (a.stale && a.stale[s]) {  // the particle's kept list, slots by the current order
+      const int32_t p = a.perm[s];
+      for (int64_t q = a.keep_off[p], e = a.keep_off[p + 1]; q < e; ++q, ++k)
+        out[k * kSlice] = uint32_t(a.slot_of[a.keep_ids[q]]);
+      continue;
+    }
     auto emit = [&](int32_t p, int32_t q, uint32_t code) {
       const uint32_t hi = code << kSlotBits;
       for (int32_t t = p; t < q; ++t) {
@@ -666,6 +678,79 @@ __global__ void k_elem_point(Grid g, const StencilRow* __restrict__ rows, int nr
   }
 }
 
+// PointMesh::reinit(false) (point_mesh.C:245-289 without the list
+// update): the slot positions take the particles' new positions; the FP32C
+// records are relative to the centre of the cell now holding the particle
+// (floor, clamped), the lists, elements and slot order stay.
+__global__ void k_update_slots(Grid g, int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ x,
+                               double* __restrict__ xs, double4* __restrict__ x4, float4* __restrict__ pc) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int64_t p = perm[s];
+  const double v[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  int c[3];
+  float o[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double q = floor((v[d] - g.bmin[d]) / g.h[d]);
+    q = q < 0.0 ? 0.0 : (q > double(g.n[d] - 1) ? double(g.n[d] - 1) : q);
+    c[d] = int(q);
+    o[d] = float(v[d] - (g.bmin[d] + (q + 0.5) * g.h[d]));
+    xs[d * n + s] = v[d];
+  }
+  x4[s] = make_double4(v[0], v[1], v[2], 0.0);
+  const uint32_t packed = uint32_t(c[0]) | (uint32_t(c[1]) << kCellBitsX) | (uint32_t(c[2]) << (kCellBitsX + kCellBitsY));
+  pc[s] = make_float4(o[0], o[1], o[2], __uint_as_float(packed));
+}
+
+// PointParticle::set_neighbor_list (point_particle.h:189-194): a slot whose
+// new list is empty keeps its particle's previous list (keep_off null: no
+// previous lists).
+__global__ void k_stale_patch(int64_t n, const int32_t* __restrict__ perm, const int64_t* __restrict__ keep_off,
+                              int32_t* __restrict__ cnt, uint8_t* __restrict__ stale, int32_t* __restrict__ flag) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint8_t st = 0;
+  if (keep_off && cnt[s] == 0) {
+    const int64_t p = perm[s];
+    const int64_t m = keep_off[p + 1] - keep_off[p];
+    if (m > 0) {
+      cnt[s] = int32_t(m);
+      st = 1;
+      atomicOr(flag, 1);
+    }
+  }
+  stale[s] = st;
+}
+
+// list length of every particle, by particle id
+__global__ void k_counts_by_particle(int64_t n, const int32_t* __restrict__ slot_of, const int32_t* __restrict__ cnt,
+                                     int64_t* __restrict__ out) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = cnt[slot_of[p]];
+}
+
+// every particle's list in the reference's order: the kept one, or the
+// stencil list of this build
+__global__ void k_fill_keep(Grid g, const StencilRow* __restrict__ rows, int nrows,
+                            const int32_t* __restrict__ cell_start, const int32_t* __restrict__ slot_cell,
+                            const int32_t* __restrict__ perm, const int32_t* __restrict__ elem, int64_t n,
+                            const uint8_t* __restrict__ stale, const int64_t* __restrict__ old_off,
+                            const int32_t* __restrict__ old_ids, const int64_t* __restrict__ off,
+                            int32_t* __restrict__ ids) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int32_t p = perm[s];
+  if (stale[s]) {
+    for (int64_t q = old_off[p], w = off[p]; q < old_off[p + 1]; ++q, ++w) ids[w] = old_ids[q];
+    return;
+  }
+  const int64_t c = slot_cell[s];
+  int ci, cj, ck;
+  cell_ijk(g, c, ci, cj, ck);
+  emit_list(g, rows, nrows, cell_start, ci, cj, ck, c, int32_t(s), perm, elem, ids + off[p]);
+}
+
 __global__ void k_locate(Grid g, int64_t m, const double* __restrict__ pts, int32_t* __restrict__ elem) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= m) return;
@@ -753,6 +838,11 @@ BrickArgs brick_args(const ggem_ctx* ctx) {
   for (int k = 0; k < ctx->nsys && ctx->nsys > 1; ++k) cap = std::max(cap, cells(ctx->sys_h[k].reach));
   a.cells_cap = std::min(cap, kBoxMaxCells);
   a.recs = ctx->brick_recs;
+  a.stale = ctx->stale_lists ? ctx->stale_slot : nullptr;
+  a.keep_off = ctx->keep_off;
+  a.keep_ids = ctx->keep_ids;
+  a.slot_of = ctx->slot_of;
+  a.perm = ctx->perm;
   return a;
 }
 
@@ -851,6 +941,43 @@ hipError_t launch_elem_point_csr(ggem_ctx* ctx, int64_t e0, int64_t ne, int64_t*
   hipLaunchKernelGGL(k_elem_point, dim3(blocks_for(ne)), dim3(kBlock), 0, ctx->stream, ctx->grid,
                      ctx->rows_d, ctx->n_rows, ctx->cell_start, ctx->perm, ctx->elem, e0, ne, offsets_d, ids_d,
                      pass);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_slots(ggem_ctx* ctx) {
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "update_slots");
+  hipLaunchKernelGGL(k_update_slots, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid, ctx->n,
+                     ctx->perm, ctx->x_in, ctx->xs, ctx->x4, ctx->pc32);
+  return hipGetLastError();
+}
+
+hipError_t launch_stale_patch(ggem_ctx* ctx, bool have_kept) {
+  hipError_t e = hipMemsetAsync(ctx->flag, 0, sizeof(int32_t), ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(ctx->pairs_d, 0, sizeof(int64_t), ctx->stream);
+  if (e != hipSuccess || ctx->n == 0) return e;
+  ProfScope ps(ctx, "stale_lists");
+  hipLaunchKernelGGL(k_stale_patch, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n, ctx->perm,
+                     have_kept ? ctx->keep_off : nullptr, ctx->nbr_cnt, ctx->stale_slot, ctx->flag);
+  hipLaunchKernelGGL(k_slice_len, dim3(blocks_for(ctx->n, kSliceBlock)), dim3(kSliceBlock), 0, ctx->stream,
+                     ctx->nbr_cnt, ctx->n, ctx->slice_len, reinterpret_cast<unsigned long long*>(ctx->pairs_d));
+  return hipGetLastError();
+}
+
+hipError_t launch_counts_by_particle(ggem_ctx* ctx, int64_t* out) {
+  if (ctx->n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_counts_by_particle, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->n,
+                     ctx->slot_of, ctx->nbr_cnt, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_keep(ggem_ctx* ctx, const int64_t* old_off, const int32_t* old_ids, const int64_t* off,
+                            int32_t* ids) {
+  if (ctx->n == 0) return hipSuccess;
+  ProfScope ps(ctx, "stale_lists");
+  hipLaunchKernelGGL(k_fill_keep, dim3(blocks_for(ctx->n)), dim3(kBlock), 0, ctx->stream, ctx->grid, ctx->rows_d,
+                     ctx->n_rows, ctx->cell_start, ctx->slot_cell, ctx->perm, ctx->elem, ctx->n, ctx->stale_slot,
+                     old_off, old_ids, off, ids);
   return hipGetLastError();
 }
 

@@ -373,6 +373,10 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
         di += (int(code & 1u) - int((code >> 1) & 1u)) * nx;
         dj += (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * ny;
         dk += (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * nz;
+      } else {  // the nearest image of the cell difference, exact in integers
+        if (px) di = 2 * di > nx ? di - nx : (2 * di < -nx ? di + nx : di);
+        if (py) dj = 2 * dj > ny ? dj - ny : (2 * dj < -ny ? dj + ny : dj);
+        if (pz) dk = 2 * dk > nz ? dk - nz : (2 * dk < -nz ? dk + nz : dk);
       }
       dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
       dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
@@ -1031,7 +1035,7 @@ PairArgs make_args(const ggem_ctx* ctx) {
   a.sys_slot = ctx->sys_slot;
   a.pc32 = ctx->pc32;
   a.xcd = 1;
-  a.exact = ctx->exact_minimg;
+  a.exact = explicit_minimg(ctx);
   a.tab = ctx->tab32;
   a.ct = ctx->tab;
   return a;
@@ -1146,7 +1150,7 @@ hipError_t launch_t(ggem_ctx* ctx, PairArgs a, int c) {
 }
 
 hipError_t launch_t_any(ggem_ctx* ctx, const PairArgs& a, int c) {
-  const bool ex = ctx->exact_minimg != 0, bt = ctx->nsys > 1;
+  const bool ex = explicit_minimg(ctx), bt = ctx->nsys > 1;
   if (bt) return ex ? launch_t<true, true>(ctx, a, c) : launch_t<false, true>(ctx, a, c);
   return ex ? launch_t<true, false>(ctx, a, c) : launch_t<false, false>(ctx, a, c);
 }
@@ -1163,7 +1167,7 @@ hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision) {
   return run(c);
 }
 
-bool box_supported(const ggem_ctx* ctx) { return ctx->n > 0 && !ctx->exact_minimg; }
+bool box_supported(const ggem_ctx* ctx) { return ctx->n > 0 && !explicit_minimg(ctx); }
 
 hipError_t launch_box_count(ggem_ctx* ctx) {
   ProfScope ps(ctx, "box_count");

@@ -319,7 +319,7 @@ template <class T, int NQ1, int WAVES>
 void launch_spk(ggem_ctx* ctx, const SpreadArgs<T>& a, const double* IF_d, int n_u, double* rho, double* Fe) {
   constexpr int KE = WAVES * (64 / NQ1);  // elements per block
   const unsigned nb = unsigned((ctx->grid.n_elem + KE - 1) / KE);
-  if (ctx->exact_minimg)
+  if (explicit_minimg(ctx))
     hipLaunchKernelGGL((k_spread<T, NQ1, true, WAVES>), dim3(nb), dim3(WAVES * 64), 0, ctx->stream, a, IF_d, n_u,
                        rho, Fe);
   else

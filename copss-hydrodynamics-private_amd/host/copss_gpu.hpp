@@ -48,30 +48,40 @@ class PointMesh {
   ggem_ctx* ctx() const { return ctx_; }
 
   // Particle store (the reference's PointParticle objects): positions and
-  // forces, interleaved [3*i + j].
+  // forces, interleaved [3*i + j].  They reach the device at the next
+  // reinit, which decides whether the lists are rebuilt.
   void set_particles(const std::vector<double>& xyz, const std::vector<double>& f) {
     if (xyz.size() % 3 != 0 || (!f.empty() && f.size() != xyz.size()))
       throw std::runtime_error("copss_ggem: particle arrays must hold 3 values per particle");
+    if (xyz.size() / 3 != n_) lists_ = false;  // a new particle set has no lists to keep
     xyz_ = xyz;
     n_ = xyz.size() / 3;
     force_ = f.empty() ? std::vector<double>(xyz.size(), 0.0) : f;
-    check(ggem_set_particles(ctx_, int64_t(n_), xyz_.data(), force_.data()), ctx_);
+    pending_ = true;
     ++version_;
   }
   void set_forces(const std::vector<double>& f) {
     if (f.size() != 3 * n_) throw std::runtime_error("copss_ggem: force array size");
     force_ = f;
-    check(ggem_set_forces(ctx_, force_.data()), ctx_);
+    if (!pending_) check(ggem_set_forces(ctx_, force_.data()), ctx_);
     ++version_;
   }
 
   // PointMesh<3>::reinit(bool& neighbor_list_update_flag) (point_mesh.C:245-289).
-  // The lists are rebuilt when the flag is set (it is then cleared, as the
-  // reference does); otherwise they are rebuilt too, since the GPU build is
-  // cheaper than keeping stale cell slots.
+  // With the flag set the lists are rebuilt and the flag cleared, as the
+  // reference does; without it the lists, element ids and element lists of
+  // the last build are kept and only the positions (and forces) change
+  // (ggem_update_positions).  The first reinit of a particle set builds.
   void reinit(bool& neighbor_list_update_flag) {
-    check(ggem_build_neighbors(ctx_), ctx_);
-    neighbor_list_update_flag = false;
+    if (neighbor_list_update_flag || !lists_) {
+      if (pending_) check(ggem_set_particles(ctx_, int64_t(n_), xyz_.data(), force_.data()), ctx_);
+      check(ggem_build_neighbors(ctx_), ctx_);
+      neighbor_list_update_flag = false;
+      lists_ = true;
+    } else {
+      check(ggem_update_positions(ctx_, xyz_.data(), force_.data()), ctx_);
+    }
+    pending_ = false;
     ++version_;
   }
 
@@ -115,6 +125,8 @@ class PointMesh {
   std::size_t n_ = 0;
   std::vector<double> xyz_, force_;
   uint64_t version_ = 0;
+  bool pending_ = false;  // positions / forces not yet on the device
+  bool lists_ = false;    // the device holds lists of this particle set
 };
 
 class GGEMStokes {

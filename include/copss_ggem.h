@@ -51,6 +51,11 @@ enum { GGEM_FP64 = 0, GGEM_FP32C = 1 };
 /* compat_flags */
 #define GGEM_COMPAT_ELEM_HYSTERESIS 1u /* keep a particle's element while it still
                                           contains it (point_mesh.C:589-596)       */
+#define GGEM_COMPAT_STALE_LISTS 2u     /* a particle whose new neighbour list is
+                                          empty keeps its previous one
+                                          (PointParticle::set_neighbor_list,
+                                          point_particle.h:189-194); plain
+                                          contexts only                          */
 
 /*
  * Reference sources of each field: viscosity_0 / alpha / br0 / ibm_beta /
@@ -111,10 +116,18 @@ int ggem_set_particles(ggem_ctx* ctx, int64_t n, const double* xyz, const double
 int ggem_set_forces(ggem_ctx* ctx, const double* f);
 
 /* ---- (a) neighbour build ---------------------------------------------- */
-/* PointMesh<3>::reinit(bool& update_flag) (point_mesh.h:387,
- * point_mesh.C:245-289): bins particles into elements, builds the
- * element-point lists and the particle lists.  Always rebuilds. */
+/* PointMesh<3>::reinit(bool& update_flag) with update_flag true
+ * (point_mesh.h:387, point_mesh.C:245-289): bins particles into elements,
+ * builds the element-point lists and the particle lists. */
 int ggem_build_neighbors(ggem_ctx* ctx);
+/* PointMesh<3>::reinit(bool& update_flag) with update_flag false: new
+ * positions (3n, particle order; NULL: the context's current ones, after
+ * ggem_move_particles) and forces (NULL: zero, as the reinit zeroes them,
+ * point_mesh.C:249-251); the particle lists, element ids and element lists
+ * of the last build are kept and every pair displacement is recomputed from
+ * the new positions with the explicit minimum image
+ * (reinit_neighbor_vector, point_mesh.C:220-241).  Plain contexts only. */
+int ggem_update_positions(ggem_ctx* ctx, const double* xyz, const double* f);
 /* Element of each particle: PointParticle::elem_id after
  * build_elem_point_containing_list (point_mesh.C:573-648). */
 int ggem_get_elem_ids(ggem_ctx* ctx, int32_t* elem /* n */);

@@ -403,6 +403,18 @@ const int32_t* orc_mesh_elem_elem(orc_mesh* m, int64_t e) { return m->ee_ids.dat
 // build_elem_point_containing_list -> build_elem_point_neighbor_list ->
 // build_point_point_neighbor_list.  Forces are set from f (the reference
 // zeroes them at :249-251 and the fixes refill them).
+// PointMesh::reinit(false) (point_mesh.C:245-289 without the list
+// update): new positions and forces (zero when null), the lists and element
+// ids kept; every r_ij is formed from the new positions when it is used
+// (reinit_neighbor_vector, :220-241, point_vector).
+int orc_mesh_update_positions(orc_mesh* m, const double* xyz, const double* f) {
+  if (m->n == 0 || m->en_off.empty()) return -1;
+  m->x.assign(xyz, xyz + 3 * m->n);
+  if (f) m->f.assign(f, f + 3 * m->n);
+  else m->f.assign(3 * m->n, 0.0);
+  return 0;
+}
+
 int orc_mesh_reinit(orc_mesh* m, int64_t n, const double* xyz, const double* f) {
   if (!m->ee_built) orc_mesh_build_elem_elem(m, m->n_elem <= 20000 ? 0 : 1);
   if (m->n != n) {

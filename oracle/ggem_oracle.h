@@ -66,6 +66,8 @@ int64_t orc_mesh_elem_elem_len(orc_mesh* m, int64_t e);
 const int32_t* orc_mesh_elem_elem(orc_mesh* m, int64_t e);
 /* full reinit with positions (3n interleaved) and forces (3n interleaved) */
 int orc_mesh_reinit(orc_mesh* m, int64_t n, const double* xyz, const double* f);
+/* PointMesh::reinit(false): new positions and forces, lists and elements kept */
+int orc_mesh_update_positions(orc_mesh* m, const double* xyz, const double* f);
 int orc_mesh_set_forces(orc_mesh* m, const double* f);
 int32_t orc_mesh_elem_of(orc_mesh* m, int64_t i);
 int64_t orc_mesh_elem_point_len(orc_mesh* m, int64_t e);

@@ -62,6 +62,7 @@ def lib():
             "orc_mesh_elem_elem": (C.POINTER(C.c_int32), [P, I64]),
             "orc_mesh_reinit": (C.c_int, [P, I64, P, P]),
             "orc_mesh_set_forces": (C.c_int, [P, P]),
+            "orc_mesh_update_positions": (C.c_int, [P, P, P]),
             "orc_mesh_elem_of": (I32, [P, I64]),
             "orc_mesh_elem_point_len": (I64, [P, I64]),
             "orc_mesh_elem_point": (C.POINTER(C.c_int32), [P, I64]),
@@ -223,6 +224,13 @@ class Mesh:
         rc = lib().orc_mesh_reinit(self._h, self.n, _ptr(self._x), _ptr(ff))
         if rc != 0:
             raise ValueError("particle out of domain (rc=%d)" % rc)
+
+    def update_positions(self, xyz, f=None):
+        """PointMesh::reinit(false): new positions, the lists kept."""
+        self._x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        ff = None if f is None else np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3)
+        if lib().orc_mesh_update_positions(self._h, _ptr(self._x), _ptr(ff)) != 0:
+            raise ValueError("no lists to keep")
 
     def set_forces(self, f):
         ff = np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 3)

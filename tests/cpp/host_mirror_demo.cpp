@@ -115,7 +115,34 @@ int main() {
                 bs.last_info().terms, lo, hi);
     print_vec("b100_dw", dw);
     print_vec("b100_y", y);
-    print_vec("b100_disp", disp, true);
+    print_vec("b100_disp", disp);
+    // PointMesh::reinit(false): moved positions on the kept lists and elements
+    std::vector<int32_t> list0 = pm.neighbor_list(0), elem0 = pm.elem_ids();
+    std::vector<double> x2(x);
+    for (size_t i = 0; i < x2.size(); ++i) {
+      x2[i] += 0.37 * std::sin(1.3 * double(i));
+      if (x2[i] < 0) x2[i] += 40;  // correct_position's single shift
+      if (x2[i] >= 40) x2[i] -= 40;
+    }
+    pm.set_particles(x2, f);
+    bool keep = false;
+    pm.reinit(keep);
+    std::vector<double> u2;
+    for (size_t b = 0; b < 100; ++b) {
+      Point u = ggem.local_solution_bead(&pm, b, "regularized");
+      u2.insert(u2.end(), u.begin(), u.end());
+    }
+    print_vec("b100_x2", x2);
+    print_vec("b100_reinit_false_u", u2);
+    std::printf("\"b100_lists_kept\": %d,\n", (pm.neighbor_list(0) == list0 && pm.elem_ids() == elem0) ? 1 : 0);
+    bool rebuild = true;
+    pm.reinit(rebuild);
+    std::vector<double> u3;
+    for (size_t b = 0; b < 100; ++b) {
+      Point u = ggem.local_solution_bead(&pm, b, "regularized");
+      u3.insert(u3.end(), u.begin(), u.end());
+    }
+    print_vec("b100_reinit_true_u", u3, true);
   }
   std::printf("}\n");
   return 0;

@@ -63,3 +63,22 @@ def test_b100_velocity_spread_brownian(demo):
     assert ch["converged"] == io.converged and ch["terms"] == io.iterations
     assert rel_l2(np.array(demo["b100_y"]).reshape(-1, 3), yo) < 1e-12
     assert rel_l2(np.array(demo["b100_disp"]).reshape(-1, 3), do) < 1e-12
+
+
+def test_b100_reinit_keeps_lists_without_the_flag(demo):
+    """PointMesh::reinit(false) through the C++ class: the lists and element
+    ids of the last build stay and the local velocity uses the new
+    positions with them (the oracle's reinit(false)); reinit(true) then
+    equals a fresh build."""
+    cfg, x, f = W.make_inputs("beads100")
+    ocfg = {k: cfg[k] for k in ("mu", "alpha", "box_min", "box_max", "periodic", "n_elem")}
+    ocfg["keep_stale_lists"] = 0
+    x2 = np.array(demo["b100_x2"]).reshape(-1, 3)
+    m = O.Mesh(ocfg)
+    m.reinit(x, f)
+    m.update_positions(x2, f)
+    assert demo["b100_lists_kept"] == 1
+    assert rel_l2(np.array(demo["b100_reinit_false_u"]).reshape(-1, 3), m.local_velocity_beads()) < 1e-12
+    m2 = O.Mesh(ocfg)
+    m2.reinit(x2, f)
+    assert rel_l2(np.array(demo["b100_reinit_true_u"]).reshape(-1, 3), m2.local_velocity_beads()) < 1e-12
