@@ -33,6 +33,10 @@ CHEB_KEEP = 0x100
 WALL_SLIT, WALL_SPHERE = 0, 1
 COMPAT_ELEM_HYSTERESIS = 1
 COMPAT_STALE_LISTS = 2
+# force-field fixes (ggem_apply_fixes)
+(FIX_P_CONSTANT, FIX_LJ_CUT, FIX_GAUSSIAN, FIX_WORM_LIKE_SPRING, FIX_FENE, FIX_SLIT_WALL_LJ_CUT,
+ FIX_SPHERE_WALL_LJ_CUT, FIX_SLIT_WALL_EMPIRICAL_DNA, FIX_SPHERE_WALL_EMPIRICAL_DNA,
+ FIX_DISCRETIZED_WALL_LJ_CUT) = range(1, 11)
 
 ERRORS = {
     -1: "GGEM_ERR_INVALID_ARG", -2: "GGEM_ERR_OUT_OF_DOMAIN", -3: "GGEM_ERR_INVALID_ELEM",
@@ -56,6 +60,7 @@ EXPORTS = [
     "ggem_noise_philox", "ggem_save_positions", "ggem_move_particles", "ggem_get_positions",
     "ggem_set_unwrapped", "ggem_get_noise", "ggem_util_noise_create", "ggem_util_noise_destroy",
     "ggem_util_noise_draw", "ggem_util_noise_skip", "ggem_update_positions",
+    "ggem_set_bonds", "ggem_set_constant_forces", "ggem_set_wall_particles", "ggem_apply_fixes",
 ]
 
 
@@ -88,6 +93,10 @@ class FixParams(C.Structure):
     _fields_ = [("chain_beads", C.c_int32), ("walls", C.c_int32), ("fene_k", C.c_double),
                 ("fene_bmax", C.c_double), ("lj_eps", C.c_double), ("lj_sigma", C.c_double),
                 ("lj_rcut", C.c_double)]
+
+
+class Fix(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 4)]
 
 
 class ChebInfo(C.Structure):
@@ -174,6 +183,10 @@ def lib():
             "ggem_set_unwrapped": [P, P],
             "ggem_get_noise": [P, P],
             "ggem_update_positions": [P, P, P],
+            "ggem_set_bonds": [P, I64, P],
+            "ggem_set_constant_forces": [P, P],
+            "ggem_set_wall_particles": [P, I64, P],
+            "ggem_apply_fixes": [P, I32, P, P],
             "ggem_util_noise_create": [U64, C.POINTER(C.c_void_p)],
             "ggem_util_noise_destroy": [P],
             "ggem_util_noise_draw": [P, I64, D, P],
@@ -467,6 +480,29 @@ class Context:
         fp = FixParams(int(chain_beads), int(bool(walls)), k, b_max, eps, sigma, rcut)
         f = np.zeros((self.n, 3)) if out else None
         self._c(lib().ggem_fix_forces(self._h, C.byref(fp), _p(f)))
+        return f
+
+    def set_bonds(self, bonds):
+        """Bonds [nb, 2] of 0-based bead ids (bead 1, bead 2)."""
+        b = np.ascontiguousarray(bonds, dtype=np.int64).reshape(-1, 2)
+        self._c(lib().ggem_set_bonds(self._h, b.shape[0], _p(b)))
+
+    def set_constant_forces(self, f):
+        self._c(lib().ggem_set_constant_forces(self._h, _p(_f64(f, (-1, 3)))))
+
+    def set_wall_particles(self, xyz):
+        w = _f64(xyz, (-1, 3))
+        self._c(lib().ggem_set_wall_particles(self._h, w.shape[0], _p(w)))
+
+    def apply_fixes(self, fixes, out=True):
+        """fixes: [(FIX_* type, params)]; their summed forces become the context's."""
+        arr = (Fix * max(len(fixes), 1))()
+        for i, (typ, p) in enumerate(fixes):
+            arr[i].type = int(typ)
+            for k, v in enumerate(p):
+                arr[i].p[k] = float(v)
+        f = np.zeros((self.n, 3)) if out else None
+        self._c(lib().ggem_apply_fixes(self._h, len(fixes), arr, _p(f)))
         return f
 
     def spread_nodes(self, nqp_1d, IF, n_nodes):

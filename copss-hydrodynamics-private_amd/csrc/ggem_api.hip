@@ -408,6 +408,10 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->keep_ids2);
   f(ctx->keep_cnt);
   f(ctx->stale_slot);
+  f(ctx->bond_off);
+  f(ctx->bond_inc);
+  f(ctx->cforce);
+  f(ctx->wall_xyz);
   f(ctx->x_saved);
   f(ctx->rout);
   f(ctx->images);

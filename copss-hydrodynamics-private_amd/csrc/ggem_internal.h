@@ -284,6 +284,19 @@ struct ggem_ctx {
   size_t par_cap = 0;
   hipEvent_t par_ev = nullptr;
 
+  // force-field fixes: bonds (incidences by bead, bond order), constant
+  // forces and discretized wall particles
+  int64_t* bond_off = nullptr;  // [bond_n + 1]
+  int4* bond_inc = nullptr;     // (bead 1, bead 2, 1 for bead 2's incidence, 0)
+  size_t bond_off_cap = 0, bond_inc_cap = 0;
+  int64_t bond_n = -1;          // particles the bonds were set for
+  double* cforce = nullptr;     // [3n] constant forces (p_constant)
+  size_t cforce_cap = 0;
+  int64_t cforce_n = -1;
+  double* wall_xyz = nullptr;   // [3 n_wall]
+  size_t wall_cap = 0;
+  int64_t n_wall = 0;
+
   // FE mesh of the global velocity: 20 node numbers per element (HEX20)
   int32_t* fe_conn = nullptr;
   int64_t fe_nodes = -1;  // -1 until ggem_set_fe_mesh

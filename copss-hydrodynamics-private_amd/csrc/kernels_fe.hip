@@ -193,8 +193,8 @@ __device__ __forceinline__ void fene_bond(const Grid& g, const double* x, int64_
   const double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
   const double ratio = R / b;
   double val0 = 1.0 - ratio * ratio;
-  constexpr double kTiny = 1e-10;
-  if (fabs(val0) < kTiny) val0 = double((val0 > 0.0) - (val0 < 0.0)) * kTiny * kTiny;
+  constexpr double kTol = 1e-3;  // FixBase's singularity guard (fix_base.h:50)
+  if (fabs(val0) < kTol) val0 = val0 / fabs(val0) * kTol * kTol;
   const double c = (k * b) * ratio / val0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) fb[d] = c * r[d] / R;

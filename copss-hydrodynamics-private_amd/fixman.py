@@ -97,8 +97,8 @@ class SlitPolymerForces:
         R = np.linalg.norm(r, axis=-1, keepdims=True)
         ratio = R / self.b
         val0 = 1.0 - ratio * ratio
-        tiny = 1e-10
-        val0 = np.where(np.abs(val0) < tiny, np.sign(val0) * tiny * tiny, val0)
+        tol = 1e-3  # FixBase's singularity guard (fix_base.h:50)
+        val0 = np.where(np.abs(val0) < tol, val0 / np.abs(val0) * tol * tol, val0)
         fb = (self.k * self.b) * ratio / val0 * r / R  # force on bead i toward i+1
         fc = f.reshape(self.nc, self.nb, 3)
         fc[:, :-1] += fb
@@ -238,8 +238,10 @@ class FixmanIntegrator:
 class DeviceFixman:
     """copss.C:1565-1763 with positions, forces, noise, velocities and
     displacements on the device (see module doc).  `fixes`: a
-    DeviceSlitForces (or None for no forces); bounds: fixed (lmin, lmax), or
-    None for Lanczos on the device mobility."""
+    DeviceSlitForces, a list of (GGEM_FIX_* type, params) for
+    ggem_apply_fixes (bonds and constant forces set on the context
+    beforehand), or None for no forces; bounds: fixed (lmin, lmax), or None
+    for Lanczos on the device mobility."""
 
     def __init__(self, ctx, x0, fixes, dt, cheb_order=30, tol_cheb=0.1, eig_factor=1.05, seed=123456789,
                  n_relax_step=1, bounds=None, lanczos_iters=40, lanczos_seed=11, wall_type=G.WALL_SLIT,
@@ -258,9 +260,13 @@ class DeviceFixman:
         ctx.set_unwrapped()
 
     def _reinit(self):
-        """neighbour build at the context's positions and the forces there"""
+        """neighbour build at the context's positions and the forces there:
+        a DeviceSlitForces (ggem_fix_forces) or a list of (GGEM_FIX_* type,
+        params) fixes (ggem_apply_fixes)"""
         self.ctx.build_neighbors()
-        if self.fixes is not None:
+        if isinstance(self.fixes, (list, tuple)):
+            self.ctx.apply_fixes(self.fixes, out=False)
+        elif self.fixes is not None:
             self.ctx.fix_forces(out=False, **self.fixes.params)
 
     def _bounds(self):

@@ -213,6 +213,63 @@ typedef struct ggem_fix_params {
  * copied to f (3n) unless f is NULL.  Single-system contexts only. */
 int ggem_fix_forces(ggem_ctx* ctx, const ggem_fix_params* fp, double* f);
 
+/* Force-field fixes of point particles, FixFactory::buildFix
+ * (fix/fix_factory.C:4-84) and the FixPoint* classes it builds, evaluated
+ * on the device.  p[] holds each fix's parameters in the reference's
+ * dimensionless form:
+ *  P_CONSTANT               each particle's constant force (ggem_set_constant_forces;
+ *                           FixPointConstant, the data file's force columns)
+ *  LJ_CUT                   {epsilon, sigma, rcut}: FixBase::lj_force of every
+ *                           neighbour-list pair within rcut (FixPointLJCut,
+ *                           fix_point_lj_cut.C:38-59; needs a build)
+ *  GAUSSIAN                 {c1, c2}: -c1 c2 exp(-c2 r^2) r_ij over the
+ *                           neighbour lists (FixPointGaussian; gaussian_dna is
+ *                           this law with c1, c2 from ev, Ss2, bk, Nks,
+ *                           fix_point_gaussian_dna.C; needs a build)
+ *  WORM_LIKE_SPRING         {c1 = a/(2 bk), Ls = Nks bk/a}: spring_force_wls over
+ *                           the bonds (FixPointWLS, fix_point_wls.C:66-96)
+ *  FENE                     {k, b_max}: spring_force_fene with c1 = k b_max,
+ *                           Ls = b_max over the bonds
+ *  SLIT_WALL_LJ_CUT         {epsilon, sigma, rcut}: the two box faces of every
+ *                           non-periodic axis (FixPointSlitWallLJCut)
+ *  SPHERE_WALL_LJ_CUT       {epsilon, sigma, rcut, radius} (FixPointSphereWallLJCut)
+ *  SLIT_WALL_EMPIRICAL_DNA  {c0, d0} (FixPointSlitWallEmpiricalDNA)
+ *  SPHERE_WALL_EMPIRICAL_DNA {c0, d0, radius} (FixPointSphereWallEmpiricalDNA)
+ *  DISCRETIZED_WALL_LJ_CUT  {epsilon, sigma, rcut}: every wall particle
+ *                           (ggem_set_wall_particles) within rcut
+ *                           (FixPointDiscretizedWallLJCut)
+ * Pair vectors use point_vector's minimum image. */
+enum {
+  GGEM_FIX_P_CONSTANT = 1,
+  GGEM_FIX_LJ_CUT = 2,
+  GGEM_FIX_GAUSSIAN = 3,
+  GGEM_FIX_WORM_LIKE_SPRING = 4,
+  GGEM_FIX_FENE = 5,
+  GGEM_FIX_SLIT_WALL_LJ_CUT = 6,
+  GGEM_FIX_SPHERE_WALL_LJ_CUT = 7,
+  GGEM_FIX_SLIT_WALL_EMPIRICAL_DNA = 8,
+  GGEM_FIX_SPHERE_WALL_EMPIRICAL_DNA = 9,
+  GGEM_FIX_DISCRETIZED_WALL_LJ_CUT = 10
+};
+typedef struct ggem_fix {
+  int32_t type;
+  int32_t pad;
+  double p[4];
+} ggem_fix;
+/* The bonds of PolymerChain (polymer_chain.C:188-202): n_bonds pairs of bead
+ * ids (bonds[2b], bonds[2b+1]), 0-based.  Bead 1 of a bond takes F_ij, bead 2
+ * F_ji = -F_ij, bond by bond. */
+int ggem_set_bonds(ggem_ctx* ctx, int64_t n_bonds, const int64_t* bonds);
+/* Each particle's constant force (3n, particle order) for P_CONSTANT. */
+int ggem_set_constant_forces(ggem_ctx* ctx, const double* f);
+/* Positions of m discretized wall particles (3m) for DISCRETIZED_WALL_LJ_CUT. */
+int ggem_set_wall_particles(ggem_ctx* ctx, int64_t m, const double* xyz);
+/* The forces of nfix (<= 8) fixes at the context's current positions, added
+ * in the order given (each reinit zeroes the forces and every fix adds its
+ * part, point_mesh.C:249-251); the sum replaces the context's forces and is
+ * copied to f (3n) unless f is NULL.  Single-system contexts only. */
+int ggem_apply_fixes(ggem_ctx* ctx, int32_t nfix, const ggem_fix* fixes, double* f);
+
 /* ---- (d) Brownian step ------------------------------------------------ */
 /* Mobility operator u = M f: _MatMult_Stokes (brownian_system.h:54-59,
  * brownian_system.C:44-140) with the host FEM part U_glob left out, i.e.
