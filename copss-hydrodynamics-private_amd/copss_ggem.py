@@ -61,6 +61,7 @@ EXPORTS = [
     "ggem_set_unwrapped", "ggem_get_noise", "ggem_util_noise_create", "ggem_util_noise_destroy",
     "ggem_util_noise_draw", "ggem_util_noise_skip", "ggem_update_positions",
     "ggem_set_bonds", "ggem_set_constant_forces", "ggem_set_wall_particles", "ggem_apply_fixes",
+    "ggem_local_velocity_points_noalloc",
 ]
 
 
@@ -187,6 +188,7 @@ def lib():
             "ggem_set_constant_forces": [P, P],
             "ggem_set_wall_particles": [P, I64, P],
             "ggem_apply_fixes": [P, I32, P, P],
+            "ggem_local_velocity_points_noalloc": [P, I64, P, P, P],
             "ggem_util_noise_create": [U64, C.POINTER(C.c_void_p)],
             "ggem_util_noise_destroy": [P],
             "ggem_util_noise_draw": [P, I64, D, P],
@@ -517,6 +519,15 @@ class Context:
         el = None if elem is None else np.ascontiguousarray(elem, dtype=np.int32)
         u = np.zeros_like(pts)
         self._c(lib().ggem_local_velocity_points(self._h, pts.shape[0], _p(pts), _p(el), _p(u)))
+        return u
+
+    def local_velocity_points_noalloc(self, pts, elem=None):
+        """local_velocity_points staged in the fixed per-GPU device array (no
+        device allocation; at most 65536 points)."""
+        pts = _f64(pts, (-1, 3))
+        el = None if elem is None else np.ascontiguousarray(elem, dtype=np.int32)
+        u = np.zeros_like(pts)
+        self._c(lib().ggem_local_velocity_points_noalloc(self._h, pts.shape[0], _p(pts), _p(el), _p(u)))
         return u
 
     def spread(self, nqp_1d, IF, want_rho=True, want_fe=True):
