@@ -289,9 +289,13 @@ extern "C" int ggem_util_fp32c_table(double eps, double alpha, double ksi, int32
   if (!(eps > 0) || !(alpha > 0) || !(ksi > 0) || !meta || !n_intervals) return GGEM_ERR_INVALID_ARG;
   const double k = std::max(alpha * alpha, ksi * ksi), m = std::min(alpha * alpha, ksi * ksi);
   const double d1 = pow2_floor(1.4 / k), d2 = pow2_floor(1.4 / m);
-  const int n1 = int(std::ceil((36.0 / k) / d1));
+  // zero from s m >= 20 on: both coefficients are below e^-20 of their size
+  // there (erfc and the Gaussians decay as e^{-m s}), far below the FP32
+  // rounding of the pairs that remain
+  const double s_zero = 20.0 / m;
+  const int n1 = int(std::ceil(std::min(36.0 / k, s_zero) / d1));
   const double s1 = n1 * d1;
-  const int n2 = std::max(0, int(std::ceil((36.0 / m - s1) / d2)));
+  const int n2 = std::max(0, int(std::ceil((s_zero - s1) / d2)));
   const int nint = n1 + n2 + 1;  // the last interval is zero
   *n_intervals = nint;
   meta[0] = float(s1);
@@ -301,7 +305,7 @@ extern "C" int ggem_util_fp32c_table(double eps, double alpha, double ksi, int32
   meta[4] = float(d2);
   meta[5] = float(n1);
   meta[6] = float(n2);
-  meta[7] = 0.f;
+  meta[7] = float(s1 + n2 * d2);  // start of the zero interval
   if (!table) return GGEM_OK;  // size query
   if (cap < nint * 16) return GGEM_ERR_INVALID_ARG;
   const double PI = 3.14159265358979323846;

@@ -69,16 +69,23 @@ struct PairArgs {
 // two), then two degree-7 Horner chains; past the cut-off the last
 // interval is zero.  One body for every FP32C kernel, so they all
 // evaluate bit-identical tensors.
-__device__ __forceinline__ void fp32c_coef(const float4* __restrict__ tab, const CoefTab& t, float s, float& A,
-                                           float& Bx) {
+// The interval of s and y in it.  The last interval (index t.n - 1) is the
+// zero one: its pairs are skipped by every FP32C pair sum (fp32c_kept),
+// which add exactly the same terms in the same order whether they store the
+// kept pairs or test each entry.
+__device__ __forceinline__ int fp32c_interval(const CoefTab& t, float s, float& y) {
   const bool r2 = s >= t.s1;
   const float base = r2 ? t.s1 : 0.f;
   const float inv = r2 ? t.inv2 : t.inv1;
   const float d = r2 ? t.d2 : t.d1;
   const int j = min(int((s - base) * inv), r2 ? t.n2 : t.n1 - 1);
-  const int i = (r2 ? t.n1 : 0) + j;
   const float mid = __builtin_fmaf(float(j) + 0.5f, d, base);
-  const float y = (s - mid) * (2.f * inv);
+  y = (s - mid) * (2.f * inv);
+  return (r2 ? t.n1 : 0) + j;
+}
+__device__ __forceinline__ bool fp32c_kept(const CoefTab& t, int i) { return i < t.n - 1; }
+
+__device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int i, float y, float& A, float& Bx) {
   const float4 a0 = tab[4 * i], a1 = tab[4 * i + 1], b0 = tab[4 * i + 2], b1 = tab[4 * i + 3];
   float pa = __builtin_fmaf(a1.w, y, a1.z);
   pa = __builtin_fmaf(pa, y, a1.y);
@@ -94,6 +101,13 @@ __device__ __forceinline__ void fp32c_coef(const float4* __restrict__ tab, const
   pb = __builtin_fmaf(pb, y, b0.z);
   pb = __builtin_fmaf(pb, y, b0.y);
   Bx = __builtin_fmaf(pb, y, b0.x);
+}
+
+__device__ __forceinline__ void fp32c_coef(const float4* __restrict__ tab, const CoefTab& t, float s, float& A,
+                                           float& Bx) {
+  float y;
+  const int i = fp32c_interval(t, s, y);
+  fp32c_eval(tab, i, y, A, Bx);
 }
 
 // r^2 of an FP32C displacement, one fixed evaluation order
@@ -387,16 +401,11 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
         if (pz) dz = dz <= -0.5f * Lz ? dz + Lz : (dz > 0.5f * Lz ? dz - Lz : dz);
       }
     };
-    auto pair_acc = [&](uint32_t e, const float4& pt, const float4* f) {
-      float dx, dy, dz;
-      disp(e, pt, dx, dy, dz);
-      float A, Bx;
-      fp32c_coef(tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
-#pragma unroll
-      for (int r = 0; r < NR; ++r) fp32c_acc(A, Bx, dx, dy, dz, f[r].x, f[r].y, f[r].z, ps[r][0], ps[r][1], ps[r][2]);
-    };
-    auto merge = [&](int k) {
-      if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+    // kept pairs are summed and merged every kMergeEvery of them; the zero
+    // interval's pairs are skipped (nk counts the kept ones)
+    int nk = 0;
+    auto merge = [&]() {
+      if ((nk & (kMergeEvery - 1)) == 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -405,6 +414,19 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
             ps[r][q] = 0.f;
           }
       }
+    };
+    auto pair_acc = [&](uint32_t e, const float4& pt, const float4* f) {
+      float dx, dy, dz;
+      disp(e, pt, dx, dy, dz);
+      float y;
+      const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
+      if (!fp32c_kept(a.ct, it)) return;
+      float A, Bx;
+      fp32c_eval(tab, it, y, A, Bx);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) fp32c_acc(A, Bx, dx, dy, dz, f[r].x, f[r].y, f[r].z, ps[r][0], ps[r][1], ps[r][2]);
+      ++nk;
+      merge();
     };
     int k = 0;
     for (; k + U <= cnt; k += U) {
@@ -424,10 +446,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
         for (int r = 0; r < NR; ++r) f[i][r] = xg[r * n + t];
       }
 #pragma unroll
-      for (int i = 0; i < U; ++i) {
-        pair_acc(c[i], pt[i], f[i]);
-        merge(k + i);
-      }
+      for (int i = 0; i < U; ++i) pair_acc(c[i], pt[i], f[i]);
     }
 #pragma unroll
     for (int i = 0; i < U - 1; ++i) {
@@ -437,7 +456,6 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 #pragma unroll
         for (int r = 0; r < NR; ++r) f[r] = xg[r * n + t];
         pair_acc(en[i], pc[t], f);
-        merge(k + i);
       }
     }
 #pragma unroll
@@ -473,8 +491,13 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // the neighbours' records from global memory.
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
-constexpr int kBaseBits = 48;       // entry offsets below 2^48
+// A target's packed entry base: the offset of its entry 0 (below 2^40), its
+// own record number (12 bits) and, in staged bricks, its count of stored
+// (kept) entries (12 bits; a staged box holds at most kBoxMaxRecs records).
+constexpr int kBaseBits = 40;
 constexpr int64_t kBaseMask = (int64_t(1) << kBaseBits) - 1;
+constexpr int kSelfShift = kBaseBits, kKeptShift = kBaseBits + 12;
+constexpr uint64_t kField12 = 0xFFFu;
 
 struct BoxArgs {
   const int32_t* cs;     // cell starts, brick order
@@ -719,31 +742,39 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       const int qs = (bz * B.ey + by) * B.ex + bx;
       self = cst[qs] + int(s - cfirst[qs]);
     }
-    b.base[sb + i] = bb | (int64_t(self) << kBaseBits);
     const int cnt = a.cnt[s];
-    if (cnt == 0) continue;  // an empty list has no storage
     if (nrec > 0) {
+      // only the kept pairs (not in the table's zero interval) are stored
       int64_t o = bb;
-      box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
-        for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
-          if (r == self) continue;
-          const float4 pt = rpos[r];
-          const uint32_t bt = __float_as_uint(pt.w);
-          const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
-          const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
-          const int dk = int(bt >> (2 * kBoxBits)) - bz;
-          const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
-          const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
-          const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-          float A, Bx;
-          fp32c_coef(s_tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
-          b.ab[o] = make_float2(A, Bx);
-          b.idx[o] = uint16_t(r);
-          o += kSlice;
-        }
-      });
+      int kept = 0;
+      if (cnt > 0)
+        box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
+          for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
+            if (r == self) continue;
+            const float4 pt = rpos[r];
+            const uint32_t bt = __float_as_uint(pt.w);
+            const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
+            const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
+            const int dk = int(bt >> (2 * kBoxBits)) - bz;
+            const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+            const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+            const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+            float y;
+            const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
+            if (!fp32c_kept(a.ct, it)) continue;
+            float A, Bx;
+            fp32c_eval(s_tab, it, y, A, Bx);
+            b.ab[o] = make_float2(A, Bx);
+            b.idx[o] = uint16_t(r);
+            o += kSlice;
+            ++kept;
+          }
+        });
+      b.base[sb + i] = bb | (int64_t(self) << kSelfShift) | (int64_t(kept) << kKeptShift);
       continue;
     }
+    b.base[sb + i] = bb;
+    if (cnt == 0) continue;  // an empty list has no storage
     // unstaged brick: the list in list order, the neighbour's position
     // record loaded one entry ahead
     const int64_t base = a.slice_off[s >> 6] + (s & 63);
@@ -790,6 +821,7 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   const int ko = int(co >> (kCellBitsX + kCellBitsY));
   float ps[3] = {0.f, 0.f, 0.f};
   KahanF kh[3];
+  int nk = 0;  // kept pairs so far
   for (int k = 0; k < cnt; ++k) {
     const uint32_t e = a.ell[base + int64_t(k) * kSlice];
     const uint32_t t = e & kSlotMask;
@@ -804,9 +836,11 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
     const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
     const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
     const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+    float y;
+    if (!fp32c_kept(a.ct, fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y))) continue;  // the zero interval
     const float2 c = ab[bb + int64_t(k) * kSlice];
     fp32c_acc(c.x, c.y, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
-    if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+    if ((++nk & (kMergeEvery - 1)) == 0) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         kh[q].add(ps[q]);
@@ -877,12 +911,12 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     const int64_t s = b.tgt[sb + i];
     const int64_t bpk = b.base[sb + i];
     const int64_t base = bpk & kBaseMask;
-    const float4 po = rpos[int(bpk >> kBaseBits)];  // the target's own record
+    const float4 po = rpos[int((uint64_t(bpk) >> kSelfShift) & kField12)];  // the target's own record
     const uint32_t co = __float_as_uint(po.w);
     const int bx = int(co & ((1u << kBoxBits) - 1));
     const int by = int((co >> kBoxBits) & ((1u << kBoxBits) - 1));
     const int bz = int(co >> (2 * kBoxBits));
-    const int cnt = a.cnt[s];
+    const int cnt = int((uint64_t(bpk) >> kKeptShift) & kField12);  // stored (kept) entries
     float ps[3] = {0.f, 0.f, 0.f};
     KahanF kh[3];
     auto pair = [&](uint32_t rec, float2 c) {

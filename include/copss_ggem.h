@@ -466,8 +466,11 @@ int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*n
  * (ggem_stokes.C:304-357) as a piecewise polynomial in s = r^2, the form
  * the FP32C kernels evaluate: n_intervals intervals of 16 floats (A's 8
  * monomial coefficients in y = 2 (s - mid)/d, then Bx's), the last one
- * zero (s beyond the cut-off); meta = {s1, 1/d1, 1/d2, d1, d2, n1, n2, 0}
- * (intervals of width d1 on [0, s1), d2 after).  table = NULL queries
+ * zero (s beyond the cut-off, where min(alpha^2, xi^2) s >= 20 and both
+ * coefficients are below e^-20 of their size); meta = {s1, 1/d1, 1/d2, d1,
+ * d2, n1, n2, s_zero} (intervals of width d1 on [0, s1), d2 after, zero
+ * from s_zero on).  The FP32C pair sums skip the pairs of the zero
+ * interval.  table = NULL queries
  * n_intervals; cap is the table's size in floats. */
 int ggem_util_fp32c_table(double eps, double alpha, double ksi, int32_t cap, float* table, float* meta,
                           int32_t* n_intervals);

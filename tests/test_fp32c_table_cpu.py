@@ -54,11 +54,13 @@ def test_table_matches_fp64_tensor(alpha, br0):
     tab, meta = G.fp32c_table(c.one_eight_pi_mu, c.alpha, c.ksi)
     assert tab.shape[0] == int(meta[5]) + int(meta[6]) + 1
     assert np.all(tab[-1] == 0)
-    # widths are powers of two and the intervals cover 36 decay lengths
+    # widths are powers of two and the intervals cover 20 decay lengths of
+    # the slower scale, after which the table is zero
     k, m = max(c.alpha2, c.ksi2), min(c.alpha2, c.ksi2)
     assert np.log2(meta[3]) == np.round(np.log2(meta[3])) and meta[3] * k <= 1.4
-    assert meta[0] + meta[6] * meta[4] >= 36 / m
+    assert meta[0] + meta[6] * meta[4] >= 20 / m
     s_cut = float(meta[0] + meta[6] * meta[4])
+    assert meta[7] == s_cut
     rng = np.random.default_rng(3)
     s = np.concatenate([np.linspace(1e-6, 1.0, 400), rng.uniform(0, s_cut, 1600),
                         np.linspace(0.9 * s_cut, 1.5 * s_cut, 200)])
