@@ -38,6 +38,9 @@ COMPAT_STALE_LISTS = 2
  FIX_SPHERE_WALL_LJ_CUT, FIX_SLIT_WALL_EMPIRICAL_DNA, FIX_SPHERE_WALL_EMPIRICAL_DNA,
  FIX_DISCRETIZED_WALL_LJ_CUT) = range(1, 11)
 
+# right-hand side options (ggem_assemble_rhs)
+RHS_DISTURBED, RHS_UNDISTURBED = 0, 1
+
 ERRORS = {
     -1: "GGEM_ERR_INVALID_ARG", -2: "GGEM_ERR_OUT_OF_DOMAIN", -3: "GGEM_ERR_INVALID_ELEM",
     -4: "GGEM_ERR_HIP", -5: "GGEM_ERR_NOT_READY", -6: "GGEM_ERR_STENCIL_TIE",
@@ -61,7 +64,8 @@ EXPORTS = [
     "ggem_set_unwrapped", "ggem_get_noise", "ggem_util_noise_create", "ggem_util_noise_destroy",
     "ggem_util_noise_draw", "ggem_util_noise_skip", "ggem_update_positions",
     "ggem_set_bonds", "ggem_set_constant_forces", "ggem_set_wall_particles", "ggem_apply_fixes",
-    "ggem_local_velocity_points_noalloc",
+    "ggem_local_velocity_points_noalloc", "ggem_assemble_elem_rhs", "ggem_assemble_rhs",
+    "ggem_util_rhs_wall_items", "ggem_util_hex20_traction_table",
 ]
 
 
@@ -98,6 +102,29 @@ class FixParams(C.Structure):
 
 class Fix(C.Structure):
     _fields_ = [("type", C.c_int32), ("pad", C.c_int32), ("p", C.c_double * 4)]
+
+
+class RhsOpts(C.Structure):
+    _fields_ = [("option", C.c_int32), ("validation", C.c_int32), ("shear", C.c_int32 * 3),
+                ("shear_direction", C.c_int32 * 3), ("inlet", C.c_int32 * 3), ("pad", C.c_int32),
+                ("shear_rate", C.c_double * 3), ("inlet_pressure", C.c_double * 3),
+                ("penalty", C.c_double)]
+
+
+def rhs_opts(option=RHS_DISTURBED, validation=False, shear=(0, 0, 0), shear_direction=(0, 0, 0),
+             shear_rate=(0.0, 0.0, 0.0), inlet=(0, 0, 0), inlet_pressure=(0.0, 0.0, 0.0), penalty=1e6):
+    """Options of the right-hand side assembly (the reference's penalty is 1e6)."""
+    o = RhsOpts()
+    o.option = int(option)
+    o.validation = int(bool(validation))
+    for a in range(3):
+        o.shear[a] = int(bool(shear[a]))
+        o.shear_direction[a] = int(shear_direction[a])
+        o.shear_rate[a] = float(shear_rate[a])
+        o.inlet[a] = int(bool(inlet[a]))
+        o.inlet_pressure[a] = float(inlet_pressure[a])
+    o.penalty = float(penalty)
+    return o
 
 
 class ChebInfo(C.Structure):
@@ -150,6 +177,10 @@ def lib():
             "ggem_global_velocity_beads": [P, P, P],
             "ggem_combine_velocity_fe": [P, P, P],
             "ggem_spread_nodes": [P, I32, P, P],
+            "ggem_assemble_elem_rhs": [P, I32, P, P, P],
+            "ggem_assemble_rhs": [P, I32, P, P, I32, P],
+            "ggem_util_rhs_wall_items": [P, P, P, P, P],
+            "ggem_util_hex20_traction_table": [P, P],
             "ggem_fix_forces": [P, P, P],
             "ggem_spread": [P, I32, I32, P, P, P],
             "ggem_mobility_apply": [P, P, P],
@@ -301,6 +332,25 @@ def hex20_if_table(h, nqp_1d):
     IF = np.zeros(20 * nqp_1d ** 3)
     _check(lib().ggem_util_hex20_if_table(_p(h), nqp_1d, _p(IF)))
     return IF
+
+
+def rhs_wall_items(n_elem, periodic, opts):
+    """(element, local node, wall side mask, inlet side mask) of every wall /
+    inlet item of the right-hand side (ggem_util_rhs_wall_items)."""
+    n = (C.c_int32 * 3)(*[int(v) for v in n_elem])
+    per = (C.c_int32 * 3)(*[int(bool(v)) for v in periodic])
+    m = C.c_int64(0)
+    _check(lib().ggem_util_rhs_wall_items(n, per, C.byref(opts), C.byref(m), None))
+    it = np.zeros((m.value, 2), np.int32)
+    _check(lib().ggem_util_rhs_wall_items(n, per, C.byref(opts), C.byref(m), _p(it)))
+    return it[:, 0], it[:, 1] & 0xFF, (it[:, 1] >> 8) & 63, (it[:, 1] >> 16) & 63
+
+
+def hex20_traction_table(h):
+    """[6, 4, 20] JxW_face phi_l at the 2 x 2 Gauss points of every hex side."""
+    tab = np.zeros((6, 4, 20))
+    _check(lib().ggem_util_hex20_traction_table(_p(_f64(h)), _p(tab)))
+    return tab
 
 
 def hex20_box_mesh(cfg):
@@ -512,6 +562,22 @@ class Context:
         IF = _f64(IF).ravel()
         F = np.zeros((int(n_nodes), 3))
         self._c(lib().ggem_spread_nodes(self._h, nqp_1d, _p(IF), _p(F)))
+        return F
+
+    def assemble_elem_rhs(self, nqp_1d, IF, opts):
+        """Element vectors Fe[n_elem, 3, 20] after the spreading (or inlet
+        traction) and the wall penalty rows; opts from rhs_opts()."""
+        IF = None if IF is None else _f64(IF).ravel()
+        Fe = np.zeros((self.n_elem, 3, 20))
+        self._c(lib().ggem_assemble_elem_rhs(self._h, nqp_1d, _p(IF), C.byref(opts), _p(Fe)))
+        return Fe
+
+    def assemble_rhs(self, nqp_1d, IF, opts, n_nodes, condense=True):
+        """The element vectors summed into F[n_nodes, 3] over the velocity
+        mesh, periodic images condensed onto the min faces when condense."""
+        IF = None if IF is None else _f64(IF).ravel()
+        F = np.zeros((int(n_nodes), 3))
+        self._c(lib().ggem_assemble_rhs(self._h, nqp_1d, _p(IF), C.byref(opts), int(bool(condense)), _p(F)))
         return F
 
     def local_velocity_points(self, pts, elem=None):

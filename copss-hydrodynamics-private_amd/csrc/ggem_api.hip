@@ -425,6 +425,12 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->fe_inc_off);
   f(ctx->fe_inc);
   f(ctx->fe_node_order);
+  f(ctx->fe_cinc_off);
+  f(ctx->fe_cinc);
+  f(ctx->rhs_items);
+  f(ctx->rhs_tab);
+  f(ctx->rhs_buf);
+  f(ctx->rhs_el);
   if (ctx->par_h) (void)hipHostFree(ctx->par_h);
   ctx->par_h = nullptr;
   if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
@@ -467,9 +473,11 @@ int grow_scan64(ggem_ctx* ctx, int64_t len) {
   return GGEM_OK;
 }
 
+}  // namespace
+
 // Element particle lists for the spreading, built on first use after a
 // neighbour build.
-int ensure_elist(ggem_ctx* ctx) {
+int ggem::ensure_elist(ggem_ctx* ctx) {
   if (ctx->elist_valid) return GGEM_OK;
   const int64_t ne = ctx->grid.n_elem;
   hipError_t e = launch_elist_count(ctx);
@@ -485,8 +493,6 @@ int ensure_elist(ggem_ctx* ctx) {
   ctx->elist_valid = true;
   return GGEM_OK;
 }
-
-}  // namespace
 
 namespace {
 
@@ -1074,12 +1080,15 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
     }
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   for (void* p : {static_cast<void*>(ctx->fe_conn), static_cast<void*>(ctx->fe_inc_off),
-                  static_cast<void*>(ctx->fe_inc), static_cast<void*>(ctx->fe_node_order)})
+                  static_cast<void*>(ctx->fe_inc), static_cast<void*>(ctx->fe_node_order),
+                  static_cast<void*>(ctx->fe_cinc_off), static_cast<void*>(ctx->fe_cinc)})
     if (p) (void)hipFree(p);
   ctx->fe_conn = nullptr;
   ctx->fe_inc_off = nullptr;
   ctx->fe_inc = nullptr;
   ctx->fe_node_order = nullptr;
+  ctx->fe_cinc_off = nullptr;
+  ctx->fe_cinc = nullptr;
   ctx->fe_nodes = -1;
   // incidences of each node, (element, local node) ascending: the order in
   // which the nodal force sums the element vectors
@@ -1108,6 +1117,53 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
     for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
     for (int64_t v = 0; v < n_nodes; ++v) order[start[key[v]]++] = int32_t(v);
   }
+  // Periodic condensation (constrain_element_vector, assemble_stokes.C:508,
+  // with the periodic boundaries of copss.C:1294-1330 on the dof map): each
+  // node sits at a point of the half-element lattice; a node on the max face
+  // of a periodic axis is constrained to the node at its image with every
+  // such coordinate moved to the min face, which then sums the incidences of
+  // all its images in (element, local node) order, the order in which the
+  // element loop adds them; the constrained node's row stays zero.
+  std::vector<int64_t> coff;
+  std::vector<int32_t> cinc;
+  const bool any_periodic = ctx->grid.periodic[0] || ctx->grid.periodic[1] || ctx->grid.periodic[2];
+  if (any_periodic) {
+    const int nx = ctx->grid.n[0], ny = ctx->grid.n[1], nz = ctx->grid.n[2];
+    const int64_t m0 = 2 * int64_t(nx) + 1, m1 = 2 * int64_t(ny) + 1, m2 = 2 * int64_t(nz) + 1;
+    static const int kLat[20][3] = {{0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2},
+                                    {0, 2, 2}, {1, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 1},
+                                    {2, 2, 1}, {0, 2, 1}, {1, 0, 2}, {2, 1, 2}, {1, 2, 2}, {0, 1, 2}};
+    std::vector<int32_t> at(size_t(m0 * m1 * m2), -1);
+    std::vector<int64_t> lat(n_nodes, -1);
+    for (int64_t k = 0; k < 20 * ne; ++k) {
+      const int64_t e = k / 20;
+      const int l = int(k % 20);
+      const int64_t p0 = 2 * (e % nx) + kLat[l][0], p1 = 2 * ((e / nx) % ny) + kLat[l][1],
+                    p2 = 2 * (e / (int64_t(nx) * ny)) + kLat[l][2];
+      const int64_t idx = p0 + m0 * (p1 + m1 * p2);
+      const int32_t v = elem_nodes[k];
+      if (at[idx] < 0) at[idx] = v;
+      if (lat[v] < 0) lat[v] = idx;
+    }
+    std::vector<int32_t> rep(n_nodes);
+    for (int64_t v = 0; v < n_nodes; ++v) {
+      if (lat[v] < 0) {
+        rep[v] = int32_t(v);
+        continue;
+      }
+      int64_t p[3] = {lat[v] % m0, (lat[v] / m0) % m1, lat[v] / (m0 * m1)};
+      const int64_t mx[3] = {m0 - 1, m1 - 1, m2 - 1};
+      for (int d = 0; d < 3; ++d)
+        if (ctx->grid.periodic[d] && p[d] == mx[d]) p[d] = 0;
+      rep[v] = at[p[0] + m0 * (p[1] + m1 * p[2])];
+    }
+    coff.assign(n_nodes + 1, 0);
+    for (int64_t k = 0; k < 20 * ne; ++k) ++coff[rep[elem_nodes[k]] + 1];
+    for (int64_t v = 0; v < n_nodes; ++v) coff[v + 1] += coff[v];
+    cinc.resize(20 * ne);
+    std::vector<int64_t> pos(coff.begin(), coff.end() - 1);
+    for (int64_t k = 0; k < 20 * ne; ++k) cinc[pos[rep[elem_nodes[k]]]++] = int32_t(k);
+  }
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_conn), sizeof(int32_t) * 20 * ne));
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_inc), sizeof(int32_t) * 20 * ne));
   GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_node_order), sizeof(int32_t) * n_nodes));
@@ -1117,9 +1173,15 @@ int ggem_set_fe_mesh(ggem_ctx* ctx, int64_t n_nodes, const int32_t* elem_nodes) 
   if ((rc = h2d(ctx, ctx->fe_inc, inc.data(), sizeof(int32_t) * 20 * ne))) return rc;
   if ((rc = h2d(ctx, ctx->fe_inc_off, off.data(), sizeof(int64_t) * (n_nodes + 1)))) return rc;
   if ((rc = h2d(ctx, ctx->fe_node_order, order.data(), sizeof(int32_t) * n_nodes))) return rc;
+  if (any_periodic) {
+    GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_cinc), sizeof(int32_t) * 20 * ne));
+    GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->fe_cinc_off), sizeof(int64_t) * (n_nodes + 1)));
+    if ((rc = h2d(ctx, ctx->fe_cinc, cinc.data(), sizeof(int32_t) * 20 * ne))) return rc;
+    if ((rc = h2d(ctx, ctx->fe_cinc_off, coff.data(), sizeof(int64_t) * (n_nodes + 1)))) return rc;
+  }
   if ((rc = ensure(ctx, &ctx->fe_u, &ctx->fe_u_cap, size_t(3 * n_nodes)))) return rc;
   ctx->fe_nodes = n_nodes;
-  return GGEM_OK;
+  return sync(ctx);  // the host tables above are freed on return
 }
 
 namespace {

@@ -57,6 +57,8 @@ inline Pools& pools(ggem_ctx* ctx) { return *reinterpret_cast<Pools*>(ctx->scrat
 int ensure_tensors(ggem_ctx* ctx, int nrhs = 1);
 // the sliced neighbour lists of the current build (filled on first use)
 int ensure_ell(ggem_ctx* ctx);
+// the element particle lists of the spreading (filled on first use)
+int ensure_elist(ggem_ctx* ctx);
 
 inline int check_ready(ggem_ctx* ctx) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");

@@ -305,6 +305,22 @@ struct ggem_ctx {
   int64_t* fe_inc_off = nullptr;  // [fe_nodes + 1] incidences of each node
   int32_t* fe_inc = nullptr;      // [20 n_elem] 20 e + l, by node, (e, l) ascending
   int32_t* fe_node_order = nullptr;  // [fe_nodes] nodes by the brick of their first element
+  // periodic condensation: a min-face node sums the incidences of all its
+  // images ((e, l) ascending), a max-face node of a periodic axis has none
+  int64_t* fe_cinc_off = nullptr;  // [fe_nodes + 1]
+  int32_t* fe_cinc = nullptr;      // [20 n_elem]
+
+  // wall and inlet items of the right-hand side (element, l | masks), for
+  // the options key rhs_key (-1: none built)
+  int2* rhs_items = nullptr;
+  size_t rhs_items_cap = 0;
+  int64_t rhs_m = 0;
+  int32_t rhs_key = -1;
+  double* rhs_tab = nullptr;  // [6][4][20] JxW_face phi_l of every side's face points
+  double* rhs_buf = nullptr;  // [9 rhs_m] item points, local and unbounded velocities
+  size_t rhs_buf_cap = 0;
+  int32_t* rhs_el = nullptr;  // [rhs_m] element of each item
+  size_t rhs_el_cap = 0;
 
   // generic scratch (grown on demand)
   void* scratch = nullptr;
@@ -384,8 +400,10 @@ hipError_t launch_fill_keep(ggem_ctx* ctx, const int64_t* old_off, const int32_t
 // ---- launchers (kernels_fe.hip) ----
 // out[3p + d] = HEX20 interpolant of node_u[3 node + d] at particle p in its element
 hipError_t launch_interp_hex20(ggem_ctx* ctx, const int32_t* conn, const double* node_u, double* out);
-// out[3 node + d] = sum over the node's incidences (e, l) of Fe[e][l][d] (n_u = 20)
-hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out);
+// out[3 node + d] = sum over the node's incidences (e, l) of Fe[e][l][d] (n_u = 20);
+// condensed: over the incidences of the node's periodic images (none for a
+// max-face node of a periodic axis)
+hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out, bool condensed = false);
 // f_in = FENE forces along chains of `chain` consecutive particles (chain > 1)
 // + LJ forces of the walls of the non-periodic axes (walls != 0), from x_in
 hipError_t launch_fix_forces(ggem_ctx* ctx, int32_t chain, double k, double b, int32_t walls, double eps,

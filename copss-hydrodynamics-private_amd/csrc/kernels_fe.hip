@@ -144,12 +144,15 @@ __global__ __launch_bounds__(kFeBlock) void k_gather_nodes(int64_t nn, const int
 
 }  // namespace
 
-hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out) {
+hipError_t launch_gather_nodes(ggem_ctx* ctx, const double* Fe, double* out, bool condensed) {
   const int64_t nn = ctx->fe_nodes;
   if (nn <= 0) return hipSuccess;
+  // without a periodic axis the condensed incidences are the plain ones
+  const bool c = condensed && ctx->fe_cinc;
   ProfScope ps(ctx, "gather_nodes");
   hipLaunchKernelGGL(k_gather_nodes, dim3(unsigned((nn + kFeBlock - 1) / kFeBlock)), dim3(kFeBlock), 0,
-                     ctx->stream, nn, ctx->fe_node_order, ctx->fe_inc_off, ctx->fe_inc, Fe, out);
+                     ctx->stream, nn, ctx->fe_node_order, c ? ctx->fe_cinc_off : ctx->fe_inc_off,
+                     c ? ctx->fe_cinc : ctx->fe_inc, Fe, out);
   return hipGetLastError();
 }
 

@@ -185,6 +185,50 @@ int ggem_combine_velocity_fe(ggem_ctx* ctx, const double* node_u, double* U);
  * caller numbers as one node are summed into it. */
 int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F_node /* 3*n_nodes */);
 
+/* ---- global right-hand side: walls, inlet traction, periodic images ---- */
+/* AssembleStokes::assemble_global_F (assemble_stokes.C:473-516) with the
+ * option of the call ("disturbed" / "undisturbed"):
+ *  - compute_element_rhs: the point-force spreading (disturbed, :643-672)
+ *    or the traction of the inlet faces, + JxW_face phi_l p_j on every side
+ *    of an inlet axis j (undisturbed, :687-727);
+ *  - apply_bc_by_penalty (:889-996): on every wall side (a box face that is
+ *    neither periodic nor an inlet, select_boundary_side :839-885) and every
+ *    node l of it, Fe(d 20 + l) += penalty uvw_d (penalize_elem_matrix_vector,
+ *    assemble_system.C:70-107), uvw = the wall's undisturbed velocity
+ *    (shear[a]: component shear_direction[a] = shear_rate[a] times the
+ *    face's coordinate, :929-948) minus the local velocity of the node in
+ *    that element (disturbed), or the unbounded solution minus it
+ *    (validation, the "ggem_validation" case, :949-976);
+ *  - constrain_element_vector (:508) with libMesh's periodic constraints:
+ *    every node on the max face of a periodic axis takes the node at its
+ *    image on the min faces as its constraint, so the condensed nodal force
+ *    of that node moves there and its own row is zero. */
+#define GGEM_RHS_DISTURBED 0
+#define GGEM_RHS_UNDISTURBED 1
+typedef struct ggem_rhs_opts {
+  int32_t option;             /* GGEM_RHS_DISTURBED / GGEM_RHS_UNDISTURBED */
+  int32_t validation;         /* wall value = exact_solution_infinite_domain */
+  int32_t shear[3];           /* per axis: shear of that axis's two walls */
+  int32_t shear_direction[3]; /* velocity component of the shear, 0..2 */
+  int32_t inlet[3];           /* inlet axes (no penalty; traction when undisturbed) */
+  int32_t pad;
+  double shear_rate[3];
+  double inlet_pressure[3];
+  double penalty;             /* 1e6 in the reference (assemble_stokes.C:910) */
+} ggem_rhs_opts;
+/* Element vectors Fe[n_elem][3][20] (the order of Fe(d 20 + l)) after the
+ * spreading or traction and the penalty rows, before the constraints.  The
+ * disturbed option needs built lists; IF as for ggem_spread (20 nodes). */
+int ggem_assemble_elem_rhs(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, const ggem_rhs_opts* o,
+                           double* Fe /* 60*n_elem */);
+/* The same element vectors summed into one force per node of the
+ * ggem_set_fe_mesh mesh, elements ascending (rhs->add_vector, :513); with
+ * condense != 0 the periodic images are condensed as constrain_element_vector
+ * does (each min-face node sums its images' incidences in element order, the
+ * max-face rows are zero). */
+int ggem_assemble_rhs(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, const ggem_rhs_opts* o, int32_t condense,
+                      double* F_node /* 3*n_nodes */);
+
 /* ---- (c) spreading ---------------------------------------------------- */
 /* AssembleStokes::compute_element_rhs point-force branch for every element
  * (assemble_stokes.h:134-141, assemble_stokes.C:643-672 via
@@ -459,6 +503,20 @@ int ggem_util_rsa_positions(const ggem_params* params, int64_t n, double min_dis
                             double* xyz);
 /* n i.i.d. N(0,1) values from std::mt19937_64(seed). */
 int ggem_util_normal(int64_t n, uint64_t seed, double* out);
+/* The (element, local node) items of the wall and inlet sides of an
+ * n_elem[0] x n_elem[1] x n_elem[2] box grid for these options, as
+ * ggem_assemble_elem_rhs visits them: items[2q] = element, items[2q+1] =
+ * l | wall side mask << 8 | inlet side mask << 16 (bit s for hex side s:
+ * 0 z-, 1 y-, 2 x+, 3 y+, 4 x-, 5 z+), elements and nodes ascending.  The
+ * inlet masks are set for the undisturbed option only.  items may be NULL
+ * to count. */
+int ggem_util_rhs_wall_items(const int32_t* n_elem /* 3 */, const int32_t* periodic /* 3 */,
+                             const ggem_rhs_opts* o, int64_t* m, int32_t* items /* 2m */);
+/* JxW_face phi_l of every side s of a HEX20 element of size h[3] at its
+ * 2 x 2 Gauss points (QGauss(2, SECOND) on the side, the side's first
+ * reference axis fastest): tab[(s 4 + qp) 20 + l], the inlet traction of
+ * compute_element_rhs (assemble_stokes.C:687-727). */
+int ggem_util_hex20_traction_table(const double* h /* 3 */, double* tab /* 480 */);
 /* IF table of a HEX20 element of size h[3] with nqp_1d^3 Gauss points:
  * IF[l*nqp + qp] = JxW[qp] * phi_l(qp), libMesh HEX20 node order. */
 int ggem_util_hex20_if_table(const double* h, int32_t nqp_1d, double* IF /* 20*nqp */);
