@@ -65,7 +65,8 @@ EXPORTS = [
     "ggem_util_noise_draw", "ggem_util_noise_skip", "ggem_update_positions",
     "ggem_set_bonds", "ggem_set_constant_forces", "ggem_set_wall_particles", "ggem_apply_fixes",
     "ggem_local_velocity_points_noalloc", "ggem_assemble_elem_rhs", "ggem_assemble_rhs",
-    "ggem_util_rhs_wall_items", "ggem_util_hex20_traction_table",
+    "ggem_util_rhs_wall_items", "ggem_util_hex20_traction_table", "ggem_velocity_extrema",
+    "ggem_check_chain",
 ]
 
 
@@ -213,6 +214,8 @@ def lib():
             "ggem_move_particles": [P, P],
             "ggem_get_positions": [P, P, P, P],
             "ggem_set_unwrapped": [P, P],
+            "ggem_velocity_extrema": [P, I32, P, P],
+            "ggem_check_chain": [P, D, P],
             "ggem_get_noise": [P, P],
             "ggem_update_positions": [P, P, P],
             "ggem_set_bonds": [P, I64, P],
@@ -700,6 +703,20 @@ class Context:
     def set_unwrapped(self, rout=None):
         r = None if rout is None else _f64(rout, (-1, 3))
         self._c(lib().ggem_set_unwrapped(self._h, _p(r)))
+
+    def velocity_extrema(self, forces=False):
+        """(max |v_i|, min |v_i|) of the last combined velocity (forces: of
+        the context's forces)."""
+        hi, lo = C.c_double(), C.c_double()
+        self._c(lib().ggem_velocity_extrema(self._h, int(bool(forces)), C.byref(hi), C.byref(lo)))
+        return hi.value, lo.value
+
+    def check_chain(self, Ls, sync=True):
+        """PolymerChain::check_chain over the bonds; True if a bond was at
+        least Ls (None when sync is False: nothing is synchronised)."""
+        b = C.c_int32(0)
+        self._c(lib().ggem_check_chain(self._h, float(Ls), C.byref(b) if sync else None))
+        return bool(b.value) if sync else None
 
     def lanczos_bounds(self, iters=30, seed=7, op=None):
         lo, hi = C.c_double(), C.c_double()

@@ -410,6 +410,9 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->stale_slot);
   f(ctx->bond_off);
   f(ctx->bond_inc);
+  f(ctx->bond_list);
+  f(ctx->chain_flag);
+  f(ctx->vext);
   f(ctx->cforce);
   f(ctx->wall_xyz);
   f(ctx->x_saved);

@@ -290,6 +290,11 @@ struct ggem_ctx {
   int4* bond_inc = nullptr;     // (bead 1, bead 2, 1 for bead 2's incidence, 0)
   size_t bond_off_cap = 0, bond_inc_cap = 0;
   int64_t bond_n = -1;          // particles the bonds were set for
+  int2* bond_list = nullptr;    // [n_bonds] (bead 1, bead 2) in bond order
+  size_t bond_list_cap = 0;
+  int64_t n_bonds = 0;
+  int32_t* chain_flag = nullptr;  // device flag of ggem_check_chain
+  double* vext = nullptr;         // [2 + partials] velocity extrema
   double* cforce = nullptr;     // [3n] constant forces (p_constant)
   size_t cforce_cap = 0;
   int64_t cforce_n = -1;

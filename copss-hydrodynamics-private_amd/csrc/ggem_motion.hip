@@ -123,6 +123,113 @@ __global__ void k_move(int64_t n, MoveArgs m, const double* __restrict__ base, c
   for (int d = 0; d < 3; ++d) x[3 * p + d] = v[d];
 }
 
+// Velocity extrema, stage 1: per block the largest and smallest |U_i|^2
+// (PointMesh::set_bead_velocity keeps norm_sq and takes sqrt of the extremes).
+constexpr int kExtBlocks = 240;
+__global__ void k_vext_partial(int64_t n, const double* __restrict__ u, double* __restrict__ part) {
+  __shared__ double smax[kBlock], smin[kBlock];
+  double mx = -1.0, mn = INFINITY;
+  for (int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += int64_t(gridDim.x) * blockDim.x) {
+    const double a = u[3 * p], b = u[3 * p + 1], c = u[3 * p + 2];
+    const double q = a * a + b * b + c * c;
+    mx = fmax(mx, q);
+    mn = fmin(mn, q);
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 + 2 * blockIdx.x] = smax[0];
+    part[3 + 2 * blockIdx.x] = smin[0];
+  }
+}
+
+// stage 2: one block; out[0] = sqrt(max), out[1] = sqrt(min)
+__global__ void k_vext_final(int nparts, double* __restrict__ part) {
+  __shared__ double smax[kBlock], smin[kBlock];
+  double mx = -1.0, mn = INFINITY;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    mx = fmax(mx, part[2 + 2 * i]);
+    mn = fmin(mn, part[3 + 2 * i]);
+  }
+  smax[threadIdx.x] = mx;
+  smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[0] = sqrt(smax[0]);
+    part[1] = sqrt(smin[0]);
+  }
+}
+
+struct ChainGeom {
+  double bmin[3], bmax[3], L[3];
+  int32_t periodic[3];
+};
+
+// PMPeriodicBoundary::point_vector (pm_periodic_boundary.C:350-374) from
+// bead 1 to bead 2
+__device__ __forceinline__ void bond_vector(const ChainGeom& g, const double* x, int b0, int b1, double* d) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double v = x[3 * int64_t(b1) + k] - x[3 * int64_t(b0) + k];
+    if (g.periodic[k]) {
+      if (v <= -g.L[k] / 2.0) v += g.L[k];
+      if (v > g.L[k] / 2.0) v -= g.L[k];
+    }
+    d[k] = v;
+  }
+}
+
+// any bond at least Ls long sets the flag (the positions are not changed)
+__global__ void k_chain_detect(int64_t nb, ChainGeom g, const int2* __restrict__ bonds, const double* __restrict__ x,
+                               double Ls, int32_t* __restrict__ flag) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int2 bd = bonds[b];
+  double d[3];
+  bond_vector(g, x, bd.x, bd.y, d);
+  if (sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) >= Ls) flag[0] = 1;
+}
+
+// PolymerChain::check_chain's loop, bond by bond in one thread, run only
+// when the detection found a long bond: each correction changes bead 2,
+// which a later bond may read.
+__global__ void k_chain_fix(int64_t nb, ChainGeom g, const int2* __restrict__ bonds, double* __restrict__ x,
+                            double Ls, const int32_t* __restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || flag[0] == 0) return;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int2 bd = bonds[b];
+    double d[3];
+    bond_vector(g, x, bd.x, bd.y, d);
+    const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (len < Ls) continue;
+    const double s = Ls * 0.9 / len;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double v = x[3 * int64_t(bd.x) + k] + d[k] * s;
+      if (g.periodic[k]) {  // correct_position
+        if (v < g.bmin[k]) v += g.L[k];
+        if (v >= g.bmax[k]) v -= g.L[k];
+      }
+      x[3 * int64_t(bd.y) + k] = v;
+    }
+  }
+}
+
 int ensure_motion(ggem_ctx* ctx) {
   if (ctx->motion_cap >= ctx->n && ctx->x_saved) return GGEM_OK;
   size_t c1 = 0, c2 = 0, c3 = 0;
@@ -307,6 +414,61 @@ int ggem_get_positions(ggem_ctx* ctx, double* xyz, double* unwrapped, int32_t* i
     if (images && (rc = d2h(ctx, images, ctx->images, sizeof(int32_t) * n3))) return rc;
   }
   return rc;
+}
+
+int ggem_velocity_extrema(ggem_ctx* ctx, int32_t which, double* vmax, double* vmin) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (which != 0 && which != 1) return fail(ctx, GGEM_ERR_INVALID_ARG, "which must be 0 (velocity) or 1 (forces)");
+  if (which == 0 && !ctx->u_out_valid)
+    return fail(ctx, GGEM_ERR_NOT_READY, "no velocity: call ggem_combine_velocity first");
+  const double* v = which == 0 ? ctx->u_out : ctx->f_in;
+  double out[2] = {0.0, 0.0};
+  if (ctx->n > 0) {
+    GGEM_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->vext) GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->vext), sizeof(double) * (2 + 2 * kExtBlocks)));
+    const int nb = int(std::min<int64_t>(kExtBlocks, (ctx->n + kBlock - 1) / kBlock));
+    ProfScope ps(ctx, "velocity_extrema");
+    hipLaunchKernelGGL(k_vext_partial, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->n, v, ctx->vext);
+    hipLaunchKernelGGL(k_vext_final, dim3(1), dim3(kBlock), 0, ctx->stream, nb, ctx->vext);
+    GGEM_HIP(ctx, hipGetLastError());
+    int rc = d2h(ctx, out, ctx->vext, sizeof(out));
+    if (rc) return rc;
+  }
+  if (vmax) *vmax = out[0];
+  if (vmin) *vmin = out[1];
+  return GGEM_OK;
+}
+
+int ggem_check_chain(ggem_ctx* ctx, double Ls, int32_t* broken) {
+  if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
+  if (ctx->nsys > 1) return fail(ctx, GGEM_ERR_INVALID_ARG, "ggem_check_chain needs a single-system context");
+  if (!(Ls > 0)) return fail(ctx, GGEM_ERR_INVALID_ARG, "Ls must be positive");
+  if (ctx->bond_n != ctx->n) return fail(ctx, GGEM_ERR_NOT_READY, "no bonds for these particles (ggem_set_bonds)");
+  if (broken) *broken = 0;
+  if (ctx->n_bonds == 0) return GGEM_OK;
+  GGEM_HIP(ctx, hipSetDevice(ctx->device));
+  if (!ctx->chain_flag) GGEM_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&ctx->chain_flag), sizeof(int32_t)));
+  ChainGeom g{};
+  for (int d = 0; d < 3; ++d) {
+    g.bmin[d] = ctx->p.box_min[d];
+    g.bmax[d] = ctx->p.box_max[d];
+    g.L[d] = ctx->grid.L[d];
+    g.periodic[d] = ctx->grid.periodic[d];
+  }
+  {
+    ProfScope ps(ctx, "check_chain");
+    GGEM_HIP(ctx, hipMemsetAsync(ctx->chain_flag, 0, sizeof(int32_t), ctx->stream));
+    hipLaunchKernelGGL(k_chain_detect, dim3(blocks_for(ctx->n_bonds)), dim3(kBlock), 0, ctx->stream, ctx->n_bonds, g,
+                       ctx->bond_list, ctx->x_in, Ls, ctx->chain_flag);
+    hipLaunchKernelGGL(k_chain_fix, dim3(1), dim3(64), 0, ctx->stream, ctx->n_bonds, g, ctx->bond_list, ctx->x_in, Ls,
+                       ctx->chain_flag);
+    GGEM_HIP(ctx, hipGetLastError());
+  }
+  // the positions may have changed: as after a move
+  ctx->slots_current = false;
+  ctx->u_out_valid = false;
+  if (broken) return d2h(ctx, broken, ctx->chain_flag, sizeof(int32_t));
+  return GGEM_OK;
 }
 
 int ggem_set_unwrapped(ggem_ctx* ctx, const double* rout) {

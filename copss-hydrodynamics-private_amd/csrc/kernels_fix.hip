@@ -244,13 +244,18 @@ int ggem_set_bonds(ggem_ctx* ctx, int64_t n_bonds, const int64_t* bonds) {
       inc[pos[b2]++] = make_int4(b1, b2, 1, 0);
     }
   }
+  std::vector<int2> list(std::max<int64_t>(n_bonds, 1));
+  for (int64_t b = 0; b < n_bonds; ++b) list[b] = make_int2(int32_t(bonds[2 * b]), int32_t(bonds[2 * b + 1]));
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   int rc = ensure(ctx, &ctx->bond_off, &ctx->bond_off_cap, size_t(n + 1));
   if (!rc) rc = ensure(ctx, &ctx->bond_inc, &ctx->bond_inc_cap, inc.size());
+  if (!rc) rc = ensure(ctx, &ctx->bond_list, &ctx->bond_list_cap, list.size());
   if (rc) return rc;
   if ((rc = h2d(ctx, ctx->bond_off, off.data(), sizeof(int64_t) * (n + 1)))) return rc;
   if ((rc = h2d(ctx, ctx->bond_inc, inc.data(), sizeof(int4) * inc.size()))) return rc;
+  if ((rc = h2d(ctx, ctx->bond_list, list.data(), sizeof(int2) * list.size()))) return rc;
   ctx->bond_n = n;
+  ctx->n_bonds = n_bonds;
   return sync(ctx);
 }
 

@@ -446,6 +446,22 @@ int ggem_get_positions(ggem_ctx* ctx, double* xyz, double* unwrapped, int32_t* i
 /* ROUT and image counts restart from rout (3n; NULL: the current
  * positions, zero counts), as Copss::run reads vector_RIN (copss.C:1392-1413). */
 int ggem_set_unwrapped(ggem_ctx* ctx, const double* rout);
+/* PointMesh::set_bead_velocity's extrema (point_mesh.C:1042-1061): the
+ * largest and smallest |v_i| over the particles, the input of
+ * PMSystemStokes::get_dt's adaptive step (pm_system_stokes.C:77-134).
+ * which = 0: v = the velocity of the last ggem_combine_velocity(_fe)
+ * (fixman_integrate, copss.C:1476-1491); which = 1: v = the context's
+ * forces (langevin_integrate takes the force as the velocity, :1812-1822).
+ * Either output may be NULL. */
+int ggem_velocity_extrema(ggem_ctx* ctx, int32_t which, double* vmax, double* vmin);
+/* PolymerChain::check_chain (polymer_chain.C:1826-1867) on the context's
+ * positions, over the bonds of ggem_set_bonds in bond order: a bond whose
+ * minimum-image length is at least Ls moves its bead 2 to 0.9 Ls from bead
+ * 1 along the bond, then into the box on periodic axes
+ * (PMPeriodicBoundary::correct_position, pm_periodic_boundary.C:305-328);
+ * later bonds see the moved bead.  broken (may be NULL, then nothing is
+ * synchronised) = 1 if some bond was at least Ls. */
+int ggem_check_chain(ggem_ctx* ctx, double Ls, int32_t* broken);
 
 /* ---- batched suspensions ---------------------------------------------- */
 /* Many independent suspensions in one context: the particles, cells and
