@@ -248,6 +248,34 @@ def bench_slit(G, W, args):
     busy_ms, _ = ctx.profile_get("*")
     stats = dict(dev.stats)
     ctx.close()
+    # the same steps with the right-hand side of each half-step's disturbed
+    # solve assembled on the device: 27-point spreading, the wall penalty
+    # rows from the local velocity at the wall side-nodes (assemble_stokes.C:
+    # 889-996, :957-959) and the condensed nodal sums, kept on the device
+    ctx = G.Context(cfg)
+    node_xyz, conn = G.hex20_box_mesh(cfg)
+    h = (np.array(cfg["box_max"], float) - np.array(cfg["box_min"], float)) / np.array(cfg["n_elem"])
+    rhs = dict(nqp_1d=3, IF=G.hex20_if_table(h, 3), opts=G.rhs_opts(), n_nodes=node_xyz.shape[0])
+    devr = F.DeviceFixman(ctx, x0, F.DeviceSlitForces(64), dt=1e-3, cheb_order=30, rhs=rhs)
+    ctx.set_fe_mesh(node_xyz.shape[0], conn)
+    devr.step()
+    ctx.sync()
+    nr = max(args.slit_steps // 4, 1)
+    t7 = time.perf_counter()
+    for _ in range(nr):
+        devr.step()
+    ctx.sync()
+    t8 = time.perf_counter()
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(nprof):
+        devr.step()
+    ctx.sync()
+    ctx.profile(False)
+    rhs_ms = {k: ctx.profile_get(k)[0] / nprof for k in ("spread_fp32c", "rhs_item_points", "pair_points_fp32c",
+                                                         "rhs_penalty", "gather_nodes")}
+    n_items = int(len(G.rhs_wall_items(cfg["n_elem"], cfg["periodic"], rhs["opts"])[0]))
+    ctx.close()
     # the host-array driver (positions, forces and noise cross each step)
     ctx = G.Context(cfg)
     host = F.FixmanIntegrator(cfg, F.DeviceMobility(ctx), F.SlitPolymerForces(cfg, 400, 64), dt=1e-3,
@@ -266,6 +294,10 @@ def bench_slit(G, W, args):
             "brownian_steps_per_s": 1e3 / ms, "gpu_busy_fraction": busy_ms / ((t4 - t3) * 1e3),
             "cheb_retries": stats["cheb_retries"], "cheb_failures": stats["cheb_failures"],
             "host_array_driver_ms_per_step": (t6 - t5) * 1e3 / nh,
+            "with_rhs": {"ms_per_step": (t8 - t7) * 1e3 / nr, "fe_nodes": int(node_xyz.shape[0]),
+                         "wall_side_nodes": n_items, "rhs_kernels_ms_per_step": rhs_ms,
+                         "note": "two right-hand sides per step (one per half-step solve); the FEM solve itself "
+                                 "is the reference's host side"},
             "note": "reference-mode Chebyshev: host-side stopping decisions each term (one small copy each)"}
 
 

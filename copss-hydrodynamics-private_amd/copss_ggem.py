@@ -575,11 +575,12 @@ class Context:
         self._c(lib().ggem_assemble_elem_rhs(self._h, nqp_1d, _p(IF), C.byref(opts), _p(Fe)))
         return Fe
 
-    def assemble_rhs(self, nqp_1d, IF, opts, n_nodes, condense=True):
+    def assemble_rhs(self, nqp_1d, IF, opts, n_nodes, condense=True, out=True):
         """The element vectors summed into F[n_nodes, 3] over the velocity
-        mesh, periodic images condensed onto the min faces when condense."""
+        mesh, periodic images condensed onto the min faces when condense
+        (out=False: kept on the device)."""
         IF = None if IF is None else _f64(IF).ravel()
-        F = np.zeros((int(n_nodes), 3))
+        F = np.zeros((int(n_nodes), 3)) if out else None
         self._c(lib().ggem_assemble_rhs(self._h, nqp_1d, _p(IF), C.byref(opts), int(bool(condense)), _p(F)))
         return F
 

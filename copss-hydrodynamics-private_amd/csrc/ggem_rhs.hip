@@ -456,11 +456,11 @@ int ggem_assemble_rhs(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, const gge
                       double* F_node) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   if (ctx->fe_nodes < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no FE mesh (call ggem_set_fe_mesh)");
-  if (!F_node) return fail(ctx, GGEM_ERR_INVALID_ARG, "null output");
   int rc = rhs_fill(ctx, nqp_1d, IF, o, true);
   if (rc) return rc;
   const hipError_t e = launch_gather_nodes(ctx, pools(ctx).fe, ctx->fe_u, condense != 0);
   if (e != hipSuccess) return hip_fail(ctx, e, "gather_nodes");
+  if (!F_node) return GGEM_OK;  // stays in the context for the caller's solve
   return d2h(ctx, F_node, ctx->fe_u, sizeof(double) * 3 * size_t(ctx->fe_nodes));
 }
 

@@ -241,12 +241,16 @@ class DeviceFixman:
     DeviceSlitForces, a list of (GGEM_FIX_* type, params) for
     ggem_apply_fixes (bonds and constant forces set on the context
     beforehand), or None for no forces; bounds: fixed (lmin, lmax), or None
-    for Lanczos on the device mobility."""
+    for Lanczos on the device mobility; rhs: None, or dict(nqp_1d, IF,
+    opts, n_nodes) to assemble the global right-hand side after every
+    reinit (ggem_assemble_rhs, kept on the device; the FE mesh set on the
+    context), as each half-step's disturbed solve does."""
 
     def __init__(self, ctx, x0, fixes, dt, cheb_order=30, tol_cheb=0.1, eig_factor=1.05, seed=123456789,
                  n_relax_step=1, bounds=None, lanczos_iters=40, lanczos_seed=11, wall_type=G.WALL_SLIT,
-                 cavity_radius=0.0):
+                 cavity_radius=0.0, rhs=None):
         self.ctx, self.fixes = ctx, fixes
+        self.rhs = rhs
         self.dt, self.N, self.tol, self.factor = dt, cheb_order, tol_cheb, eig_factor
         self.n_relax = n_relax_step
         self.fixed_bounds = bounds
@@ -268,6 +272,11 @@ class DeviceFixman:
             self.ctx.apply_fixes(self.fixes, out=False)
         elif self.fixes is not None:
             self.ctx.fix_forces(out=False, **self.fixes.params)
+        if self.rhs is not None:
+            # the right-hand side of the half-step's disturbed solve (the
+            # spreading and the wall penalty rows, kept on the device)
+            r = self.rhs
+            self.ctx.assemble_rhs(r["nqp_1d"], r["IF"], r["opts"], r["n_nodes"], condense=True, out=False)
 
     def _bounds(self):
         lmin, lmax = self.fixed_bounds or self.ctx.lanczos_bounds(iters=self.iters, seed=self.lseed)

@@ -225,7 +225,8 @@ int ggem_assemble_elem_rhs(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, cons
  * ggem_set_fe_mesh mesh, elements ascending (rhs->add_vector, :513); with
  * condense != 0 the periodic images are condensed as constrain_element_vector
  * does (each min-face node sums its images' incidences in element order, the
- * max-face rows are zero). */
+ * max-face rows are zero).  F_node may be NULL: the sums stay on the device
+ * (queued, not synchronised). */
 int ggem_assemble_rhs(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, const ggem_rhs_opts* o, int32_t condense,
                       double* F_node /* 3*n_nodes */);
 
