@@ -71,8 +71,8 @@ def profiled_traffic(kernel):
     try:
         with open(PROFILE_TRAFFIC) as fh:
             for line in fh:
-                if line.startswith(kernel):
-                    return float(line.split()[-1]) * 1e6
+                if line.startswith(kernel):  # columns ... total MB, avg us, TB/s
+                    return float(line.split()[-3]) * 1e6
     except OSError:
         pass
     return None
