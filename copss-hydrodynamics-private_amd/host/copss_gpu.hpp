@@ -11,6 +11,12 @@
 //   BrownianSystem <- BrownianSystem (brownian_system.h):
 //                    chebyshev_polynomial_approximation, hi_ewald,
 //                    compute_eigenvalues
+//   AssembleStokes <- AssembleStokes (assemble_stokes.h): assemble_global_F
+//                    (spreading or inlet traction, penalty rows, periodic
+//                    constraints) into one force per velocity node
+//   FixFactory    <- FixFactory::buildFix (fix/fix_factory.C:4-84) with each
+//                    fix's initParams; PointMesh::apply_fixes runs the fixes'
+//                    compute() as reinit_system does
 //
 // The reference computes one bead or one element per call; here a call
 // computes every bead / element once on the GPU and later calls for other
@@ -19,6 +25,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -85,6 +92,21 @@ class PointMesh {
     ++version_;
   }
 
+  // The fixes' compute() of PMSystemStokes::reinit_system
+  // (pm_system_stokes.C:178-185) after a reinit: their summed forces become
+  // the particle forces (the reinit zeroed them).  The bonds of the springs
+  // (bead 1, bead 2 pairs, 0-based) are set once the particles are on the
+  // device, i.e. after the first reinit.
+  void set_bonds(const std::vector<int64_t>& bonds) {
+    if (pending_) throw std::runtime_error("copss_ggem: set_bonds needs a reinit after set_particles");
+    check(ggem_set_bonds(ctx_, int64_t(bonds.size() / 2), bonds.data()), ctx_);
+  }
+  void apply_fixes(const std::vector<ggem_fix>& fixes) {
+    if (pending_) throw std::runtime_error("copss_ggem: apply_fixes needs a reinit after set_particles");
+    force_.assign(3 * n_, 0.0);
+    check(ggem_apply_fixes(ctx_, int32_t(fixes.size()), fixes.data(), force_.data()), ctx_);
+    ++version_;
+  }
   std::size_t num_particles() const { return n_; }
   const std::vector<double>& positions() const { return xyz_; }
   const std::vector<double>& forces() const { return force_; }
@@ -265,6 +287,155 @@ class BrownianSystem {
   PointMesh* pm_;
   ggem_op op_{nullptr, nullptr};
   ggem_cheb_info last_{};
+};
+
+// The settings of the right-hand side that the reference reads from its
+// equation_systems parameters (shear, shear_direction, shear_rate, inlet,
+// inlet_pressure, simulation_name) and its quadrature (the IF table of
+// assemble_int_force, assemble_stokes.C:733-757).
+struct StokesRhsParams {
+  int nqp_1d = 3;
+  std::vector<double> IF;  // 20 * nqp_1d^3
+  std::array<bool, 3> shear{{false, false, false}}, inlet{{false, false, false}};
+  std::array<int, 3> shear_direction{{0, 0, 0}};
+  std::array<double, 3> shear_rate{{0, 0, 0}}, inlet_pressure{{0, 0, 0}};
+  std::string simulation_name = "general";
+};
+
+class AssembleStokes {
+ public:
+  explicit AssembleStokes(const StokesRhsParams& p) : p_(p) {}
+
+  // AssembleStokes::assemble_global_F (assemble_stokes.C:473-516) for
+  // option "disturbed" or "undisturbed": the element vectors (spreading or
+  // inlet traction, then apply_bc_by_penalty), constrain_element_vector
+  // with the periodic constraints, and rhs->add_vector, returned as one
+  // force per node of the velocity mesh set by
+  // GGEMStokes::set_velocity_mesh (F_node[3 node + d]).
+  void assemble_global_F(PointMesh* pm, const std::string& option, std::vector<double>& F_node) const {
+    if (n_nodes_ <= 0) throw std::runtime_error("assemble_global_F: set_node_count of the velocity mesh first");
+    ggem_rhs_opts o = opts(option);
+    F_node.assign(3 * size_t(n_nodes_), 0.0);
+    check(ggem_assemble_rhs(pm->ctx(), p_.nqp_1d, p_.IF.empty() ? nullptr : p_.IF.data(), &o, 1, F_node.data()),
+          pm->ctx());
+  }
+  // The element vectors before the constraints, Fe[e][d][l] (the Fe of the
+  // element loop after apply_bc_by_penalty).
+  void element_vectors(PointMesh* pm, const std::string& option, std::vector<double>& Fe) const {
+    ggem_rhs_opts o = opts(option);
+    Fe.assign(size_t(pm->n_elem()) * 60, 0.0);
+    check(ggem_assemble_elem_rhs(pm->ctx(), p_.nqp_1d, p_.IF.empty() ? nullptr : p_.IF.data(), &o, Fe.data()),
+          pm->ctx());
+  }
+  // the node count of the mesh given to GGEMStokes::set_velocity_mesh
+  void set_node_count(int64_t n_nodes) { n_nodes_ = n_nodes; }
+
+ private:
+  ggem_rhs_opts opts(const std::string& option) const {
+    if (option != "disturbed" && option != "undisturbed")
+      throw std::runtime_error("assemble_global_F: invalid option " + option);
+    ggem_rhs_opts o{};
+    o.option = option == "disturbed" ? GGEM_RHS_DISTURBED : GGEM_RHS_UNDISTURBED;
+    o.validation = p_.simulation_name == "ggem_validation" ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+      o.shear[a] = p_.shear[a];
+      o.shear_direction[a] = p_.shear_direction[a];
+      o.shear_rate[a] = p_.shear_rate[a];
+      o.inlet[a] = p_.inlet[a];
+      o.inlet_pressure[a] = p_.inlet_pressure[a];
+    }
+    o.penalty = 1E6;  // assemble_stokes.C:910
+    return o;
+  }
+  StokesRhsParams p_;
+  int64_t n_nodes_ = 0;
+};
+
+// What FixFactory::buildFix's fixes read from the equation_systems
+// parameters in their initParams.
+struct FixParams {
+  std::string point_particle_model = "polymer_chain";
+  std::string wall_type = "slit";
+  std::vector<double> wall_params;  // slit: 6 box bounds; sphere: radius
+  double bead_radius = 0, bk = 0, Nks = 0, Ss2 = 0;
+};
+
+class FixFactory {
+ public:
+  // FixFactory::buildFix (fix/fix_factory.C:4-84) for point particles, with
+  // the parameters of the fix's initParams from force_params (the control
+  // file's values for that fix) and the system's parameters.
+  static ggem_fix buildFix(const std::string& fix_name, const std::vector<double>& force_params,
+                           const FixParams& sp) {
+    ggem_fix f{};
+    auto need = [&](std::size_t k) {
+      if (force_params.size() != k)
+        throw std::runtime_error("The force type '" + fix_name + "' requires " + std::to_string(k) + " parameter");
+    };
+    auto polymer_only = [&]() {
+      if (sp.point_particle_model != "polymer_chain")
+        throw std::runtime_error("The force field '" + fix_name + "' is only for polymer_chain");
+    };
+    auto copy = [&](int k) {
+      for (int i = 0; i < k; ++i) f.p[i] = force_params[i];
+    };
+    if (fix_name == "lj_cut") {  // fix_point_lj_cut.C
+      need(3);
+      f.type = GGEM_FIX_LJ_CUT;
+      copy(3);
+    } else if (fix_name == "gaussian") {  // fix_point_gaussian.C:14-30
+      need(2);
+      f.type = GGEM_FIX_GAUSSIAN;
+      copy(2);
+    } else if (fix_name == "gaussian_dna") {  // fix_point_gaussian_dna.C:31-58
+      polymer_only();
+      need(1);
+      const double r = sp.bead_radius, ev = force_params[0] * r * r * r;
+      const double c1_tmp = 3. / (4. * M_PI * sp.Ss2), c1_tmp3 = c1_tmp * c1_tmp * c1_tmp;
+      f.type = GGEM_FIX_GAUSSIAN;
+      f.p[0] = ev * sp.Nks * sp.Nks * std::sqrt(c1_tmp3);
+      f.p[1] = 3. * r * r / (4. * sp.Ss2);
+    } else if (fix_name == "worm_like_spring") {  // fix_point_wls.C:31-60
+      polymer_only();
+      need(0);
+      f.type = GGEM_FIX_WORM_LIKE_SPRING;
+      f.p[0] = sp.bead_radius / (2.0 * sp.bk);
+      f.p[1] = sp.Nks * sp.bk / sp.bead_radius;
+    } else if (fix_name == "wall/lj_cut") {
+      need(3);
+      copy(3);
+      if (sp.wall_type == "slit") {
+        f.type = GGEM_FIX_SLIT_WALL_LJ_CUT;
+      } else if (sp.wall_type == "sphere") {
+        f.type = GGEM_FIX_SPHERE_WALL_LJ_CUT;
+        f.p[3] = sp.wall_params.at(0);
+      } else {
+        throw std::runtime_error("Error: FixFactory::buildFix: undefined wall_type for point_particles: " + fix_name);
+      }
+    } else if (fix_name == "discretized_wall/lj_cut") {  // wall particles via PointMesh's context
+      need(4);
+      f.type = GGEM_FIX_DISCRETIZED_WALL_LJ_CUT;
+      copy(3);
+    } else if (fix_name == "wall/empirical_dna") {  // fix_point_slitWall_empirical_dna.C:35-62
+      polymer_only();
+      const double c1 = sp.bead_radius / sp.bk, c2 = c1 / std::sqrt(sp.Nks);
+      f.p[0] = 25.0 * c1;
+      f.p[1] = 0.5 / c2;
+      if (sp.wall_type == "slit") {
+        f.type = GGEM_FIX_SLIT_WALL_EMPIRICAL_DNA;
+      } else if (sp.wall_type == "sphere") {
+        f.type = GGEM_FIX_SPHERE_WALL_EMPIRICAL_DNA;
+        f.p[2] = sp.wall_params.at(0);
+      } else {
+        throw std::runtime_error("Error: FixFactory::buildFix: undefined wall_type for point_particles: " + fix_name);
+      }
+    } else if (fix_name == "p_constant") {
+      f.type = GGEM_FIX_P_CONSTANT;
+    } else {
+      throw std::runtime_error("Error: undefined force field for point_particles: " + fix_name);
+    }
+    return f;
+  }
 };
 
 }  // namespace copss_amd

@@ -57,6 +57,20 @@ int main() {
       for (int d = 0; d < 3; ++d) ub.push_back(u[d] - ex[d]);
     }
     print_vec("ggem3_bead_minus_exclusion", ub);
+    // AssembleStokes::assemble_global_F's element vectors on the walled box:
+    // spreading at 8 Gauss points, penalty rows with sheared z walls
+    StokesRhsParams rp;
+    rp.nqp_1d = 2;
+    rp.IF.resize(20 * 8);
+    const double h2[3] = {2, 2, 2};
+    ggem_util_hex20_if_table(h2, 2, rp.IF.data());
+    rp.shear = {{false, false, true}};
+    rp.shear_direction = {{0, 0, 0}};
+    rp.shear_rate = {{0, 0, 0.05}};
+    AssembleStokes as(rp);
+    std::vector<double> Fe_w;
+    as.element_vectors(&pm, "disturbed", Fe_w);
+    print_vec("ggem3_rhs_Fe", Fe_w);
     std::printf("\"update_flag_cleared\": %d,\n", update ? 0 : 1);
     bool threw = false;
     try {
@@ -142,7 +156,43 @@ int main() {
       Point u = ggem.local_solution_bead(&pm, b, "regularized");
       u3.insert(u3.end(), u.begin(), u.end());
     }
-    print_vec("b100_reinit_true_u", u3, true);
+    print_vec("b100_reinit_true_u", u3);
+    // the periodic box's nodal right-hand side, condensed (every element
+    // with its own 20 nodes, as set above)
+    StokesRhsParams rq;
+    rq.nqp_1d = 3;
+    rq.IF = IF;
+    AssembleStokes aq(rq);
+    aq.set_node_count(20 * 8000);
+    std::vector<double> F_node;
+    aq.assemble_global_F(&pm, "disturbed", F_node);
+    print_vec("b100_rhs_F_node", F_node);
+    // fixes built by FixFactory from control-file style parameters, bonds
+    // along chains of 10 beads
+    std::vector<int64_t> bonds;
+    for (int64_t c = 0; c < 10; ++c)
+      for (int64_t i = 0; i < 9; ++i) {
+        bonds.push_back(10 * c + i);
+        bonds.push_back(10 * c + i + 1);
+      }
+    pm.set_bonds(bonds);
+    FixParams sp;
+    sp.bead_radius = 0.077;
+    sp.bk = 0.106;
+    sp.Nks = 200.0;
+    sp.Ss2 = sp.Nks * sp.bk * sp.bk / 6.0;
+    std::vector<ggem_fix> fixes = {FixFactory::buildFix("gaussian_dna", {2.628506592}, sp),
+                                   FixFactory::buildFix("worm_like_spring", {}, sp),
+                                   FixFactory::buildFix("lj_cut", {1.0, 1.0, 2.5}, sp)};
+    pm.apply_fixes(fixes);
+    print_vec("b100_fix_forces", pm.forces());
+    bool bad_fix = false;
+    try {
+      FixFactory::buildFix("spring", {}, sp);
+    } catch (const std::runtime_error&) {
+      bad_fix = true;
+    }
+    std::printf("\"unknown_fix_throws\": %d\n", bad_fix ? 1 : 0);
   }
   std::printf("}\n");
   return 0;

@@ -82,3 +82,47 @@ def test_b100_reinit_keeps_lists_without_the_flag(demo):
     m2 = O.Mesh(ocfg)
     m2.reinit(x2, f)
     assert rel_l2(np.array(demo["b100_reinit_true_u"]).reshape(-1, 3), m2.local_velocity_beads()) < 1e-12
+
+
+def test_assemble_stokes_and_fix_factory(demo, golden):
+    """AssembleStokes::assemble_global_F (element vectors with sheared walls;
+    condensed nodal sums on the periodic box) and FixFactory::buildFix +
+    PointMesh::apply_fixes through the C++ classes, against the oracle's
+    right-hand side (oracle/rhs.py) and fixes (oracle/fixes.py) with the
+    parameters of copss_run.build_fixes."""
+    import types
+
+    import copss_run as R
+    import fixes as FX
+    import rhs as RH
+    inp, _ = golden
+    cfg = dict(mu=1.0 / (6.0 * np.pi), alpha=0.35, box_min=[-15] * 3, box_max=[15] * 3, periodic=[0] * 3,
+               n_elem=[15] * 3, keep_stale_lists=0)
+    m = O.Mesh(cfg)
+    m.reinit(np.array(inp["beads"]), np.array(inp["forces"]))
+    IF2 = G.hex20_if_table([2.0, 2.0, 2.0], 2)
+    ref = RH.elem_rhs(cfg, m, 2, IF2, shear=(0, 0, 1), shear_direction=(0, 0, 0), shear_rate=(0.0, 0.0, 0.05))
+    Fe = np.array(demo["ggem3_rhs_Fe"]).reshape(ref.shape)
+    assert np.abs(Fe - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the periodic box at the rebuilt positions x2
+    wcfg, x, f = W.make_inputs("beads100")
+    ocfg = {k: wcfg[k] for k in ("mu", "alpha", "box_min", "box_max", "periodic", "n_elem")}
+    ocfg["keep_stale_lists"] = 0
+    x2 = np.array(demo["b100_x2"]).reshape(-1, 3)
+    m2 = O.Mesh(ocfg)
+    m2.reinit(x2, f)
+    _, Fe2 = m2.spread(3, G.hex20_if_table([2.0, 2.0, 2.0], 3), want_rho=False)
+    conn = np.arange(20 * 8000, dtype=np.int64).reshape(8000, 20)
+    F_ref = RH.nodal_rhs(ocfg, conn, 20 * 8000, Fe2, condense=True)
+    assert rel_l2(np.array(demo["b100_rhs_F_node"]).reshape(-1, 3), F_ref) < 1e-12
+    # fixes
+    ctl = types.SimpleNamespace(Rb=0.077, bk=0.106, Nks=200.0, Ss2=200.0 * 0.106 ** 2 / 6.0,
+                                point_particle_model="polymer_chain", wall_type="slit",
+                                wall_params=[0, 40, 0, 40, 0, 40], box_min=[0, 0, 0], box_max=[40, 40, 40],
+                                forces=[("gaussian_dna", [2.628506592]), ("worm_like_spring", []),
+                                        ("lj_cut", [1.0, 1.0, 2.5])])
+    fixes, _ = R.build_fixes(ctl)
+    bonds = np.array([(10 * c + i, 10 * c + i + 1) for c in range(10) for i in range(9)])
+    fo = FX.apply_fixes(ocfg, x2, fixes, lists=m2.csr(), bonds=bonds)
+    assert rel_l2(np.array(demo["b100_fix_forces"]).reshape(-1, 3), fo) < 1e-12
+    assert demo["unknown_fix_throws"] == 1
