@@ -169,7 +169,6 @@ int prepare_box(ggem_ctx* ctx) {
   if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total[0])))) return rc;
   const size_t ne = size_t(std::max<int64_t>(total[1], 1));
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
-  if ((rc = ensure(ctx, &ctx->box_ab, &ctx->box_ab_cap, ne))) return rc;
   // bricks that are not staged fill their tensors from the lists
   ctx->box_unstaged = mx[2] > 0;
   if (ctx->box_unstaged && (rc = ensure_ell(ctx))) return rc;
@@ -394,7 +393,6 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->box_slot);
   f(ctx->box_pos);
   f(ctx->box_idx);
-  f(ctx->box_ab);
   f(ctx->box_max_d);
   f(ctx->brick_recs);
   f(ctx->pc32);

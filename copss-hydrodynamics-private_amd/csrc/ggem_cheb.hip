@@ -227,7 +227,7 @@ int run_fixed_graph(ggem_ctx* ctx, Applier& apply, FixedStep s) {
       uintptr_t(ctx->x4), uintptr_t(ctx->nbr_cnt), uintptr_t(ctx->slice_off), uintptr_t(ctx->ell),
       uintptr_t(ctx->perm), uintptr_t(ctx->fs), uintptr_t(ctx->tab32), uintptr_t(ctx->coef64),
       uintptr_t(ctx->box_valid), uintptr_t(ctx->box_max), uintptr_t(ctx->box_cnt), uintptr_t(ctx->box_slot),
-      uintptr_t(ctx->box_pos), uintptr_t(ctx->box_idx), uintptr_t(ctx->box_ab), uintptr_t(ctx->cell_start),
+      uintptr_t(ctx->box_pos), uintptr_t(ctx->box_idx), uintptr_t(ctx->cell_start),
       uintptr_t(ctx->box_tgt), uintptr_t(ctx->box_base),
       uintptr_t(s.sys), uintptr_t(ctx->nsys)};
   if (!ctx->cheb_exec || key != ctx->cheb_key) {

@@ -217,8 +217,6 @@ struct ggem_ctx {
   size_t box_rec_cap = 0;
   float4* box_pos = nullptr;      // [records] cell-relative FP32 position, packed box cell
   size_t box_pos_cap = 0;
-  float2* box_ab = nullptr;       // [entries] pair tensor (A, Bx) from the coefficient table, brick layout
-  size_t box_ab_cap = 0;
   uint16_t* box_idx = nullptr;    // [entries] the neighbour's record number (staged bricks)
   size_t box_idx_cap = 0;
   int32_t* brick_recs = nullptr;  // [n_bricks] particles in each brick's box (list count pass)

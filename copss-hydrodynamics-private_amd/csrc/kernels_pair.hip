@@ -85,8 +85,12 @@ __device__ __forceinline__ int fp32c_interval(const CoefTab& t, float s, float& 
 }
 __device__ __forceinline__ bool fp32c_kept(const CoefTab& t, int i) { return i < t.n - 1; }
 
-__device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int i, float y, float& A, float& Bx) {
-  const float4 a0 = tab[4 * i], a1 = tab[4 * i + 1], b0 = tab[4 * i + 2], b1 = tab[4 * i + 3];
+// (A, Bx) at y in interval i of the table's LDS copy, which holds the
+// intervals' coefficient quadruples component by component (load_tab):
+// lanes at different intervals read neighbouring words
+__device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int n, int i, float y, float& A,
+                                           float& Bx) {
+  const float4 a0 = tab[i], a1 = tab[n + i], b0 = tab[2 * n + i], b1 = tab[3 * n + i];
   float pa = __builtin_fmaf(a1.w, y, a1.z);
   pa = __builtin_fmaf(pa, y, a1.y);
   pa = __builtin_fmaf(pa, y, a1.x);
@@ -101,13 +105,6 @@ __device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int i
   pb = __builtin_fmaf(pb, y, b0.z);
   pb = __builtin_fmaf(pb, y, b0.y);
   Bx = __builtin_fmaf(pb, y, b0.x);
-}
-
-__device__ __forceinline__ void fp32c_coef(const float4* __restrict__ tab, const CoefTab& t, float s, float& A,
-                                           float& Bx) {
-  float y;
-  const int i = fp32c_interval(t, s, y);
-  fp32c_eval(tab, i, y, A, Bx);
 }
 
 // r^2 of an FP32C displacement, one fixed evaluation order
@@ -126,9 +123,10 @@ __device__ __forceinline__ void fp32c_acc(float A, float Bx, float dx, float dy,
   p2 += __builtin_fmaf(A, fz, b * dz);
 }
 
-// the interval table into LDS (the whole block takes part)
+// the interval table into LDS, component by component: quadruple c of
+// interval i (a.tab[4 i + c]) to dst[c n + i] (the whole block takes part)
 __device__ __forceinline__ void load_tab(float4* dst, const PairArgs& a) {
-  for (int i = threadIdx.x; i < 4 * a.ct.n; i += blockDim.x) dst[i] = a.tab[i];
+  for (int q = threadIdx.x; q < 4 * a.ct.n; q += blockDim.x) dst[(q & 3) * a.ct.n + (q >> 2)] = a.tab[q];
 }
 
 // Launch shapes the autotuner chooses from (index 0 is the default); every
@@ -422,7 +420,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
       const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
       if (!fp32c_kept(a.ct, it)) return;
       float A, Bx;
-      fp32c_eval(tab, it, y, A, Bx);
+      fp32c_eval(tab, a.ct.n, it, y, A, Bx);
 #pragma unroll
       for (int r = 0; r < NR; ++r) fp32c_acc(A, Bx, dx, dy, dz, f[r].x, f[r].y, f[r].z, ps[r][0], ps[r][1], ps[r][2]);
       ++nk;
@@ -480,15 +478,17 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
 // range of records.  Once per list build k_box_fill writes every staged
 // brick's records (slot, cell-relative FP32 position with the packed box
 // cell) and walks each target's stencil through them in list order
-// (box_walk): per entry the neighbour's record number (16 bits) and (A, Bx)
-// from the coefficient table.  k_apply_box copies the records and their
-// operands to LDS and streams record numbers and tensors (10 bytes per
-// pair), reading the neighbour from LDS.  The summation order (list order)
-// and the arithmetic are k_apply's; the displacement (box_t - box_s) h +
-// (offset_t - offset_s) has the same integer cell difference as the list
-// entry's periodic shift gives, so the results are bit-identical.  A brick
-// whose box holds more than kBoxMaxRecs records reads its list and gathers
-// the neighbours' records from global memory.
+// (box_walk): per kept entry the neighbour's record number (16 bits).
+// k_apply_box copies the records, their operands and the coefficient table
+// to LDS and streams the record numbers (2 bytes per pair): per pair the
+// neighbour from LDS and (A, Bx) from the table at the displacement, a
+// group of entries at a time (records first, then intervals, then tensors
+// and sums) so that the LDS reads of a group overlap.  The summation order
+// (list order) and the arithmetic are k_apply's; the displacement (box_t -
+// box_s) h + (offset_t - offset_s) has the same integer cell difference as
+// the list entry's periodic shift gives, so the results are bit-identical.
+// A brick whose box holds more than kBoxMaxRecs records reads its list and
+// gathers the neighbours' records from global memory.
 constexpr int kBoxThreads = 256;
 constexpr int kBoxMaxRecs = 2048;   // records of a staged brick (64 KiB of LDS)
 // A target's packed entry base: the offset of its entry 0 (below 2^40), its
@@ -514,7 +514,6 @@ struct BoxArgs {
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
   uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
-  float2* ab;            // [entries] pair tensor (A, Bx) from the coefficient table, brick layout
   int32_t cells_cap;     // k_box_fill: box cells, waves and records of a brick its LDS tables hold
   int32_t waves_cap;
   int32_t recs_cap;
@@ -638,8 +637,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_box_max(BoxArgs b, int64_t nbri
 template <bool BATCH>
 __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b) {
   extern __shared__ float4 dyn[];
-  float4* s_tab = dyn;                                                 // coefficient table
-  float4* rpos = dyn + 4 * a.ct.n;                                     // [recs_cap] records of a staged brick
+  float4* rpos = dyn;                                                  // [recs_cap] records of a staged brick
   int32_t* cfirst = reinterpret_cast<int32_t*>(rpos + b.recs_cap);     // [cells_cap] first slot of each box cell
   int32_t* cst = cfirst + b.cells_cap;                                 // [cells_cap + 1] record start of each box cell
   int32_t* wpre = cst + b.cells_cap + 1;                               // [waves_cap + 1] entry rows before each wave
@@ -650,7 +648,6 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
   if (sb >= se) return;
   const int nt = se - sb;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  load_tab(s_tab, a);  // (visible after the barriers below)
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   BoxGeom B;
@@ -714,11 +711,11 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     }
   }
   __syncthreads();
-  // (4) the entries of the brick's targets: the pair tensor (A, Bx) from
-  // the coefficient table at the FP32C displacement, exactly as the
-  // list-order kernel forms both, and in staged bricks the neighbour's
-  // record number.  Staged bricks walk the stencil through the box records
-  // in LDS (list order, box_walk); the others read their lists.
+  // (4) the entries of the staged bricks' targets: the neighbour's record
+  // number of every kept pair (outside the table's zero interval at the
+  // FP32C displacement, formed exactly as the list-order kernel forms it),
+  // walking the stencil through the box records in LDS (list order,
+  // box_walk); the targets of other bricks keep their entry offset only.
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
   const StencilRow* rows = b.rows;
   int nrows = b.nrows;
@@ -760,11 +757,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
             const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
             const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
             float y;
-            const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
-            if (!fp32c_kept(a.ct, it)) continue;
-            float A, Bx;
-            fp32c_eval(s_tab, it, y, A, Bx);
-            b.ab[o] = make_float2(A, Bx);
+            if (!fp32c_kept(a.ct, fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y))) continue;
             b.idx[o] = uint16_t(r);
             o += kSlice;
             ++kept;
@@ -773,42 +766,14 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       b.base[sb + i] = bb | (int64_t(self) << kSelfShift) | (int64_t(kept) << kKeptShift);
       continue;
     }
-    b.base[sb + i] = bb;
-    if (cnt == 0) continue;  // an empty list has no storage
-    // unstaged brick: the list in list order, the neighbour's position
-    // record loaded one entry ahead
-    const int64_t base = a.slice_off[s >> 6] + (s & 63);
-    const int last = cnt - 1;
-    uint32_t e1 = a.ell[base];
-    float4 p1 = a.pc32[e1 & kSlotMask];
-    for (int k = 0; k < cnt; ++k) {
-      const uint32_t e = e1;
-      const float4 pt = p1;
-      e1 = a.ell[base + int64_t(min(k + 1, last)) * kSlice];
-      p1 = a.pc32[e1 & kSlotMask];
-      const uint32_t code = e >> kSlotBits;
-      // the neighbour's unwrapped cell: its cell plus the entry's periodic shift
-      const uint32_t ct = __float_as_uint(pt.w);
-      const int ci = int(ct & ((1u << kCellBitsX) - 1));
-      const int cj = int((ct >> kCellBitsX) & ((1u << kCellBitsY) - 1));
-      const int ck = int(ct >> (kCellBitsX + kCellBitsY));
-      const int ui = ci + (int(code & 1u) - int((code >> 1) & 1u)) * B.n[0];
-      const int uj = cj + (int((code >> 2) & 1u) - int((code >> 3) & 1u)) * B.n[1];
-      const int uk = ck + (int((code >> 4) & 1u) - int((code >> 5) & 1u)) * B.n[2];
-      const float dx = __builtin_fmaf(float(ui - io), hx, pt.x - po.x);
-      const float dy = __builtin_fmaf(float(uj - jo), hy, pt.y - po.y);
-      const float dz = __builtin_fmaf(float(uk - ko), hz, pt.z - po.z);
-      float A, Bx;
-      fp32c_coef(s_tab, a.ct, fp32c_r2(dx, dy, dz), A, Bx);
-      b.ab[bb + int64_t(k) * kSlice] = make_float2(A, Bx);
-    }
+    b.base[sb + i] = bb;  // unstaged brick: the pair sum reads its list
   }
 }
 
-// pair sum of slot s through global gathers (FP32C, one operand)
-__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float2* __restrict__ ab,
-                                                   int64_t bb, const float4* __restrict__ xg,
-                                                   double (&acc)[1][3]) {
+// pair sum of slot s through global gathers (FP32C, one operand; the
+// coefficient table in LDS)
+__device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s, const float4* __restrict__ tab,
+                                                   const float4* __restrict__ xg, double (&acc)[1][3]) {
   const int cnt = a.cnt[s];
   const int64_t base = a.slice_off[s >> 6] + (s & 63);
   const Grid& g = a.sys ? a.sys[a.sys_slot[s]].g : a.g;
@@ -837,9 +802,11 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
     const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
     const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
     float y;
-    if (!fp32c_kept(a.ct, fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y))) continue;  // the zero interval
-    const float2 c = ab[bb + int64_t(k) * kSlice];
-    fp32c_acc(c.x, c.y, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
+    const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
+    if (!fp32c_kept(a.ct, it)) continue;  // the zero interval
+    float A, Bx;
+    fp32c_eval(tab, a.ct.n, it, y, A, Bx);
+    fp32c_acc(A, Bx, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
     if ((++nk & (kMergeEvery - 1)) == 0) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -868,13 +835,16 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   const int nt = se - sb;
-  float4* rpos = lds;         // [nrec] offset xyz, packed box cell
-  float4* rval = lds + nrec;  // [nrec] operand
+  float4* s_tab = lds;                // coefficient table
+  float4* rpos = lds + 4 * a.ct.n;    // [nrec] offset xyz, packed box cell
+  float4* rval = rpos + nrec;         // [nrec] operand
+  load_tab(s_tab, a);
   if (nrec == 0) {
+    __syncthreads();
     for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
       const int64_t s = b.tgt[sb + i];
       double acc[1][3];
-      fp32c_pairs_global(a, s, b.ab, b.base[sb + i] & kBaseMask, xg, acc);
+      fp32c_pairs_global(a, s, s_tab, xg, acc);
       apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
     }
     return;
@@ -906,7 +876,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   BoxGeom B;
   box_init(B, a, b.reach, brick, sb);
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
-  constexpr int U = 5;  // entries per group, the next group loaded one ahead (measured best of 3..10)
+  constexpr int U = 4;  // entries per group, the next group's record numbers loaded one ahead
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = b.tgt[sb + i];
     const int64_t bpk = b.base[sb + i];
@@ -919,17 +889,17 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     const int cnt = int((uint64_t(bpk) >> kKeptShift) & kField12);  // stored (kept) entries
     float ps[3] = {0.f, 0.f, 0.f};
     KahanF kh[3];
-    auto pair = [&](uint32_t rec, float2 c) {
-      const float4 pt = rpos[rec];
-      const float4 f = rval[rec];
+    // displacement to record rec and its interval in the table (every
+    // stored entry is a kept pair)
+    auto disp = [&](const float4& pt, float& dx, float& dy, float& dz, float& y) {
       const uint32_t bt = __float_as_uint(pt.w);
       const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
       const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
       const int dk = int(bt >> (2 * kBoxBits)) - bz;
-      const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
-      const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
-      const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-      fp32c_acc(c.x, c.y, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
+      dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
+      dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
+      dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+      return fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
     };
     auto merge = [&](int k) {
       if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
@@ -940,43 +910,52 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
         }
       }
     };
+    auto one = [&](uint32_t rec, int k) {
+      const float4 pt = rpos[rec];
+      const float4 f = rval[rec];
+      float dx, dy, dz, y, A, Bx;
+      const int it = disp(pt, dx, dy, dz, y);
+      fp32c_eval(s_tab, a.ct.n, it, y, A, Bx);
+      fp32c_acc(A, Bx, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
+      merge(k);
+    };
     if (cnt > 0) {  // an empty list has no storage
       const uint16_t* __restrict__ ci = b.idx + base;
-      const float2* __restrict__ cab = b.ab + base;
       const int last = cnt - 1;
-      uint32_t en[U];  // record numbers and tensors of the next group
-      float2 et[U];
+      uint32_t en[U];  // record numbers of the next group
 #pragma unroll
-      for (int i = 0; i < U; ++i) {
-        const int64_t o = int64_t(min(i, last)) * kSlice;
-        en[i] = ci[o];
-        et[i] = cab[o];
-      }
+      for (int u = 0; u < U; ++u) en[u] = ci[int64_t(min(u, last)) * kSlice];
       int k = 0;
       for (; k + U <= cnt; k += U) {
         uint32_t c[U];
-        float2 t[U];
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
-          c[i] = en[i];
-          t[i] = et[i];
-          const int64_t o = int64_t(min(k + U + i, last)) * kSlice;
-          en[i] = ci[o];
-          et[i] = cab[o];
+        for (int u = 0; u < U; ++u) {
+          c[u] = en[u];
+          en[u] = ci[int64_t(min(k + U + u, last)) * kSlice];
         }
+        // a group: its records, then displacements and intervals, then the
+        // tensors from the table and the sums in list order
+        float4 pt[U], f[U];
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
-          pair(c[i], t[i]);
-          merge(k + i);
+        for (int u = 0; u < U; ++u) {
+          pt[u] = rpos[c[u]];
+          f[u] = rval[c[u]];
+        }
+        float dx[U], dy[U], dz[U], y[U];
+        int it[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) it[u] = disp(pt[u], dx[u], dy[u], dz[u], y[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          float A, Bx;
+          fp32c_eval(s_tab, a.ct.n, it[u], y[u], A, Bx);
+          fp32c_acc(A, Bx, dx[u], dy[u], dz[u], f[u].x, f[u].y, f[u].z, ps[0], ps[1], ps[2]);
+          merge(k + u);
         }
       }
 #pragma unroll
-      for (int i = 0; i < U - 1; ++i) {
-        if (k + i < cnt) {  // entries en[0 .. cnt - k)
-          pair(en[i], et[i]);
-          merge(k + i);
-        }
-      }
+      for (int u = 0; u < U - 1; ++u)
+        if (k + u < cnt) one(en[u], k + u);  // entries en[0 .. cnt - k)
     }
     double acc[1][3];
 #pragma unroll
@@ -1104,7 +1083,6 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.slot = ctx->box_slot;
   b.pos = ctx->box_pos;
   b.idx = ctx->box_idx;
-  b.ab = ctx->box_ab;
   return b;
 }
 
@@ -1115,7 +1093,7 @@ hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double*
                       const ChebEpi& epi, const double* u_glob, double* out) {
   if (!ctx->box_valid || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs) return hipErrorInvalidConfiguration;
   a.xcd = 1;
-  const size_t lds = size_t(ctx->box_max) * 2 * sizeof(float4);
+  const size_t lds = sizeof(float4) * (4 * size_t(ctx->tab.n) + size_t(ctx->box_max) * 2);
   hipLaunchKernelGGL((k_apply_box<EPI>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
                      box_args(ctx), xg, soa, v, epi, u_glob, out);
   return hipGetLastError();
@@ -1224,8 +1202,7 @@ hipError_t launch_box_fill(ggem_ctx* ctx, int32_t max_records, int32_t max_targe
   b.waves_cap = (max_targets + 63) / 64;
   b.recs_cap = max_records;
   if (b.waves_cap > kBoxMaxWaves || max_records > kBoxMaxRecs) return hipErrorInvalidValue;
-  const size_t lds = sizeof(float4) * (4 * size_t(ctx->tab.n) + size_t(b.recs_cap)) +
-                     sizeof(int32_t) * (2 * size_t(b.cells_cap) + b.waves_cap + 2);
+  const size_t lds = sizeof(float4) * size_t(b.recs_cap) + sizeof(int32_t) * (2 * size_t(b.cells_cap) + b.waves_cap + 2);
   if (ctx->nsys > 1)
     hipLaunchKernelGGL(k_box_fill<true>, dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, make_args(ctx),
                        b);
