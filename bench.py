@@ -473,7 +473,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_apply_box FP32C Chebyshev application (pair sum + recurrence update)",
                      "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM,
-                     "traffic": profiled_traffic("k_apply_box<2>") if args.workload == "beads1m" else None,
+                     "traffic": profiled_traffic("k_apply_box<2, true>") if args.workload == "beads1m" else None,
                      "algorithmic": "%d B per bead x %d beads per launch (48 B pair sum in FP32C + 144 B "
                                     "recurrence minimum)" % (B_BEAD_APPLY, n),
                      "valu_tflops_algorithmic": F_PAIR * pairs / apply_s / 1e12,

@@ -93,7 +93,22 @@ __device__ __forceinline__ int div_small(int q, float inv_d) { return int((float
 
 constexpr int kBrickCells = 1 << (3 * kBrickBits);
 constexpr int kBoxMaxCells = 8192;  // box cells whose tables a brick kernel keeps in LDS
-constexpr int kBoxBits = 6;         // box coordinates packed 6 bits per axis
+constexpr int kBoxBits = 6;         // box coordinates: below 2^6 per axis
+constexpr int kBoxField = 8;        // ... packed one byte per axis (x, y, z from the low byte)
+
+// Box-cell differences t - o per axis of two packed box cells, given kb =
+// 0x808080 - o: each byte of t + kb holds its axis' difference + 128 (the
+// coordinates are below 64, so no byte carries into the next), and the
+// byte-to-float conversions are exact.
+__device__ __forceinline__ void box_diff(uint32_t t, uint32_t kb, float& dx, float& dy, float& dz) {
+  const uint32_t d = t + kb;
+  dx = float(d & 0xffu) - 128.f;
+  dy = float((d >> 8) & 0xffu) - 128.f;
+  dz = float((d >> 16) & 0xffu) - 128.f;
+}
+__device__ __forceinline__ uint32_t box_pack(int x, int y, int z) {
+  return uint32_t(x) | (uint32_t(y) << kBoxField) | (uint32_t(z) << (2 * kBoxField));
+}
 
 // Box of one brick: the brick widened by the stencil reach on every side,
 // origin (ox, oy, oz) and extent (ex, ey, ez) in cells of the brick's grid,

@@ -84,13 +84,23 @@ __device__ __forceinline__ int fp32c_interval(const CoefTab& t, float s, float& 
   return (r2 ? t.n1 : 0) + j;
 }
 __device__ __forceinline__ bool fp32c_kept(const CoefTab& t, int i) { return i < t.n - 1; }
+// fp32c_interval for a table of one width (n2 = 0, so s1 lies at or past
+// the cut): the same arithmetic below s1; s >= s1 gives the zero interval
+__device__ __forceinline__ int fp32c_interval1(const CoefTab& t, float s, float& y) {
+  const int j = min(int(s * t.inv1), t.n1);
+  const float mid = __builtin_fmaf(float(j) + 0.5f, t.d1, 0.f);
+  y = (s - mid) * (2.f * t.inv1);
+  return j;
+}
 
-// (A, Bx) at y in interval i of the table's LDS copy, which holds the
-// intervals' coefficient quadruples component by component (load_tab):
-// lanes at different intervals read neighbouring words
-__device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int n, int i, float y, float& A,
-                                           float& Bx) {
-  const float4 a0 = tab[i], a1 = tab[n + i], b0 = tab[2 * n + i], b1 = tab[3 * n + i];
+// (A, Bx) at y in interval i of the table's LDS copy (load_tab: the four
+// coefficient quadruples of an interval padded to kTabStride, so that lanes
+// at different intervals spread over the LDS banks and the four reads
+// share one address)
+constexpr int kTabStride = 5;
+__device__ __forceinline__ void fp32c_eval(const float4* __restrict__ tab, int i, float y, float& A, float& Bx) {
+  const float4* t = tab + __mul24(i, kTabStride);  // (a full-rate multiply)
+  const float4 a0 = t[0], a1 = t[1], b0 = t[2], b1 = t[3];
   float pa = __builtin_fmaf(a1.w, y, a1.z);
   pa = __builtin_fmaf(pa, y, a1.y);
   pa = __builtin_fmaf(pa, y, a1.x);
@@ -123,10 +133,10 @@ __device__ __forceinline__ void fp32c_acc(float A, float Bx, float dx, float dy,
   p2 += __builtin_fmaf(A, fz, b * dz);
 }
 
-// the interval table into LDS, component by component: quadruple c of
-// interval i (a.tab[4 i + c]) to dst[c n + i] (the whole block takes part)
+// the interval table into LDS: quadruple c of interval i (a.tab[4 i + c])
+// to dst[kTabStride i + c] (the whole block takes part)
 __device__ __forceinline__ void load_tab(float4* dst, const PairArgs& a) {
-  for (int q = threadIdx.x; q < 4 * a.ct.n; q += blockDim.x) dst[(q & 3) * a.ct.n + (q >> 2)] = a.tab[q];
+  for (int q = threadIdx.x; q < 4 * a.ct.n; q += blockDim.x) dst[kTabStride * (q >> 2) + (q & 3)] = a.tab[q];
 }
 
 // Launch shapes the autotuner chooses from (index 0 is the default); every
@@ -283,7 +293,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   const float4* tab = nullptr;
   if constexpr (P != GGEM_FP64) {
-    __shared__ float4 s_tab[4 * kTabMaxIntervals];
+    __shared__ float4 s_tab[kTabStride * kTabMaxIntervals];
     load_tab(s_tab, a);
     __syncthreads();
     tab = s_tab;
@@ -420,7 +430,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
       const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
       if (!fp32c_kept(a.ct, it)) return;
       float A, Bx;
-      fp32c_eval(tab, a.ct.n, it, y, A, Bx);
+      fp32c_eval(tab, it, y, A, Bx);
 #pragma unroll
       for (int r = 0; r < NR; ++r) fp32c_acc(A, Bx, dx, dy, dz, f[r].x, f[r].y, f[r].z, ps[r][0], ps[r][1], ps[r][2]);
       ++nk;
@@ -685,7 +695,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
         if (i >= nrec) break;
         int x, y, z;
         B.xyz(lo[u], x, y, z);
-        const uint32_t packed = uint32_t(x) | (uint32_t(y) << kBoxBits) | (uint32_t(z) << (2 * kBoxBits));
+        const uint32_t packed = box_pack(x, y, z);
         const float4 rec = make_float4(p[u].x, p[u].y, p[u].z, __uint_as_float(packed));
         b.slot[r0 + i] = uint32_t(t[u]);
         b.pos[r0 + i] = rec;
@@ -734,6 +744,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     const int ko = int(co >> (kCellBitsX + kCellBitsY));
     // staged bricks: the target's own record, packed above the entry offset
     const int bx = io - B.ox, by = jo - B.oy, bz = ko - B.oz;
+    const uint32_t kb = 0x808080u - box_pack(bx, by, bz);
     int self = 0;
     if (nrec > 0) {
       const int qs = (bz * B.ey + by) * B.ex + bx;
@@ -749,13 +760,11 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
           for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
             if (r == self) continue;
             const float4 pt = rpos[r];
-            const uint32_t bt = __float_as_uint(pt.w);
-            const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
-            const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
-            const int dk = int(bt >> (2 * kBoxBits)) - bz;
-            const float dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
-            const float dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
-            const float dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
+            float di, dj, dk;
+            box_diff(__float_as_uint(pt.w), kb, di, dj, dk);
+            const float dx = __builtin_fmaf(di, hx, pt.x - po.x);
+            const float dy = __builtin_fmaf(dj, hy, pt.y - po.y);
+            const float dz = __builtin_fmaf(dk, hz, pt.z - po.z);
             float y;
             if (!fp32c_kept(a.ct, fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y))) continue;
             b.idx[o] = uint16_t(r);
@@ -805,7 +814,7 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
     const int it = fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
     if (!fp32c_kept(a.ct, it)) continue;  // the zero interval
     float A, Bx;
-    fp32c_eval(tab, a.ct.n, it, y, A, Bx);
+    fp32c_eval(tab, it, y, A, Bx);
     fp32c_acc(A, Bx, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
     if ((++nk & (kMergeEvery - 1)) == 0) {
 #pragma unroll
@@ -822,7 +831,7 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
   }
 }
 
-template <int EPI>
+template <int EPI, bool ONE>  // ONE: the table has one interval width
 __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b, const float4* __restrict__ xg,
                                                           const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
                                                           const double* __restrict__ u_glob,
@@ -835,8 +844,8 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
   const int64_t r0 = b.off[brick];
   const int nrec = int(b.off[brick + 1] - r0);
   const int nt = se - sb;
-  float4* s_tab = lds;                // coefficient table
-  float4* rpos = lds + 4 * a.ct.n;    // [nrec] offset xyz, packed box cell
+  float4* s_tab = lds;                         // coefficient table
+  float4* rpos = lds + kTabStride * a.ct.n;    // [nrec] offset xyz, packed box cell
   float4* rval = rpos + nrec;         // [nrec] operand
   load_tab(s_tab, a);
   if (nrec == 0) {
@@ -882,24 +891,19 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
     const int64_t bpk = b.base[sb + i];
     const int64_t base = bpk & kBaseMask;
     const float4 po = rpos[int((uint64_t(bpk) >> kSelfShift) & kField12)];  // the target's own record
-    const uint32_t co = __float_as_uint(po.w);
-    const int bx = int(co & ((1u << kBoxBits) - 1));
-    const int by = int((co >> kBoxBits) & ((1u << kBoxBits) - 1));
-    const int bz = int(co >> (2 * kBoxBits));
+    const uint32_t kb = 0x808080u - __float_as_uint(po.w);
     const int cnt = int((uint64_t(bpk) >> kKeptShift) & kField12);  // stored (kept) entries
     float ps[3] = {0.f, 0.f, 0.f};
     KahanF kh[3];
     // displacement to record rec and its interval in the table (every
     // stored entry is a kept pair)
     auto disp = [&](const float4& pt, float& dx, float& dy, float& dz, float& y) {
-      const uint32_t bt = __float_as_uint(pt.w);
-      const int di = int(bt & ((1u << kBoxBits) - 1)) - bx;
-      const int dj = int((bt >> kBoxBits) & ((1u << kBoxBits) - 1)) - by;
-      const int dk = int(bt >> (2 * kBoxBits)) - bz;
-      dx = __builtin_fmaf(float(di), hx, pt.x - po.x);
-      dy = __builtin_fmaf(float(dj), hy, pt.y - po.y);
-      dz = __builtin_fmaf(float(dk), hz, pt.z - po.z);
-      return fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
+      float di, dj, dk;
+      box_diff(__float_as_uint(pt.w), kb, di, dj, dk);
+      dx = __builtin_fmaf(di, hx, pt.x - po.x);
+      dy = __builtin_fmaf(dj, hy, pt.y - po.y);
+      dz = __builtin_fmaf(dk, hz, pt.z - po.z);
+      return ONE ? fp32c_interval1(a.ct, fp32c_r2(dx, dy, dz), y) : fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
     };
     auto merge = [&](int k) {
       if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
@@ -915,7 +919,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
       const float4 f = rval[rec];
       float dx, dy, dz, y, A, Bx;
       const int it = disp(pt, dx, dy, dz, y);
-      fp32c_eval(s_tab, a.ct.n, it, y, A, Bx);
+      fp32c_eval(s_tab, it, y, A, Bx);
       fp32c_acc(A, Bx, dx, dy, dz, f.x, f.y, f.z, ps[0], ps[1], ps[2]);
       merge(k);
     };
@@ -948,7 +952,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           float A, Bx;
-          fp32c_eval(s_tab, a.ct.n, it[u], y[u], A, Bx);
+          fp32c_eval(s_tab, it[u], y[u], A, Bx);
           fp32c_acc(A, Bx, dx[u], dy[u], dz[u], f[u].x, f[u].y, f[u].z, ps[0], ps[1], ps[2]);
           merge(k + u);
         }
@@ -1093,9 +1097,13 @@ hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double*
                       const ChebEpi& epi, const double* u_glob, double* out) {
   if (!ctx->box_valid || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs) return hipErrorInvalidConfiguration;
   a.xcd = 1;
-  const size_t lds = sizeof(float4) * (4 * size_t(ctx->tab.n) + size_t(ctx->box_max) * 2);
-  hipLaunchKernelGGL((k_apply_box<EPI>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
-                     box_args(ctx), xg, soa, v, epi, u_glob, out);
+  const size_t lds = sizeof(float4) * (kTabStride * size_t(ctx->tab.n) + size_t(ctx->box_max) * 2);
+  if (ctx->tab.n2 == 0)
+    hipLaunchKernelGGL((k_apply_box<EPI, true>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
+                       box_args(ctx), xg, soa, v, epi, u_glob, out);
+  else
+    hipLaunchKernelGGL((k_apply_box<EPI, false>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
+                       box_args(ctx), xg, soa, v, epi, u_glob, out);
   return hipGetLastError();
 }
 
