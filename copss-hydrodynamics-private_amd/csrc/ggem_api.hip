@@ -134,6 +134,9 @@ int prepare_box(ggem_ctx* ctx) {
   ctx->box_valid = false;
   ctx->box_unstaged = false;
   ctx->box_max = 0;
+  // FP64 stores every pair's record number at its list position (the
+  // tensors stay in list order); FP32C the kept pairs' in the brick layout
+  ctx->box_fp64 = ctx->p.precision == GGEM_FP64;
   if (!box_supported(ctx)) return GGEM_OK;
   const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
   if (ctx->box_brick_cap < nb + 1) {
@@ -167,7 +170,7 @@ int prepare_box(ggem_ctx* ctx) {
   if (total[0] <= 0 || mx[0] <= 0 || mx[1] > kBoxMaxTargets) return GGEM_OK;
   if ((rc = ensure(ctx, &ctx->box_slot, &ctx->box_rec_cap, size_t(total[0])))) return rc;
   if ((rc = ensure(ctx, &ctx->box_pos, &ctx->box_pos_cap, size_t(total[0])))) return rc;
-  const size_t ne = size_t(std::max<int64_t>(total[1], 1));
+  const size_t ne = size_t(std::max<int64_t>(ctx->box_fp64 ? ctx->ell_len : total[1], 1));
   if ((rc = ensure(ctx, &ctx->box_idx, &ctx->box_idx_cap, ne))) return rc;
   // bricks that are not staged fill their tensors from the lists
   ctx->box_unstaged = mx[2] > 0;
@@ -201,13 +204,19 @@ int ensure_tensors(ggem_ctx* ctx, int nrhs) {
   int rc;
   if (ctx->p.precision == GGEM_FP64) {
     if ((rc = ensure_ell(ctx))) return rc;
-    if (ctx->coef64_valid) return GGEM_OK;
-    size_t cap = size_t(ctx->coef64_cap);
-    if ((rc = ensure(ctx, &ctx->coef64, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1))))) return rc;
-    ctx->coef64_cap = int64_t(cap);
-    hipError_t e = launch_pair_tensors(ctx, GGEM_FP64);
-    if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
-    ctx->coef64_valid = true;
+    if (!ctx->coef64_valid) {
+      size_t cap = size_t(ctx->coef64_cap);
+      if ((rc = ensure(ctx, &ctx->coef64, &cap, size_t(std::max<int64_t>(ctx->ell_len, 1))))) return rc;
+      ctx->coef64_cap = int64_t(cap);
+      hipError_t e = launch_pair_tensors(ctx, GGEM_FP64);
+      if (e != hipSuccess) return hip_fail(ctx, e, "pair tensors");
+      ctx->coef64_valid = true;
+    }
+    // the staged-record kernel serves one right-hand side
+    if (!ctx->box_built) {
+      if ((rc = prepare_box(ctx))) return rc;
+      ctx->box_built = true;
+    }
     return GGEM_OK;
   }
   // FP32C keeps no tensors (the pair sums evaluate the coefficient table);
@@ -685,6 +694,7 @@ int ggem_set_precision(ggem_ctx* ctx, int32_t precision) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");
   if (precision != GGEM_FP64 && precision != GGEM_FP32C)
     return fail(ctx, GGEM_ERR_INVALID_ARG, "precision must be GGEM_FP64 or GGEM_FP32C");
+  if (precision != ctx->p.precision) ctx->box_built = false;  // the staged records' layout is per precision
   ctx->p.precision = precision;
   return GGEM_OK;
 }

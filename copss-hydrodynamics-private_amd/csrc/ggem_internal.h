@@ -224,6 +224,7 @@ struct ggem_ctx {
   int32_t* box_max_d = nullptr;   // [3] largest record count of a staged brick, most targets of a brick, unstaged bricks with targets
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
+  bool box_fp64 = false;          // box_idx holds every pair in the lists' slice layout (FP64)
   bool box_valid = false;         // ... and some brick is staged
   bool box_unstaged = false;      // ... and some brick with targets is not
   int bsign = 1;  // sign of Bx: +1 when xi > alpha

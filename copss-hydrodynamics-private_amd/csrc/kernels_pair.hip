@@ -133,6 +133,16 @@ __device__ __forceinline__ void fp32c_acc(float A, float Bx, float dx, float dy,
   p2 += __builtin_fmaf(A, fz, b * dz);
 }
 
+// G f = A f + sgn (w . f) w with the stored FP64 (A, w) = c, every product
+// fused explicitly, so that all FP64 pair sums add bit-identical terms
+__device__ __forceinline__ void fp64_acc(const double4& c, double sg, const double4& f, double& p0, double& p1,
+                                         double& p2) {
+  const double wf = sg * __builtin_fma(c.w, f.z, __builtin_fma(c.z, f.y, c.y * f.x));
+  p0 += __builtin_fma(c.x, f.x, wf * c.y);
+  p1 += __builtin_fma(c.x, f.y, wf * c.z);
+  p2 += __builtin_fma(c.x, f.z, wf * c.w);
+}
+
 // the interval table into LDS: quadruple c of interval i (a.tab[4 i + c])
 // to dst[kTabStride i + c] (the whole block takes part)
 __device__ __forceinline__ void load_tab(float4* dst, const PairArgs& a) {
@@ -324,12 +334,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
     // G f = A f + sgn (w . f) w with the stored (A, w)
     auto pair_acc = [&](const double4& c, const double4* f) {
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const double wf = sg * (c.y * f[r].x + c.z * f[r].y + c.w * f[r].z);
-        ps[r][0] += c.x * f[r].x + wf * c.y;
-        ps[r][1] += c.x * f[r].y + wf * c.z;
-        ps[r][2] += c.x * f[r].z + wf * c.w;
-      }
+      for (int r = 0; r < NR; ++r) fp64_acc(c, sg, f[r], ps[r][0], ps[r][1], ps[r][2]);
     };
     int k = 0;
     for (; k + U <= cnt; k += U) {
@@ -523,7 +528,10 @@ struct BoxArgs {
                          // its own record in a staged brick (above)
   uint32_t* slot;        // [records] slot of each record
   float4* pos;           // [records] offset xyz, packed box cell
-  uint16_t* idx;         // [entries] neighbour's record number (staged bricks), brick layout
+  uint16_t* idx;         // [entries] neighbour's record number (staged bricks): FP32C brick layout,
+                         // FP64 the lists' slice layout
+  int32_t fp64;          // FP64: every pair stored, at its list position
+  const int64_t* slice_off;
   int32_t cells_cap;     // k_box_fill: box cells, waves and records of a brick its LDS tables hold
   int32_t waves_cap;
   int32_t recs_cap;
@@ -751,6 +759,20 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
       self = cst[qs] + int(s - cfirst[qs]);
     }
     const int cnt = a.cnt[s];
+    if (nrec > 0 && b.fp64) {
+      // FP64: every pair, at its position in the lists' slice layout
+      int64_t o = b.slice_off[s >> 6] + (s & 63);
+      if (cnt > 0)
+        box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
+          for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
+            if (r == self) continue;
+            b.idx[o] = uint16_t(r);
+            o += kSlice;
+          }
+        });
+      b.base[sb + i] = bb;
+      continue;
+    }
     if (nrec > 0) {
       // only the kept pairs (not in the table's zero interval) are stored
       int64_t o = bb;
@@ -972,6 +994,84 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
 }
 
 
+// FP64 pair sum with the operand records staged in LDS: one workgroup per
+// brick, its targets in slot order; per entry the stored tensor (list
+// layout) and the neighbour's record number in the brick's box (the same
+// layout), the operand read from LDS.  List order and fp64_acc as in
+// k_apply, so the sums are bit-identical.  A brick that is not staged
+// gathers the operands from global memory.
+template <int EPI>
+__global__ void __launch_bounds__(kBoxThreads) k_apply_box64(PairArgs a, BoxArgs b, const double4* __restrict__ c64,
+                                                            const double4* __restrict__ xg,
+                                                            const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
+                                                            const double* __restrict__ u_glob,
+                                                            double* __restrict__ out) {
+  extern __shared__ double4 lds64[];
+  const int64_t brick = block_index(a.xcd);
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  if (sb >= se) return;  // no targets (uniform over the block)
+  const int64_t r0 = b.off[brick];
+  const int nrec = int(b.off[brick + 1] - r0);
+  const int nt = se - sb;
+  double4* rval = lds64;  // [nrec] operand
+  if (nrec > 0) {
+    // four records in flight per thread, then the remainder one at a time
+    int i = int(threadIdx.x);
+    for (; i + 3 * kBoxThreads < nrec; i += 4 * kBoxThreads) {
+      const uint32_t t0 = b.slot[r0 + i], t1 = b.slot[r0 + i + kBoxThreads];
+      const uint32_t t2 = b.slot[r0 + i + 2 * kBoxThreads], t3 = b.slot[r0 + i + 3 * kBoxThreads];
+      const double4 f0 = xg[t0], f1 = xg[t1], f2 = xg[t2], f3 = xg[t3];
+      rval[i] = f0;
+      rval[i + kBoxThreads] = f1;
+      rval[i + 2 * kBoxThreads] = f2;
+      rval[i + 3 * kBoxThreads] = f3;
+    }
+    for (; i < nrec; i += kBoxThreads) rval[i] = xg[b.slot[r0 + i]];
+    __syncthreads();
+  }
+  const double sg = a.bsign;
+  constexpr int U = 4;  // entries per group, the next group's record numbers loaded one ahead
+  for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+    const int64_t s = sb + i;
+    const int cnt = a.cnt[s];
+    const int64_t base = a.slice_off[s >> 6] + (s & 63);
+    const double4* __restrict__ cf = c64 + base;
+    double ps[3] = {0.0, 0.0, 0.0};
+    if (cnt > 0 && nrec > 0) {
+      const uint16_t* __restrict__ ci = b.idx + base;
+      const int last = cnt - 1;
+      uint32_t en[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) en[u] = ci[int64_t(min(u, last)) * kSlice];
+      int k = 0;
+      for (; k + U <= cnt; k += U) {
+        uint32_t e[U];
+        double4 c[U], f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          e[u] = en[u];
+          en[u] = ci[int64_t(min(k + U + u, last)) * kSlice];
+          c[u] = cf[int64_t(k + u) * kSlice];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[u] = rval[e[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) fp64_acc(c[u], sg, f[u], ps[0], ps[1], ps[2]);
+      }
+#pragma unroll
+      for (int u = 0; u < U - 1; ++u)
+        if (k + u < cnt) fp64_acc(cf[int64_t(k + u) * kSlice], sg, rval[en[u]], ps[0], ps[1], ps[2]);
+    } else if (cnt > 0) {
+      const uint32_t* __restrict__ lst = a.ell + base;
+      for (int k = 0; k < cnt; ++k)
+        fp64_acc(cf[int64_t(k) * kSlice], sg, xg[lst[int64_t(k) * kSlice] & kSlotMask], ps[0], ps[1], ps[2]);
+    }
+    const double acc[1][3] = {{ps[0], ps[1], ps[2]}};
+    apply_epilogue<1, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
+  }
+}
+
 // Field points: sum over the element's particle list, own element
 // included, no exclusion (ggem_stokes.C:396-420), full minimum image.
 template <int P>
@@ -1087,6 +1187,8 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.slot = ctx->box_slot;
   b.pos = ctx->box_pos;
   b.idx = ctx->box_idx;
+  b.fp64 = ctx->box_fp64 ? 1 : 0;
+  b.slice_off = ctx->slice_off;
   return b;
 }
 
@@ -1095,7 +1197,8 @@ unsigned n_bricks(const ggem_ctx* ctx) { return unsigned(ctx->grid.n_cells >> (3
 template <int EPI>
 hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double* soa, const SlotVec& v,
                       const ChebEpi& epi, const double* u_glob, double* out) {
-  if (!ctx->box_valid || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs) return hipErrorInvalidConfiguration;
+  if (!ctx->box_valid || ctx->box_fp64 || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs)
+    return hipErrorInvalidConfiguration;
   a.xcd = 1;
   const size_t lds = sizeof(float4) * (kTabStride * size_t(ctx->tab.n) + size_t(ctx->box_max) * 2);
   if (ctx->tab.n2 == 0)
@@ -1107,11 +1210,24 @@ hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double*
   return hipGetLastError();
 }
 
-// candidate index of the staged-record kernel (FP32C, one operand, after
-// the block shapes of k_apply); it is the default where some brick is staged
+template <int EPI>
+hipError_t launch_box64(ggem_ctx* ctx, PairArgs a, const double4* xg, const double* soa, const SlotVec& v,
+                        const ChebEpi& epi, const double* u_glob, double* out) {
+  if (!ctx->box_valid || !ctx->box_fp64 || ctx->box_max <= 0 || ctx->box_max > kBoxMaxRecs || !ctx->coef64_valid)
+    return hipErrorInvalidConfiguration;
+  a.xcd = 1;
+  const size_t lds = size_t(ctx->box_max) * sizeof(double4);
+  hipLaunchKernelGGL((k_apply_box64<EPI>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
+                     box_args(ctx), ctx->coef64, xg, soa, v, epi, u_glob, out);
+  return hipGetLastError();
+}
+
+// candidate index of the staged-record kernels (one operand, after the
+// block shapes of k_apply); it is the default where some brick is staged
 constexpr int kBoxShape = kNApply;
 bool box_ready(const ggem_ctx* ctx, int precision, int nr) {
-  return precision == GGEM_FP32C && nr == 1 && ctx->box_valid && ctx->box_max > 0;
+  if (nr != 1 || !ctx->box_valid || ctx->box_max <= 0) return false;
+  return precision == GGEM_FP32C ? !ctx->box_fp64 : (ctx->box_fp64 && ctx->coef64_valid);
 }
 int apply_default(const ggem_ctx* ctx, int precision, int nr) { return box_ready(ctx, precision, nr) ? kBoxShape : 0; }
 int apply_candidates(const ggem_ctx* ctx, int precision, int nr) {
@@ -1124,6 +1240,7 @@ hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const 
                     const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
   if (c == kBoxShape) {
     if constexpr (P == GGEM_FP32C && NR == 1) return launch_box<EPI>(ctx, a, g32, soa, v, epi, u_glob, out);
+    if constexpr (P == GGEM_FP64 && NR == 1) return launch_box64<EPI>(ctx, a, g64, soa, v, epi, u_glob, out);
     return hipErrorInvalidConfiguration;
   }
   if (c < 0 || c >= kNApply) return hipErrorInvalidValue;
