@@ -1031,7 +1031,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box64(PairArgs a, BoxArgs
     __syncthreads();
   }
   const double sg = a.bsign;
-  constexpr int U = 4;  // entries per group, the next group's record numbers loaded one ahead
+  constexpr int U = 2;  // entries per group, the next group's record numbers loaded one ahead (2 measured best of 2, 4, 8)
   for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
     const int64_t s = sb + i;
     const int cnt = a.cnt[s];
