@@ -71,11 +71,13 @@ def test_autotune_results_identical_and_picks_recorded():
 
 
 BOX = 5  # candidate index of the staged-record kernel (after the five block shapes)
+PREC = {G.FP32C: "fp32c", G.FP64: "fp64"}
 
 
-def pin(ctx, n, cand):
+def pin(ctx, n, cand, prec=G.FP32C):
     lg = int(math.floor(math.log2(n)))
-    for key in ("apply/fp32c/1/%d" % lg, "pair_beads/fp32c/0/%d" % lg, "pair_beads/fp32c/1/%d" % lg):
+    p = PREC[prec]
+    for key in ("apply/%s/1/%d" % (p, lg), "pair_beads/%s/0/%d" % (p, lg), "pair_beads/%s/1/%d" % (p, lg)):
         ctx.tune_set(key, cand)
     ctx.set_autotune(True)
 
@@ -87,20 +89,21 @@ def fp32c_results(ctx, f, dw):
     return u, U, y, d
 
 
-def test_box_kernel_matches_global_gather_kernel():
-    """The staged-record kernel (the FP32C default for one right-hand side)
-    gives the global-gather kernel's results bit for bit, pinned either way,
-    on a plain and on a batched context."""
+@pytest.mark.parametrize("prec", [G.FP32C, G.FP64], ids=["fp32c", "fp64"])
+def test_box_kernel_matches_global_gather_kernel(prec):
+    """The staged-record kernel (the default for one right-hand side) gives
+    the global-gather kernel's results bit for bit, pinned either way, on a
+    plain and on a batched context."""
     cfg, x, f = W.make_inputs("beads20k")
-    cfg["precision"] = G.FP32C
+    cfg["precision"] = prec
     dw = G.normal(3 * len(x), 23).reshape(1, -1, 3)
     ctx = G.Context(cfg)
     try:
         ctx.reinit(x, f)
         r0 = fp32c_results(ctx, f, dw)
-        pin(ctx, len(x), 0)
+        pin(ctx, len(x), 0, prec)
         r1 = fp32c_results(ctx, f, dw)
-        pin(ctx, len(x), BOX)
+        pin(ctx, len(x), BOX, prec)
         r2 = fp32c_results(ctx, f, dw)
         for a, b, c in zip(r0, r1, r2):
             assert np.array_equal(a, b) and np.array_equal(a, c)
@@ -108,26 +111,27 @@ def test_box_kernel_matches_global_gather_kernel():
         ctx.close()
     cfgs, xs, fs, dwb = W.make_batch(nsys=4, nrhs=1, seed=12, n_lo=400, n_hi=2000)
     for c in cfgs:
-        c["precision"] = G.FP32C
+        c["precision"] = prec
     b = G.BatchContext(cfgs)
     try:
         b.set_particles(xs, fs)
         b.build_neighbors()
         u0 = b.local_velocity_beads()
-        pin(b, b.n, 0)
+        pin(b, b.n, 0, prec)
         u1 = b.local_velocity_beads()
         assert np.array_equal(u0, u1)
     finally:
         b.close()
 
 
-def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks():
+@pytest.mark.parametrize("prec", [G.FP32C, G.FP64], ids=["fp32c", "fp64"])
+def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks(prec):
     """A dense cluster puts more records in some bricks' boxes than LDS
     holds: those bricks gather from global memory, the rest read staged
     records, and the results equal the global-gather kernel's bit for bit
-    and the oracle's to the FP32C bound."""
+    and the oracle's to the precision's bound."""
     cfg, x, f = W.make_inputs("beads20k")
-    cfg["precision"] = G.FP32C
+    cfg["precision"] = prec
     lo = np.array(cfg["box_min"], float)
     hi = np.array(cfg["box_max"], float)
     rng = np.random.default_rng(5)
@@ -140,7 +144,7 @@ def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks():
     try:
         ctx.reinit(x, f)
         r0 = fp32c_results(ctx, f, dw)
-        pin(ctx, len(x), 0)
+        pin(ctx, len(x), 0, prec)
         r1 = fp32c_results(ctx, f, dw)
         for a, b in zip(r0, r1):
             assert np.array_equal(a, b)
@@ -150,7 +154,7 @@ def test_box_kernel_dense_cluster_mixes_staged_and_gathered_bricks():
     m = O.Mesh(cfg64)
     m.reinit(x, f)
     u_ref = m.local_velocity_beads()
-    assert rel_l2(r0[0], u_ref) < 1e-6
+    assert rel_l2(r0[0], u_ref) < (1e-6 if prec == G.FP32C else 1e-12)
 
 
 def test_lists_filled_on_demand_between_one_and_several_operands():
