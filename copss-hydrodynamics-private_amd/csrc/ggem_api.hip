@@ -160,11 +160,18 @@ int prepare_box(ggem_ctx* ctx) {
       (e = launch_scan64(ctx, ctx->box_ecnt, int64_t(nb), ctx->box_eoff)) != hipSuccess)
     return hip_fail(ctx, e, "box scan");
   int64_t total[2] = {0, 0};
-  int32_t mx[3] = {0, 0, 0};
-  GGEM_HIP(ctx, hipMemcpyAsync(&total[0], ctx->box_off + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&total[1], ctx->box_eoff + nb, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(mx, ctx->box_max_d, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  int64_t mx[3] = {0, 0, 0};
+  {
+    int64_t v[5];
+    if ((rc = peek(ctx, {{ctx->box_off + nb, 8}, {ctx->box_eoff + nb, 8}, {ctx->box_max_d, 4}, {ctx->box_max_d + 1, 4},
+                         {ctx->box_max_d + 2, 4}}, v)))
+      return rc;
+    total[0] = v[0];
+    total[1] = v[1];
+    mx[0] = v[2];
+    mx[1] = v[3];
+    mx[2] = v[4];
+  }
   // no brick staged, or a brick too crowded for the entry layout: the
   // list-order kernels serve
   if (total[0] <= 0 || mx[0] <= 0 || mx[1] > kBoxMaxTargets) return GGEM_OK;
@@ -175,8 +182,8 @@ int prepare_box(ggem_ctx* ctx) {
   // bricks that are not staged fill their tensors from the lists
   ctx->box_unstaged = mx[2] > 0;
   if (ctx->box_unstaged && (rc = ensure_ell(ctx))) return rc;
-  if ((e = launch_box_fill(ctx, mx[0], mx[1])) != hipSuccess) return hip_fail(ctx, e, "box_fill");
-  ctx->box_max = mx[0];
+  if ((e = launch_box_fill(ctx, int32_t(mx[0]), int32_t(mx[1]))) != hipSuccess) return hip_fail(ctx, e, "box_fill");
+  ctx->box_max = int32_t(mx[0]);
   ctx->box_valid = true;
   return GGEM_OK;
 }
@@ -189,7 +196,7 @@ int ensure_ell(ggem_ctx* ctx) {
   hipError_t e = launch_scan64(ctx, ctx->slice_len, nsl, ctx->slice_off);
   if (e != hipSuccess) return hip_fail(ctx, e, "scan64");
   int64_t len = 0;
-  int rc = d2h(ctx, &len, ctx->slice_off + nsl, sizeof(int64_t));
+  int rc = peek(ctx, {{ctx->slice_off + nsl, 8}}, &len);
   if (rc) return rc;
   ctx->ell_len = len;
   size_t cap = size_t(ctx->ell_cap);
@@ -443,6 +450,10 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->rhs_el);
   if (ctx->par_h) (void)hipHostFree(ctx->par_h);
   ctx->par_h = nullptr;
+  f(ctx->peek_d);
+  ctx->peek_d = nullptr;
+  if (ctx->peek_h) (void)hipHostFree(ctx->peek_h);
+  ctx->peek_h = nullptr;
   if (ctx->par_ev) (void)hipEventDestroy(ctx->par_ev);
   ctx->par_ev = nullptr;
   if (ctx->cheb_exec) (void)hipGraphExecDestroy(ctx->cheb_exec);
@@ -485,6 +496,28 @@ int grow_scan64(ggem_ctx* ctx, int64_t len) {
 
 }  // namespace
 
+int ggem::peek(ggem_ctx* ctx, std::initializer_list<std::pair<const void*, int32_t>> items, int64_t* out) {
+  if (items.size() > size_t(kPeekMax)) return fail(ctx, GGEM_ERR_INVALID_ARG, "too many values to read back");
+  if (!ctx->peek_d) GGEM_HIP(ctx, hipMalloc(&ctx->peek_d, sizeof(int64_t) * kPeekMax));
+  if (!ctx->peek_h) GGEM_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->peek_h), sizeof(int64_t) * kPeekMax));
+  PeekArgs p{};
+  p.n = int32_t(items.size());
+  p.dst = ctx->peek_d;
+  int i = 0;
+  for (const auto& it : items) {
+    p.src[i] = it.first;
+    p.bytes[i] = it.second;
+    ++i;
+  }
+  hipError_t e = launch_peek(ctx, p);
+  if (e != hipSuccess) return hip_fail(ctx, e, "readback");
+  GGEM_HIP(ctx, hipMemcpyAsync(ctx->peek_h, ctx->peek_d, sizeof(int64_t) * p.n, hipMemcpyDeviceToHost, ctx->stream));
+  const int rc = sync(ctx);
+  if (rc) return rc;
+  for (int k = 0; k < p.n; ++k) out[k] = ctx->peek_h[k];
+  return GGEM_OK;
+}
+
 // Element particle lists for the spreading, built on first use after a
 // neighbour build.
 int ggem::ensure_elist(ggem_ctx* ctx) {
@@ -494,7 +527,7 @@ int ggem::ensure_elist(ggem_ctx* ctx) {
   if (e == hipSuccess) e = launch_scan64(ctx, ctx->elist_off, ne, ctx->elist_off);
   if (e != hipSuccess) return hip_fail(ctx, e, "element lists");
   int64_t total = 0;
-  int rc = d2h(ctx, &total, ctx->elist_off + ne, sizeof(int64_t));
+  int rc = peek(ctx, {{ctx->elist_off + ne, 8}}, &total);
   if (rc) return rc;
   size_t cap = size_t(ctx->elist_cap);
   if ((rc = ensure(ctx, &ctx->elist, &cap, size_t(std::max<int64_t>(total, 1))))) return rc;
@@ -847,16 +880,13 @@ int keep_lists(ggem_ctx* ctx) {
   const bool have = ctx->keep_n == n && ctx->keep_off != nullptr;
   hipError_t e = launch_stale_patch(ctx, have);
   if (e != hipSuccess) return hip_fail(ctx, e, "stale lists");
-  int32_t flag = 0;
-  int64_t pairs = 0;
-  GGEM_HIP(ctx, hipMemcpyAsync(&flag, ctx->flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&pairs, ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  if ((rc = sync(ctx))) return rc;
-  ctx->stale_lists = flag != 0;
-  ctx->pairs = pairs;
+  int64_t v[2] = {0, 0};
+  if ((rc = peek(ctx, {{ctx->flag, 4}, {ctx->pairs_d, 8}}, v))) return rc;
+  ctx->stale_lists = v[0] != 0;
+  ctx->pairs = v[1];
   if ((rc = ensure(ctx, &ctx->keep_cnt, &ctx->keep_cnt_cap, size_t(std::max<int64_t>(n, 1))))) return rc;
   if ((rc = ensure(ctx, &ctx->keep_off2, &ctx->keep_off2_cap, size_t(n + 1)))) return rc;
-  if ((rc = ensure(ctx, &ctx->keep_ids2, &ctx->keep_ids2_cap, size_t(std::max<int64_t>(pairs, 1))))) return rc;
+  if ((rc = ensure(ctx, &ctx->keep_ids2, &ctx->keep_ids2_cap, size_t(std::max<int64_t>(v[1], 1))))) return rc;
   if ((rc = grow_scan64(ctx, n + 1))) return rc;
   if ((e = launch_counts_by_particle(ctx, ctx->keep_cnt)) != hipSuccess ||
       (e = launch_scan64(ctx, ctx->keep_cnt, n, ctx->keep_off2)) != hipSuccess ||
@@ -909,12 +939,10 @@ int ggem_build_neighbors(ggem_ctx* ctx) {
   // one synchronisation: the out-of-domain flag and the pair count (the
   // sliced layout is laid out and filled by ensure_ell when a kernel that
   // reads it is about to run)
-  int32_t flag = 0;
-  int64_t pairs = 0;
-  GGEM_HIP(ctx, hipMemcpyAsync(&flag, ctx->flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  GGEM_HIP(ctx, hipMemcpyAsync(&pairs, ctx->pairs_d, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  int rc = sync(ctx);
+  int64_t v[2] = {0, 0};
+  int rc = peek(ctx, {{ctx->flag, 4}, {ctx->pairs_d, 8}}, v);
   if (rc) return rc;
+  const int64_t flag = v[0], pairs = v[1];
   if (flag) {
     ctx->elem_valid = false;
     // point_mesh.C:602-612: a particle no element contains is fatal

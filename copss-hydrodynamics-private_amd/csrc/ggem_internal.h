@@ -6,9 +6,11 @@
 
 #include <cstdint>
 #include <functional>
+#include <initializer_list>
 #include <map>
 #include <random>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "copss_ggem.h"
@@ -280,6 +282,8 @@ struct ggem_ctx {
   int32_t* sys_slot = nullptr;       // [n] system of each slot
   int64_t sys_cap = 0;
   double* par_h = nullptr;           // pinned staging of per-system recurrence parameters
+  int64_t* peek_d = nullptr;         // [kPeekMax] device values gathered for one readback
+  int64_t* peek_h = nullptr;         // [kPeekMax] their pinned host copy
   size_t par_cap = 0;
   hipEvent_t par_ev = nullptr;
 
@@ -468,6 +472,19 @@ hipError_t launch_apply(ggem_ctx* ctx, int precision, int nrhs, const SlotVec& x
 
 // ---- launchers (kernels_spread.hip) ----
 // Fe[e][j][l] (node_major: Fe[e][l][j])
+// Small device values (int32 or int64 each) read back in one kernel, one
+// copy into pinned host memory and one synchronisation; out gets them as
+// int64.
+constexpr int kPeekMax = 8;
+struct PeekArgs {
+  const void* src[kPeekMax];
+  int32_t bytes[kPeekMax];  // 4 or 8
+  int32_t n;
+  int64_t* dst;
+};
+hipError_t launch_peek(ggem_ctx* ctx, const PeekArgs& p);
+int peek(ggem_ctx* ctx, std::initializer_list<std::pair<const void*, int32_t>> items, int64_t* out);
+
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
                          double* rho, double* Fe, bool node_major = false);
 

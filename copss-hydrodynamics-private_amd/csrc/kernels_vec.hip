@@ -250,4 +250,17 @@ hipError_t launch_fill(ggem_ctx* ctx, int64_t len, double v, double* y) {
   return hipGetLastError();
 }
 
+namespace {
+__global__ void k_peek(PeekArgs p) {
+  const int i = int(threadIdx.x);
+  if (i >= p.n) return;
+  p.dst[i] = p.bytes[i] == 8 ? *static_cast<const int64_t*>(p.src[i]) : int64_t(*static_cast<const int32_t*>(p.src[i]));
+}
+}  // namespace
+
+hipError_t launch_peek(ggem_ctx* ctx, const PeekArgs& p) {
+  hipLaunchKernelGGL(k_peek, dim3(1), dim3(64), 0, ctx->stream, p);
+  return hipGetLastError();
+}
+
 }  // namespace ggem
