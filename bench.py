@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-slit", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replayed measurement")
     ap.add_argument("--slit-steps", type=int, default=100)
     ap.add_argument("--batch-systems", type=int, default=256)
     return ap.parse_args()
@@ -319,10 +320,12 @@ def measure(G, ctx, args, IF, nqp_1d, order, lo, hi, dw, precisions, names):
             t_ms, _ = ctx.profile_get(nm)
             if t_ms > 0:
                 per_step[nm] = t_ms / args.steps
-        ctx.set_graph(True)
-        run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, 1, False)
-        ms_graph = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
-        ctx.set_graph(False)
+        ms_graph = None
+        if not args.no_graph:
+            ctx.set_graph(True)
+            run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, 1, False)
+            ms_graph = run_steps(ctx, G, args, IF, nqp_1d, order, lo, hi, dw, args.steps, False)
+            ctx.set_graph(False)
         r = dict(ms=ms, ms_prof=ms_prof, ms_graph=ms_graph, apply_avg_ms=k_ms / max(k_n, 1),
                  apply_launches=k_n / max(args.steps, 1), spread_avg_ms=s_ms / max(s_n, 1),
                  local_avg_ms=l_ms / max(l_n, 1), per_step=per_step)
@@ -469,7 +472,8 @@ def main():
         "brownian_steps_per_s": 1e3 / r32["ms"],
         "pair_evaluations_per_step": evals,
         "pair_kernel_interactions_per_s": pairs / apply_s,
-        "graph": {"ms_per_step": r32["ms_graph"], "brownian_steps_per_s": 1e3 / r32["ms_graph"]},
+        "graph": ({"ms_per_step": r32["ms_graph"], "brownian_steps_per_s": 1e3 / r32["ms_graph"]}
+                  if r32["ms_graph"] else None),
         "roofline": {"bound": "hbm", "kernel": "k_apply_box FP32C Chebyshev application (pair sum + recurrence update)",
                      "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM,

@@ -530,13 +530,25 @@ struct BoxArgs {
   float4* pos;           // [records] offset xyz, packed box cell
   uint16_t* idx;         // [entries] neighbour's record number (staged bricks): FP32C brick layout,
                          // FP64 the lists' slice layout
-  int32_t fp64;          // FP64: every pair stored, at its list position
-  const int64_t* slice_off;
   int32_t cells_cap;     // k_box_fill: box cells, waves and records of a brick its LDS tables hold
   int32_t waves_cap;
   int32_t recs_cap;
+  int32_t fp64;          // FP64: every pair stored, at its list position
   const StencilRow* rows;  // stencil rows (a batched context: all systems', SysGrid::row_off)
   int32_t nrows;
+};
+
+// What the staged pair sums read of the layout (their kernel arguments
+// stay small)
+struct BoxView {
+  const int32_t* cs;
+  int32_t reach[3];
+  const int64_t* off;
+  const int32_t* tgt;
+  const int64_t* base;
+  const uint32_t* slot;
+  const float4* pos;
+  const uint16_t* idx;
 };
 
 // Entry layout of a brick: its targets in the order tgt, in waves of 64;
@@ -761,7 +773,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_box_fill(PairArgs a, BoxArgs b)
     const int cnt = a.cnt[s];
     if (nrec > 0 && b.fp64) {
       // FP64: every pair, at its position in the lists' slice layout
-      int64_t o = b.slice_off[s >> 6] + (s & 63);
+      int64_t o = a.slice_off[s >> 6] + (s & 63);
       if (cnt > 0)
         box_walk(B, rows, nrows, io, jo, ko, [&](int q0, int q1, uint32_t) {
           for (int r = cst[q0], re = cst[q1 + 1]; r < re; ++r) {
@@ -854,7 +866,7 @@ __device__ __forceinline__ void fp32c_pairs_global(const PairArgs& a, int64_t s,
 }
 
 template <int EPI, bool ONE>  // ONE: the table has one interval width
-__global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b, const float4* __restrict__ xg,
+__global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxView b, const float4* __restrict__ xg,
                                                           const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
                                                           const double* __restrict__ u_glob,
                                                           double* __restrict__ out) {
@@ -1001,7 +1013,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxArgs b
 // k_apply, so the sums are bit-identical.  A brick that is not staged
 // gathers the operands from global memory.
 template <int EPI>
-__global__ void __launch_bounds__(kBoxThreads) k_apply_box64(PairArgs a, BoxArgs b, const double4* __restrict__ c64,
+__global__ void __launch_bounds__(kBoxThreads) k_apply_box64(PairArgs a, BoxView b, const double4* __restrict__ c64,
                                                             const double4* __restrict__ xg,
                                                             const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
                                                             const double* __restrict__ u_glob,
@@ -1188,7 +1200,19 @@ BoxArgs box_args(const ggem_ctx* ctx) {
   b.pos = ctx->box_pos;
   b.idx = ctx->box_idx;
   b.fp64 = ctx->box_fp64 ? 1 : 0;
-  b.slice_off = ctx->slice_off;
+  return b;
+}
+
+BoxView box_view(const ggem_ctx* ctx) {
+  BoxView b;
+  b.cs = ctx->cell_start;
+  for (int d = 0; d < 3; ++d) b.reach[d] = ctx->reach[d];
+  b.off = ctx->box_off;
+  b.tgt = ctx->box_tgt;
+  b.base = ctx->box_base;
+  b.slot = ctx->box_slot;
+  b.pos = ctx->box_pos;
+  b.idx = ctx->box_idx;
   return b;
 }
 
@@ -1203,10 +1227,10 @@ hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double*
   const size_t lds = sizeof(float4) * (kTabStride * size_t(ctx->tab.n) + size_t(ctx->box_max) * 2);
   if (ctx->tab.n2 == 0)
     hipLaunchKernelGGL((k_apply_box<EPI, true>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
-                       box_args(ctx), xg, soa, v, epi, u_glob, out);
+                       box_view(ctx), xg, soa, v, epi, u_glob, out);
   else
     hipLaunchKernelGGL((k_apply_box<EPI, false>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
-                       box_args(ctx), xg, soa, v, epi, u_glob, out);
+                       box_view(ctx), xg, soa, v, epi, u_glob, out);
   return hipGetLastError();
 }
 
@@ -1218,7 +1242,7 @@ hipError_t launch_box64(ggem_ctx* ctx, PairArgs a, const double4* xg, const doub
   a.xcd = 1;
   const size_t lds = size_t(ctx->box_max) * sizeof(double4);
   hipLaunchKernelGGL((k_apply_box64<EPI>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
-                     box_args(ctx), ctx->coef64, xg, soa, v, epi, u_glob, out);
+                     box_view(ctx), ctx->coef64, xg, soa, v, epi, u_glob, out);
   return hipGetLastError();
 }
 
