@@ -410,6 +410,7 @@ void free_all(ggem_ctx* ctx) {
   f(ctx->box_pos);
   f(ctx->box_idx);
   f(ctx->box_max_d);
+  f(ctx->spread_unstaged);
   f(ctx->brick_recs);
   f(ctx->pc32);
   f(ctx->coef64);
@@ -1287,7 +1288,7 @@ int ggem_spread_nodes(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, double* F
   if (ctx->fe_nodes < 0) return fail(ctx, GGEM_ERR_INVALID_ARG, "no FE mesh (call ggem_set_fe_mesh)");
   if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
   if (!IF || !F_node) return fail(ctx, GGEM_ERR_INVALID_ARG, "null IF table or output");
-  if ((rc = ensure_elist(ctx))) return rc;
+  if ((rc = ensure_spread(ctx, nqp_1d))) return rc;
   const int n_u = 20, nqp = nqp_1d * nqp_1d * nqp_1d;
   const int64_t ne = ctx->grid.n_elem;
   Pools& P = pools(ctx);
@@ -1341,7 +1342,7 @@ int ggem_spread(ggem_ctx* ctx, int32_t nqp_1d, int32_t n_u, const double* IF, do
   if ((rc = single_only(ctx, "ggem_spread"))) return rc;
   if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
   if (n_u < 1 || n_u > 27 || !IF) return fail(ctx, GGEM_ERR_INVALID_ARG, "n_u must be in [1, 27] with an IF table");
-  if ((rc = ensure_elist(ctx))) return rc;
+  if ((rc = ensure_spread(ctx, nqp_1d))) return rc;
   const int nqp = nqp_1d * nqp_1d * nqp_1d;
   const int64_t ne = ctx->grid.n_elem;
   Pools& P = pools(ctx);

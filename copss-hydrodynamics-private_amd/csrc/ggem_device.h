@@ -244,16 +244,28 @@ __device__ __forceinline__ void stage_box_cells(const BoxGeom& B, const int32_t*
   __syncthreads();
 }
 
-// The stencil of the element in grid cell (ci, cj, ck) of the brick as runs
+// Element l (0..511, x fastest) of the brick of box B: its grid cell and
+// reference element id; false for the padding cells of the last bricks of
+// an axis, which are no elements.
+__device__ __forceinline__ bool brick_elem(const BoxGeom& B, int l, int& ci, int& cj, int& ck, int64_t& e) {
+  constexpr int M = (1 << kBrickBits) - 1;
+  ci = B.bx0 + (l & M);
+  cj = B.by0 + ((l >> kBrickBits) & M);
+  ck = B.bz0 + (l >> (2 * kBrickBits));
+  e = ci + int64_t(B.n[0]) * (cj + int64_t(B.n[1]) * ck);
+  return ci < B.n[0] && cj < B.n[1] && ck < B.n[2];
+}
+
+// Stencil rows r0 .. r1 - 1 of the element in grid cell (ci, cj, ck) of the brick as runs
 // of box cells: calls run(q0, q1, code) for box cells q0..q1 of one row
 // with the periodic shift code of that run, in the order for_each_segment
 // visits the same cells (rows in stencil order; within a row the part past
 // the low face, then the part past the high face, then the rest, x
 // ascending), so the particles come out in the same order.
 template <class Run>
-__device__ __forceinline__ void box_walk(const BoxGeom& B, const StencilRow* __restrict__ rows, int nrows,
-                                         int ci, int cj, int ck, Run&& run) {
-  for (int r = 0; r < nrows; ++r) {
+__device__ __forceinline__ void box_walk_rows(const BoxGeom& B, const StencilRow* __restrict__ rows, int r0, int r1,
+                                              int ci, int cj, int ck, Run&& run) {
+  for (int r = r0; r < r1; ++r) {
     const StencilRow R = rows[r];
     const int jj = cj + R.dj, kk = ck + R.dk;
     uint32_t cy = 0, cz = 0;
@@ -283,6 +295,13 @@ __device__ __forceinline__ void box_walk(const BoxGeom& B, const StencilRow* __r
     }
     if (lo <= hi) run(rowq + lo, rowq + hi, cyz);
   }
+}
+
+// box_walk_rows over the whole stencil
+template <class Run>
+__device__ __forceinline__ void box_walk(const BoxGeom& B, const StencilRow* __restrict__ rows, int nrows,
+                                         int ci, int cj, int ck, Run&& run) {
+  box_walk_rows(B, rows, 0, nrows, ci, cj, ck, run);
 }
 
 // Walk the particle slots of every element of the stencil of element

@@ -59,6 +59,17 @@ int ensure_tensors(ggem_ctx* ctx, int nrhs = 1);
 int ensure_ell(ggem_ctx* ctx);
 // the element particle lists of the spreading (filled on first use)
 int ensure_elist(ggem_ctx* ctx);
+// what launch_spread reads at nqp_1d Gauss points per axis: the element
+// lists unless the brick kernel serves it (and the autotuner does not time
+// the other shapes)
+inline int ensure_spread(ggem_ctx* ctx, int nqp_1d) {
+  if (spread_by_brick(ctx, ctx->p.precision, nqp_1d)) {
+    const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
+    const int rc = ensure(ctx, &ctx->spread_unstaged, &ctx->spread_unstaged_cap, nb);
+    if (rc || !ctx->tune.on) return rc;
+  }
+  return ensure_elist(ctx);
+}
 
 inline int check_ready(ggem_ctx* ctx) {
   if (!ctx) return fail(nullptr, GGEM_ERR_INVALID_ARG, "null context");

@@ -116,6 +116,7 @@ struct ggem_ctx {
   ggem::Grid grid;
   ggem::GGEMConst gc;
   int device = 0;
+  int32_t lds_block = 0;  // LDS bytes a workgroup may use (queried on first use)
   hipStream_t stream = nullptr;
   std::string err;
   // periodic shifts are taken from the neighbour entries unless the stencil
@@ -224,6 +225,8 @@ struct ggem_ctx {
   int32_t* brick_recs = nullptr;  // [n_bricks] particles in each brick's box (list count pass)
   size_t brick_recs_cap = 0;
   int32_t* box_max_d = nullptr;   // [3] largest record count of a staged brick, most targets of a brick, unstaged bricks with targets
+  uint8_t* spread_unstaged = nullptr;  // [bricks] the brick spreading's boxes over its record table
+  size_t spread_unstaged_cap = 0;
   int32_t box_max = 0;            // same on the host; 0 when no brick is staged
   bool box_built = false;         // box_* describe the current lists
   bool box_fp64 = false;          // box_idx holds every pair in the lists' slice layout (FP64)
@@ -487,6 +490,9 @@ int peek(ggem_ctx* ctx, std::initializer_list<std::pair<const void*, int32_t>> i
 
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
                          double* rho, double* Fe, bool node_major = false);
+// launch_spread runs the brick kernel, which needs no element lists (FP32C
+// with a box that fits the packing and the LDS)
+bool spread_by_brick(ggem_ctx* ctx, int precision, int nqp_1d);
 
 // ---- launchers (kernels_vec.hip) ----
 hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in_interleaved, double* out_slots);

@@ -342,7 +342,7 @@ int rhs_fill(ggem_ctx* ctx, int32_t nqp_1d, const double* IF, const ggem_rhs_opt
     if ((rc = check_ready(ctx))) return rc;
     if (nqp_1d != 2 && nqp_1d != 3) return fail(ctx, GGEM_ERR_INVALID_ARG, "nqp_1d must be 2 or 3");
     if (!IF) return fail(ctx, GGEM_ERR_INVALID_ARG, "null IF table");
-    if ((rc = ensure_elist(ctx))) return rc;
+    if ((rc = ensure_spread(ctx, nqp_1d))) return rc;
   }
   GGEM_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t ne = ctx->grid.n_elem;

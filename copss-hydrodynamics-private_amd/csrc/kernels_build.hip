@@ -459,15 +459,6 @@ __global__ void __launch_bounds__(kBrickThreads) k_brick_fill_ell(BrickArgs a, c
 // Element lists of the spreading by brick (one system): element = cell,
 // the own element is part of its stencil and nothing is excluded.  Every
 // brick has elements, particles or not.
-__device__ __forceinline__ bool brick_elem(const BoxGeom& B, int l, int& ci, int& cj, int& ck, int64_t& e) {
-  constexpr int M = (1 << kBrickBits) - 1;
-  ci = B.bx0 + (l & M);
-  cj = B.by0 + ((l >> kBrickBits) & M);
-  ck = B.bz0 + (l >> (2 * kBrickBits));
-  e = ci + int64_t(B.n[0]) * (cj + int64_t(B.n[1]) * ck);
-  return ci < B.n[0] && cj < B.n[1] && ck < B.n[2];  // padding cells of the last bricks are no elements
-}
-
 __global__ void __launch_bounds__(kBrickThreads) k_brick_elist_count(BrickArgs a, int64_t* __restrict__ cnt) {
   extern __shared__ int32_t cst[];  // [cells_cap + 1]
   __shared__ int32_t s_w[kBrickThreads / 64];

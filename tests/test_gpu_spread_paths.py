@@ -4,6 +4,8 @@ relative L2) on periodic, fine-grid, slit, wall-bounded, wrapping and
 crowded geometries, with the default and with autotuned launch shapes.
 The wrapping box exercises the explicit minimum image, taken per Gauss
 point as point_vector(q, x_p) does."""
+import math
+
 import numpy as np
 import pytest
 
@@ -103,3 +105,48 @@ def test_lists_and_local_velocity_paths_match_oracle(name):
             assert rel_l2(ctx.local_velocity_beads(), u_o) < TOL[prec], (name, prec)
         finally:
             ctx.close()
+
+
+SPREAD_BRICK = 3  # candidate index of the brick kernel (after k_spread's three block shapes)
+BRICK_CASES = ("periodic", "fine_grid", "slit", "walls", "crowded")  # no explicit minimum image
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("nq", [2, 3])
+def test_spread_by_brick_equals_element_list_kernel(name, nq):
+    """FP32C spreading by brick (box records and element lists in LDS, the
+    default where the box fits) gives k_spread's bits: on the crowded
+    geometry some boxes exceed the record table and go to the kernel that
+    reads their records from HBM; after a pair sum the table takes the size
+    of the staged layout's largest box.  Every candidate launches and
+    matches, pinned one by one."""
+    cfg, x, f = CASES[name]()
+    ctx = G.Context(dict(cfg, precision=G.FP32C))
+    try:
+        ctx.reinit(x, f)
+        IF = G.hex20_if_table(ctx.consts().h, nq)
+        rho_d, fe_d = ctx.spread(nq, IF)  # default shape
+        key = "spread/fp32c/%d/%d" % (nq, int(math.floor(math.log2(ctx.n_elem))))
+        ncand = SPREAD_BRICK + 1 if name in BRICK_CASES else SPREAD_BRICK
+        for c in range(ncand):
+            ctx.tune_set(key, c)
+            ctx.set_autotune(True)
+            rho, Fe = ctx.spread(nq, IF)
+            ctx.set_autotune(False)
+            assert np.array_equal(rho, rho_d) and np.array_equal(Fe, fe_d), (name, c)
+        ctx.local_velocity_beads()  # the staged pair layout (sizes the brick record table)
+        rho, Fe = ctx.spread(nq, IF)
+        assert np.array_equal(rho, rho_d) and np.array_equal(Fe, fe_d), name
+    finally:
+        ctx.close()
+    # the autotuner sees the brick kernel exactly where it applies
+    ctx = G.Context(dict(cfg, precision=G.FP32C))
+    try:
+        ctx.reinit(x, f)
+        ctx.set_autotune(True)
+        rho, Fe = ctx.spread(nq, IF)
+        assert np.array_equal(rho, rho_d) and np.array_equal(Fe, fe_d), name
+        pick, ms = ctx.tuning()[key]
+        assert len(ms) == ncand and max(ms) < 1e29, (name, ms)
+    finally:
+        ctx.close()
