@@ -63,10 +63,11 @@ int ensure_elist(ggem_ctx* ctx);
 // lists unless the brick kernel serves it (and the autotuner does not time
 // the other shapes)
 inline int ensure_spread(ggem_ctx* ctx, int nqp_1d) {
-  if (spread_by_brick(ctx, ctx->p.precision, nqp_1d)) {
+  const int mode = spread_brick_mode(ctx, ctx->p.precision, nqp_1d);
+  if (mode > 0) {
     const size_t nb = size_t(ctx->grid.n_cells >> (3 * kBrickBits));
     const int rc = ensure(ctx, &ctx->spread_unstaged, &ctx->spread_unstaged_cap, nb);
-    if (rc || !ctx->tune.on) return rc;
+    if (rc || (mode == 2 && !ctx->tune.on)) return rc;
   }
   return ensure_elist(ctx);
 }

@@ -490,9 +490,10 @@ int peek(ggem_ctx* ctx, std::initializer_list<std::pair<const void*, int32_t>> i
 
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
                          double* rho, double* Fe, bool node_major = false);
-// launch_spread runs the brick kernel, which needs no element lists (FP32C
-// with a box that fits the packing and the LDS)
-bool spread_by_brick(ggem_ctx* ctx, int precision, int nqp_1d);
+// FP32C spreading by brick (no element lists): 0 = it does not apply (box
+// over the packing or the LDS), 1 = an autotuner candidate, 2 = also the
+// default of launch_spread
+int spread_brick_mode(ggem_ctx* ctx, int precision, int nqp_1d);
 
 // ---- launchers (kernels_vec.hip) ----
 hipError_t launch_permute_in(ggem_ctx* ctx, int nrhs, const double* in_interleaved, double* out_slots);

@@ -736,10 +736,14 @@ bool brick_plan(ggem_ctx* ctx, int precision, int nqp_1d, BrickPlan& P) {
     return int64_t(2 * sizeof(float4)) * r + int64_t(sizeof(int32_t)) * P.cst_len +
            int64_t(sizeof(uint16_t)) * kBrickWaves * lists(r);
   };
+  // the table takes what two blocks per CU leave (more LDS would not cost
+  // occupancy), at least the wanted size when one block per CU holds it
   const int64_t limit = int64_t(lds_per_block(ctx)) - kBrickStaticLds;
+  const int64_t two = std::min<int64_t>(limit, int64_t(lds_per_block(ctx)) / 2 - kBrickStaticLds);
   int64_t cap = std::min<int64_t>(std::max(want, min_recs), 65535);
   while (cap > min_recs && bytes(cap) > limit) cap = std::max(min_recs, cap - std::max<int64_t>(1, cap / 8));
   if (bytes(cap) > limit) return false;
+  while (cap < 65535 && bytes(cap + 64) <= two) cap += 64;
   P.recs_cap = int32_t(cap);
   P.list_cap = int32_t(lists(cap));
   P.lds = size_t(bytes(cap));
@@ -770,9 +774,22 @@ hipError_t launch_brick(ggem_ctx* ctx, const BrickPlan& P, const double* gp, con
 
 }  // namespace
 
-bool spread_by_brick(ggem_ctx* ctx, int precision, int nqp_1d) {
+// The brick kernel walks every element's stencil rows twice per launch;
+// the element lists walk them once per build.  At reach 2 (21 rows, 1M
+// beads: 1.09 ms against 1.08 ms plus the 0.37 ms list build) the brick
+// kernel is the default; at larger reach (the slit's 69 rows: 0.65 ms
+// against 0.30 ms) the element-list kernel stays the default and the brick
+// kernel an autotuner candidate.
+static bool brick_default(const ggem_ctx* ctx) {
+  for (int d = 0; d < 3; ++d)
+    if (ctx->reach[d] > 2) return false;
+  return true;
+}
+
+int spread_brick_mode(ggem_ctx* ctx, int precision, int nqp_1d) {
   BrickPlan P;
-  return brick_plan(ctx, precision, nqp_1d, P);
+  if (!brick_plan(ctx, precision, nqp_1d, P)) return 0;
+  return brick_default(ctx) ? 2 : 1;
 }
 
 hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, const double* IF_d,
@@ -788,7 +805,8 @@ hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, cons
     gp[2] = std::sqrt(3. / 5.);
   }
   // candidates: the block shapes of k_spread, then (FP32C, when its box
-  // fits) the brick kernel, the default; all give the same bits
+  // fits) the brick kernel, the default at reach <= 2; all give the same
+  // bits
   BrickPlan bp;
   const bool brick = brick_plan(ctx, precision, nqp_1d, bp);
   auto run = [&](int c) {
@@ -805,7 +823,7 @@ hipError_t launch_spread(ggem_ctx* ctx, int precision, int nqp_1d, int n_u, cons
   };
   // rho and Fe are outputs only: idempotent
   int c = tune_pick(ctx, tune_key("spread", precision, nqp_1d, ctx->grid.n_elem), brick ? kNElem + 1 : kNElem, run,
-                    brick ? kNElem : 0);
+                    brick && brick_default(ctx) ? kNElem : 0);
   if (c == kNElem && !brick) c = 0;  // a pick recorded while the brick kernel applied
   ProfScope ps(ctx, precision == GGEM_FP64 ? "spread_fp64" : "spread_fp32c");
   return run(c);
