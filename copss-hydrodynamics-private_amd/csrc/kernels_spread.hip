@@ -589,19 +589,24 @@ k_spread_brick(BrickSpreadArgs a, const double* __restrict__ IF, int n_u, double
         };
         if constexpr (STAGED) {
           // the next record is read while the current one is summed, its
-          // list entry one step earlier
+          // list entry one step earlier; two records in turn, so no copy
           const uint16_t* lst = mine + base;
           if (n > 0) {
             const int r0 = lst[0];
             int e2 = lst[min(1, n - 1)];
-            float4 x1 = rpos[r0], f1 = rfrc[r0];
-            for (int k = 0; k < n; ++k) {
-              const float4 xt = x1, f = f1;
-              x1 = rpos[e2];
-              f1 = rfrc[e2];
+            float4 xa = rpos[r0], fa = rfrc[r0], xb, fb;
+            int k = 0;
+            for (; k + 1 < n; k += 2) {
+              xb = rpos[e2];
+              fb = rfrc[e2];
               e2 = lst[min(k + 2, n - 1)];
-              one(xt, f, k);
+              one(xa, fa, k);
+              xa = rpos[e2];
+              fa = rfrc[e2];
+              e2 = lst[min(k + 3, n - 1)];
+              one(xb, fb, k + 1);
             }
+            if (k < n) one(xa, fa, k);
           }
         } else {
           int k = 0;
