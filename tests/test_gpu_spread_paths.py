@@ -150,3 +150,32 @@ def test_spread_by_brick_equals_element_list_kernel(name, nq):
         assert len(ms) == ncand and max(ms) < 1e29, (name, ms)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["periodic", "walls", "crowded"])
+def test_node_major_spreading_by_brick(name):
+    """The node-major element vectors (ggem_spread_nodes, the right-hand side
+    assembly) from the brick kernel equal the element-list kernel's bit for
+    bit, each pinned in turn."""
+    cfg, x, f = CASES[name]()
+    node_xyz, conn = G.hex20_box_mesh(cfg)
+    nn = node_xyz.shape[0]
+    ctx = G.Context(dict(cfg, precision=G.FP32C))
+    try:
+        ctx.reinit(x, f)
+        ctx.set_fe_mesh(nn, conn)
+        out = []
+        for nq in (2, 3):
+            IF = G.hex20_if_table(ctx.consts().h, nq)
+            key = "spread/fp32c/%d/%d" % (nq, int(math.floor(math.log2(ctx.n_elem))))
+            got = []
+            for c in (0, SPREAD_BRICK):
+                ctx.tune_set(key, c)
+                ctx.set_autotune(True)
+                got.append((ctx.spread_nodes(nq, IF, nn), ctx.assemble_elem_rhs(nq, IF, G.rhs_opts())))
+                ctx.set_autotune(False)
+            assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1]), (name, nq)
+            out.append(got[0][0])
+        assert all(np.abs(F).sum() > 0 for F in out)
+    finally:
+        ctx.close()
