@@ -349,10 +349,11 @@ __global__ void __launch_bounds__(WAVES * 64) k_spread(SpreadArgs<T> a, const do
 // Gauss x-index, gathers the 27 (8) values of rho it needs from the
 // element's other lanes and sums them in k_spread's order.  The results
 // equal k_spread's bit for bit.  A brick whose box holds more particles
-// than the record table is flagged by that launch (STAGED) and taken by a
-// second one, which walks each element's whole stencil through the box
-// cell table and reads the records from HBM (same order, same values; more
-// registers, so it has a kernel of its own).  No element lists in HBM.
+// than the record table is flagged by that launch (k_spread_brick) and
+// taken by a second one (k_spread_brick_hbm), which walks each element's
+// whole stencil through the box cell table and reads the records from HBM
+// (same order, same values; more registers, so it has a kernel of its own).
+// No element lists in HBM.
 constexpr int kBrickWaves = 8;  // waves per block
 
 struct BrickSpreadArgs {
@@ -364,7 +365,7 @@ struct BrickSpreadArgs {
   int32_t recs_cap;        // records the LDS table holds
   int32_t cst_len;         // int32 words of the box cell table (box cells + 1, rounded up to 4)
   int32_t list_cap;        // 16-bit entries of each wave's list buffer (at least recs_cap)
-  uint8_t* unstaged;       // [bricks] 1: the box exceeds the record table (written by the STAGED launch)
+  uint8_t* unstaged;       // [bricks] 1: the box exceeds the record table (written by k_spread_brick)
 };
 
 // LDS writes of a wave visible to its other lanes
@@ -437,9 +438,8 @@ __device__ __forceinline__ void brick_contract(const float (&v)[NQ1 * NQ1][3], i
 }
 
 template <int NQ1, bool STAGED>
-__global__ void __launch_bounds__(kBrickWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
-k_spread_brick(BrickSpreadArgs a, const double* __restrict__ IF, int n_u, double* __restrict__ rho,
-               double* __restrict__ Fe) {
+__device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const double* __restrict__ IF, int n_u,
+                                             double* __restrict__ rho, double* __restrict__ Fe) {
   constexpr int W = kBrickWaves, NT = W * 64;
   constexpr int NQP = NQ1 * NQ1 * NQ1;
   constexpr int NB = NQ1 * NQ1;
@@ -649,6 +649,22 @@ k_spread_brick(BrickSpreadArgs a, const double* __restrict__ IF, int n_u, double
   }
 }
 
+// the staged launch: 4 waves per SIMD (at most 128 VGPRs), two blocks per CU
+template <int NQ1>
+__global__ void __launch_bounds__(kBrickWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
+k_spread_brick(BrickSpreadArgs a, const double* __restrict__ IF, int n_u, double* __restrict__ rho,
+               double* __restrict__ Fe) {
+  spread_brick<NQ1, true>(a, IF, n_u, rho, Fe);
+}
+
+// the flagged bricks, records from HBM (registers as the walk needs them)
+template <int NQ1>
+__global__ void __launch_bounds__(kBrickWaves * 64)
+k_spread_brick_hbm(BrickSpreadArgs a, const double* __restrict__ IF, int n_u, double* __restrict__ rho,
+                   double* __restrict__ Fe) {
+  spread_brick<NQ1, false>(a, IF, n_u, rho, Fe);
+}
+
 template <class T, int NQ1, int WAVES>
 void launch_spk(ggem_ctx* ctx, const SpreadArgs<T>& a, const double* IF_d, int n_u, double* rho, double* Fe) {
   constexpr int KE = WAVES * (64 / NQ1);  // elements per block
@@ -770,10 +786,10 @@ hipError_t launch_brick(ggem_ctx* ctx, const BrickPlan& P, const double* gp, con
   a.unstaged = ctx->spread_unstaged;
   const unsigned nb = unsigned(ctx->grid.n_cells >> (3 * kBrickBits));
   if (!a.unstaged || ctx->spread_unstaged_cap < nb) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_spread_brick<NQ1, true>), dim3(nb), dim3(kBrickWaves * 64), P.lds, ctx->stream, a, IF_d,
-                     n_u, rho, Fe);
-  hipLaunchKernelGGL((k_spread_brick<NQ1, false>), dim3(nb), dim3(kBrickWaves * 64), P.lds, ctx->stream, a, IF_d,
-                     n_u, rho, Fe);
+  hipLaunchKernelGGL((k_spread_brick<NQ1>), dim3(nb), dim3(kBrickWaves * 64), P.lds, ctx->stream, a, IF_d, n_u,
+                     rho, Fe);
+  hipLaunchKernelGGL((k_spread_brick_hbm<NQ1>), dim3(nb), dim3(kBrickWaves * 64), P.lds, ctx->stream, a, IF_d, n_u,
+                     rho, Fe);
   return hipGetLastError();
 }
 
