@@ -472,6 +472,8 @@ void free_all(ggem_ctx* ctx) {
     f(P.fe);
     f(P.rho);
     f(P.ifd);
+    f(P.cref);
+    f(P.cstop);
     delete &P;
     ctx->scratch = nullptr;
   }

@@ -48,6 +48,10 @@ struct Pools {
   size_t rho_cap = 0;
   double* ifd = nullptr;
   size_t ifd_cap = 0;
+  double* cref = nullptr;   // reference-mode Chebyshev state per right-hand side (8 doubles)
+  size_t cref_cap = 0;
+  int32_t* cstop = nullptr;  // ... and its stop flag
+  size_t cstop_cap = 0;
 };
 
 inline Pools& pools(ggem_ctx* ctx) { return *reinterpret_cast<Pools*>(ctx->scratch); }

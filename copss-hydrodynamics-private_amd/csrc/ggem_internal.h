@@ -454,6 +454,9 @@ struct ChebEpi {
   // before term)
   int ymask = 4;
   double cm1 = 0.0, c0 = 0.0;
+  // non-null: the launch does nothing while *skip is set (the reference
+  // mode's recurrence after its stopping rule fired on the device)
+  const int32_t* skip = nullptr;
 };
 // pair tensors of every list entry for the given precision
 hipError_t launch_pair_tensors(ggem_ctx* ctx, int precision);

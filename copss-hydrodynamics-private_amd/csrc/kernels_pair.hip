@@ -300,6 +300,7 @@ __global__ void __launch_bounds__(BLK) k_apply(PairArgs a, const void* __restric
   // (the tensor pointer is untyped in the signature: a type computed from
   // the header's unnamed precision enum would be mangled differently by the
   // host and device compilations; FP32C keeps no tensors)
+  if (epi.skip && *epi.skip) return;  // uniform over the grid
   const int64_t s = block_index(a.xcd) * BLK + threadIdx.x;
   const float4* tab = nullptr;
   if constexpr (P != GGEM_FP64) {
@@ -871,6 +872,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxView b
                                                           const double* __restrict__ u_glob,
                                                           double* __restrict__ out) {
   extern __shared__ float4 lds[];
+  if (epi.skip && *epi.skip) return;  // uniform over the grid
   const int64_t brick = block_index(a.xcd);
   const int64_t c0 = brick * kBrickCells;
   const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
@@ -1019,6 +1021,7 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box64(PairArgs a, BoxView
                                                             const double* __restrict__ u_glob,
                                                             double* __restrict__ out) {
   extern __shared__ double4 lds64[];
+  if (epi.skip && *epi.skip) return;  // uniform over the grid
   const int64_t brick = block_index(a.xcd);
   const int64_t c0 = brick * kBrickCells;
   const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
