@@ -302,36 +302,6 @@ double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, i
   return 0.5 * (lo + hi);
 }
 
-// The reference mode's estimators and stopping rule (brownian_system.C:1030-
-// 1155) on the device, so the recurrence needs no synchronisation per term.
-// st = [aux2 = (M x0).x0, y.y before the term, x_{k+1}.x_{k+1},
-// (M y).(M y), err1, err2, terms, converged]; once the rule fires, *stop is
-// set and the later terms' applications do nothing (the host keeps
-// launching them; their dot products and gather records change nothing
-// that is read).  The formulas and their FP64 order are the host's.
-__global__ void k_cheb_ref_init(double* st, int32_t* stop) {
-  if (threadIdx.x < 8) st[threadIdx.x] = 0.0;
-  if (threadIdx.x == 0) *stop = 0;
-}
-
-__global__ void k_cheb_ref_step(double* st, int32_t* stop, double xabs, double tol, int term, int check) {
-  if (threadIdx.x != 0 || *stop) return;
-  const double aux2 = st[0], aux4 = sqrt(st[1]), aux3 = sqrt(st[2]) * xabs, aux1 = st[3];
-  double err1 = st[4], err2 = st[5];
-  if (aux2 != 0.0) err1 = sqrt(fabs(aux1 - aux2) / aux2);
-  if (aux4 != 0.0) err2 = aux3 / aux4;
-  st[4] = err1;
-  st[5] = err2;
-  st[6] = double(term);
-  if (!check) return;
-  if (err1 <= tol) {
-    st[7] = 1.0;
-    *stop = 1;
-  } else if (err2 <= 1E-9) {
-    *stop = 1;
-  }
-}
-
 // one dot product into a device scalar (no readback)
 int dot_dev(ggem_ctx* ctx, const double* a, const double* b, double* out_d) {
   Pools& P = pools(ctx);
@@ -414,6 +384,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
     // device, a host operator on the host
     if (!op) {
       if ((rc = ensure(ctx, &P.cref, &P.cref_cap, size_t(8 * nrhs)))) return rc;
+      if ((rc = ensure(ctx, &P.red, &P.red_cap, size_t(513 + 3 * 512)))) return rc;
       if ((rc = ensure(ctx, &P.cstop, &P.cstop_cap, size_t(nrhs)))) return rc;
     }
     for (int r = 0; r < nrhs && !op; ++r) {
@@ -421,7 +392,9 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
       const SlotVec ay = offset(yv, r, n), am = offset(dwm, r, n);
       double* st = P.cref + 8 * r;
       int32_t* stop = P.cstop + r;
-      hipLaunchKernelGGL(k_cheb_ref_init, dim3(1), dim3(64), 0, ctx->stream, st, stop);
+      double* px = P.red + 513;  // partials: x . x, then (M y).(M y) and y . y
+      double* pmy = px + 512;
+      if ((e = launch_cheb_ref_init(ctx, st, stop)) != hipSuccess) return hip_fail(ctx, e, "chebyshev");
       if ((rc = apply(1, a0, a1, plain))) return rc;  // x1 = M x0
       if ((rc = dot_dev(ctx, a1.soa, a0.soa, st + 0))) return rc;
       if ((e = launch_axpby(ctx, len, db, a0.soa, da, a1.soa)) != hipSuccess) return hip_fail(ctx, e, "axpby");
@@ -430,27 +403,27 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
       if ((e = launch_axpby(ctx, len, X[0], a0.soa, 1.0, ay.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
       if ((rc = dot_dev(ctx, ay.soa, ay.soa, st + 1))) return rc;
       if ((e = launch_axpby(ctx, len, X[1], a1.soa, 1.0, ay.soa)) != hipSuccess) return hip_fail(ctx, e, "axpy");
-      if ((rc = dot_dev(ctx, a1.soa, a1.soa, st + 2))) return rc;
+      if ((e = launch_dot_partial(ctx, a1.soa, a1.soa, px)) != hipSuccess) return hip_fail(ctx, e, "dot");
       if ((rc = refresh(ctx, 1, ay))) return rc;
       if ((rc = apply(1, ay, am, plain))) return rc;
-      if ((rc = dot_dev(ctx, am.soa, am.soa, st + 3))) return rc;
-      hipLaunchKernelGGL(k_cheb_ref_step, dim3(1), dim3(64), 0, ctx->stream, st, stop, std::fabs(X[1]), tol, 1, 0);
+      if ((e = launch_dot2_partial(ctx, am.soa, am.soa, ay.soa, ay.soa, pmy)) != hipSuccess ||
+          (e = launch_cheb_ref_term(ctx, st, stop, px, pmy, std::fabs(X[1]), tol, 1, 0)) != hipSuccess)
+        return hip_fail(ctx, e, "chebyshev");
       ChebEpi skip_plain = plain;
       skip_plain.skip = stop;
       for (int i = 1; i < N; ++i) {
-        if ((rc = dot_dev(ctx, ay.soa, ay.soa, st + 1))) return rc;
+        // y . y before the term is in st[1] (from the previous term)
         ChebEpi stp{2, da, db, X[i + 1], a0, ay.soa};
         stp.skip = stop;
         if ((rc = apply(1, a1, a0, stp))) return rc;  // a0 <- x_{k+1}; y += c x_{k+1}
-        if ((rc = dot_dev(ctx, a0.soa, a0.soa, st + 2))) return rc;
+        if ((e = launch_dot_partial(ctx, a0.soa, a0.soa, px)) != hipSuccess) return hip_fail(ctx, e, "dot");
         if ((rc = refresh(ctx, 1, ay))) return rc;
         if ((rc = apply(1, ay, am, skip_plain))) return rc;
-        if ((rc = dot_dev(ctx, am.soa, am.soa, st + 3))) return rc;
-        hipLaunchKernelGGL(k_cheb_ref_step, dim3(1), dim3(64), 0, ctx->stream, st, stop, std::fabs(X[i + 1]), tol,
-                           i + 1, 1);
+        if ((e = launch_dot2_partial(ctx, am.soa, am.soa, ay.soa, ay.soa, pmy)) != hipSuccess ||
+            (e = launch_cheb_ref_term(ctx, st, stop, px, pmy, std::fabs(X[i + 1]), tol, i + 1, 1)) != hipSuccess)
+          return hip_fail(ctx, e, "chebyshev");
         std::swap(a0, a1);
       }
-      if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "chebyshev");
     }
     if (!op && info) {  // one readback for every right-hand side's statistics
       std::vector<double> h(size_t(8 * nrhs));

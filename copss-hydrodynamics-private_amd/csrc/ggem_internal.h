@@ -512,6 +512,15 @@ hipError_t launch_cheb_init_y(ggem_ctx* ctx, int nrhs, const double* x0, const d
 hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
                       double* partials);
 hipError_t launch_axpby(ggem_ctx* ctx, int64_t len, double a, const double* x, double b, double* y);
+// partials of one dot product of 3n slot vectors (512 doubles) or of two
+// (1024), and the reference mode's per-term estimators and stopping rule
+// over them (state st[8] and stop flag per right-hand side)
+hipError_t launch_dot_partial(ggem_ctx* ctx, const double* a, const double* b, double* partials);
+hipError_t launch_dot2_partial(ggem_ctx* ctx, const double* a1, const double* b1, const double* a2, const double* b2,
+                               double* partials);
+hipError_t launch_cheb_ref_init(ggem_ctx* ctx, double* st, int32_t* stop);
+hipError_t launch_cheb_ref_term(ggem_ctx* ctx, double* st, int32_t* stop, const double* px, const double* pmy,
+                                double xabs, double tol, int term, int check);
 // batched contexts: out_d[k] = a . b over system k's slots ([3][n] vectors)
 hipError_t launch_seg_dot(ggem_ctx* ctx, const double* a, const double* b, double* out_d);
 // w -= ca[k] x + cb[k] y, out = sc[k] x on the slots of system k

@@ -204,6 +204,102 @@ hipError_t launch_make_gather(ggem_ctx* ctx, int nrhs, const double* soa, float4
   return hipGetLastError();
 }
 
+namespace {
+
+// Two dot products a1 . b1 and a2 . b2 in one pass: each one's partials are
+// k_dot_partial's (same partition, same order within a thread).
+__global__ void k_dot2_partial(int64_t len, const double* __restrict__ a1, const double* __restrict__ b1,
+                               const double* __restrict__ a2, const double* __restrict__ b2,
+                               double* __restrict__ partials) {
+  __shared__ double sh[kBlock / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += int64_t(gridDim.x) * blockDim.x) {
+    s1 += a1[i] * b1[i];
+    s2 += a2[i] * b2[i];
+  }
+  const double t1 = block_sum(s1, sh);
+  const double t2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t1;
+    partials[kDotBlocks + blockIdx.x] = t2;
+  }
+}
+
+// k_dot_final's sum of kDotBlocks partials (one block of kBlock threads;
+// the total on thread 0)
+__device__ __forceinline__ double final_sum(const double* __restrict__ partials, double* sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += blockDim.x) s += partials[i];
+  return block_sum(s, sh);
+}
+
+// One term of the reference mode's estimators and stopping rule
+// (brownian_system.C:1030-1155) on the device, with the host's formulas
+// and FP64 order.  st = [aux2 = (M x0).x0, y.y before the term, -, -,
+// err1, err2, terms, converged]; px: partials of x_{k+1} . x_{k+1}, pmy:
+// of (M y).(M y) and of y.y after the term (kept in st[1] for the next).
+// Once the rule fires *stop is set; the later terms change nothing here.
+__global__ void __launch_bounds__(kBlock) k_cheb_ref_term(double* st, int32_t* stop, const double* __restrict__ px,
+                                                           const double* __restrict__ pmy, double xabs, double tol,
+                                                           int term, int check) {
+  __shared__ double sh[kBlock / 64];
+  const double xx = final_sum(px, sh);
+  const double mm = final_sum(pmy, sh);
+  const double yy = final_sum(pmy + kDotBlocks, sh);
+  if (threadIdx.x != 0) return;
+  if (!*stop) {
+    const double aux2 = st[0], aux4 = sqrt(st[1]), aux3 = sqrt(xx) * xabs, aux1 = mm;
+    double err1 = st[4], err2 = st[5];
+    if (aux2 != 0.0) err1 = sqrt(fabs(aux1 - aux2) / aux2);
+    if (aux4 != 0.0) err2 = aux3 / aux4;
+    st[4] = err1;
+    st[5] = err2;
+    st[6] = double(term);
+    if (check) {
+      if (err1 <= tol) {
+        st[7] = 1.0;
+        *stop = 1;
+      } else if (err2 <= 1E-9) {
+        *stop = 1;
+      }
+    }
+  }
+  st[1] = yy;
+}
+
+__global__ void k_cheb_ref_init(double* st, int32_t* stop) {
+  if (threadIdx.x < 8) st[threadIdx.x] = 0.0;
+  if (threadIdx.x == 0) *stop = 0;
+}
+
+}  // namespace
+
+hipError_t launch_cheb_ref_init(ggem_ctx* ctx, double* st, int32_t* stop) {
+  hipLaunchKernelGGL(k_cheb_ref_init, dim3(1), dim3(64), 0, ctx->stream, st, stop);
+  return hipGetLastError();
+}
+
+hipError_t launch_cheb_ref_term(ggem_ctx* ctx, double* st, int32_t* stop, const double* px, const double* pmy,
+                                double xabs, double tol, int term, int check) {
+  hipLaunchKernelGGL(k_cheb_ref_term, dim3(1), dim3(kBlock), 0, ctx->stream, st, stop, px, pmy, xabs, tol, term,
+                     check);
+  return hipGetLastError();
+}
+
+// partials of one dot product (kDotBlocks doubles) and of two (2 kDotBlocks)
+hipError_t launch_dot_partial(ggem_ctx* ctx, const double* a, const double* b, double* partials) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kBlock), 0, ctx->stream, int64_t(3) * ctx->n, 1, a, b,
+                     partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot2_partial(ggem_ctx* ctx, const double* a1, const double* b1, const double* a2, const double* b2,
+                               double* partials) {
+  hipLaunchKernelGGL(k_dot2_partial, dim3(kDotBlocks), dim3(kBlock), 0, ctx->stream, int64_t(3) * ctx->n, a1, b1,
+                     a2, b2, partials);
+  return hipGetLastError();
+}
+
 // partials must hold nrhs * kDotBlocks doubles
 hipError_t launch_dot(ggem_ctx* ctx, int nrhs, const double* a, const double* b, double* out_d,
                       double* partials) {
