@@ -299,7 +299,7 @@ def bench_slit(G, W, args):
                          "wall_side_nodes": n_items, "rhs_kernels_ms_per_step": rhs_ms,
                          "note": "two right-hand sides per step (one per half-step solve); the FEM solve itself "
                                  "is the reference's host side"},
-            "note": "reference-mode Chebyshev: host-side stopping decisions each term (one small copy each)"}
+            "note": "reference-mode Chebyshev, its stopping rule decided on the device (one readback per call)"}
 
 
 def measure(G, ctx, args, IF, nqp_1d, order, lo, hi, dw, precisions, names):
