@@ -524,7 +524,7 @@ __device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const dou
   int g = wv;
   while (g < NG) {  // uniform over the wave
     const int ng = min(EPW, kBrickCells - g * EPW);
-    for (int s0 = 0; s0 < ng;) {  // rounds of the group's elements whose lists fit the buffer
+    for (int s0 = 0; s0 < ng;) {  // passes over the group's elements, as many as fit the buffer each
       const int le = s0 + lw;
       int ci = 0, cj = 0, ck = 0;
       int64_t e = 0;
@@ -641,7 +641,7 @@ __device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const dou
       }
       if (Fe) brick_contract<NQ1>(v, qa, grp, IF, active ? n_u : 0, a.s.fe_node_major, Fe + e * per);
       s0 += take;
-      if constexpr (STAGED) wave_lds_sync();  // the lists are read before the next round writes
+      if constexpr (STAGED) wave_lds_sync();  // the lists are read before the next pass writes
     }
     int nx = 0;
     if (lane == 0) nx = atomicAdd(&s_next, 1);
