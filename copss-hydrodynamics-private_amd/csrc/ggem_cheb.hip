@@ -411,7 +411,16 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
         return hip_fail(ctx, e, "chebyshev");
       ChebEpi skip_plain = plain;
       skip_plain.skip = stop;
+      // terms are launched up to two past the last call's stop; one small
+      // readback then tells whether the rest are needed (terms launched after
+      // the stop return at entry, so the results do not depend on the guess)
+      const int guess = std::min(N, std::max(2, ctx->cheb_terms + 2));
       for (int i = 1; i < N; ++i) {
+        if (i == guess) {
+          int64_t stopped = 0;
+          if ((rc = peek(ctx, {{stop, 4}}, &stopped))) return rc;
+          if (stopped) break;
+        }
         // y . y before the term is in st[1] (from the previous term)
         ChebEpi stp{2, da, db, X[i + 1], a0, ay.soa};
         stp.skip = stop;
@@ -431,6 +440,7 @@ extern "C" int ggem_cheb_brownian(ggem_ctx* ctx, const ggem_op* op, int32_t N, d
       for (int r = 0; r < nrhs; ++r) {
         const double* q = h.data() + 8 * r;
         const int terms = int(q[6]);
+        ctx->cheb_terms = std::max(r ? ctx->cheb_terms : 0, terms);
         inf[r] = ggem_cheb_info{q[7] != 0.0 ? 1 : 0, terms, 2 * terms + (disp || keep ? 1 : 0), 0, q[4], q[5]};
       }
     }

@@ -287,6 +287,7 @@ struct ggem_ctx {
   double* par_h = nullptr;           // pinned staging of per-system recurrence parameters
   int64_t* peek_d = nullptr;         // [kPeekMax] device values gathered for one readback
   int64_t* peek_h = nullptr;         // [kPeekMax] their pinned host copy
+  int32_t cheb_terms = 1 << 30;      // terms of the last reference-mode Chebyshev call (its statistics read)
   size_t par_cap = 0;
   hipEvent_t par_ev = nullptr;
 
