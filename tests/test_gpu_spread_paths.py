@@ -179,3 +179,26 @@ def test_node_major_spreading_by_brick(name):
         assert all(np.abs(F).sum() > 0 for F in out)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32C], ids=["fp64", "fp32c"])
+def test_spread_without_particles(prec):
+    """A context that holds no particles spreads nothing: rho and Fe are
+    zero, by default and with the brick kernel pinned (FP32C)."""
+    cfg = W.WORKLOADS["beads20k"]["cfg"]()
+    ctx = G.Context(dict(cfg, precision=prec))
+    try:
+        ctx.reinit(np.zeros((0, 3)), np.zeros((0, 3)))
+        for nq in (2, 3):
+            IF = G.hex20_if_table(ctx.consts().h, nq)
+            rho, Fe = ctx.spread(nq, IF)
+            assert rho.shape[0] == ctx.n_elem and not rho.any() and not Fe.any()
+            if prec == G.FP32C:
+                key = "spread/fp32c/%d/%d" % (nq, int(math.floor(math.log2(ctx.n_elem))))
+                ctx.tune_set(key, SPREAD_BRICK)
+                ctx.set_autotune(True)
+                rho, Fe = ctx.spread(nq, IF)
+                ctx.set_autotune(False)
+                assert not rho.any() and not Fe.any()
+    finally:
+        ctx.close()
