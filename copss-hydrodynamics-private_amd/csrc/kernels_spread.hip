@@ -355,6 +355,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_spread(SpreadArgs<T> a, const do
 // (same order, same values; more registers, so it has a kernel of its own).
 // No element lists in HBM.
 constexpr int kBrickWaves = 8;  // waves per block
+constexpr int kSortBuckets = 256;  // list lengths the element order tells apart (longer ones share the last)
 
 struct BrickSpreadArgs {
   SpreadArgs<float> s;     // grid, constants, cell-relative records and forces
@@ -510,7 +511,6 @@ __device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const dou
   __syncthreads();
   const int32_t* first = reinterpret_cast<const int32_t*>(rpos);  // unstaged bricks
 
-  // ---- per wave from here on ----
   const int lane = threadIdx.x & 63, wv = int(threadIdx.x >> 6);
   const int lw = lane / NQ1;  // element of the lane within its group
   const int qa = lane - lw * NQ1;
@@ -518,6 +518,60 @@ __device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const dou
   uint16_t* const mine = lbuf + wv * a.list_cap;
   const int rpl = (a.nrows + NQ1 - 1) / NQ1;  // stencil rows each lane of an element walks
   const int rlo = min(a.nrows, qa * rpl), rhi = min(a.nrows, rlo + rpl);
+  // staged bricks: every element's list length first (each lane of an
+  // element counting its share of the rows), then the elements by length,
+  // longest first, so that a group's lists are alike (a group loops as
+  // long as its longest list; where an element is summed does not change
+  // its bits)
+  __shared__ uint16_t s_cnt[STAGED ? kBrickCells * NQ1 : 1];
+  __shared__ uint16_t s_order[STAGED ? kBrickCells : 1];
+  __shared__ int32_t s_hist[STAGED ? kSortBuckets : 1];
+  if constexpr (STAGED) {
+    for (int g0 = wv; g0 < NG; g0 += W) {
+      const int l = g0 * EPW + lw;
+      int c = 0, ci, cj, ck;
+      int64_t e;
+      if (lw < EPW && l < kBrickCells && brick_elem(B, l, ci, cj, ck, e))
+        box_walk_rows(B, a.rows, rlo, rhi, ci, cj, ck, [&](int q0, int q1, uint32_t) { c += cst[q1 + 1] - cst[q0]; });
+      if (lw < EPW && l < kBrickCells) s_cnt[l * NQ1 + qa] = uint16_t(c);
+    }
+    for (int i = int(threadIdx.x); i < kSortBuckets; i += NT) s_hist[i] = 0;
+    __syncthreads();
+    auto bucket = [&](int l) {
+      int t = 0;
+#pragma unroll
+      for (int k = 0; k < NQ1; ++k) t += s_cnt[l * NQ1 + k];
+      return kSortBuckets - 1 - min(t, kSortBuckets - 1);
+    };
+    for (int l = int(threadIdx.x); l < kBrickCells; l += NT) atomicAdd(&s_hist[bucket(l)], 1);
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the buckets, kSortBuckets / 64 a lane
+      constexpr int PL = kSortBuckets / 64;
+      int v[PL], sum = 0;
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        v[k] = s_hist[lane * PL + k];
+        sum += v[k];
+      }
+      int incl = sum;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      int run = incl - sum;
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        s_hist[lane * PL + k] = run;
+        run += v[k];
+      }
+    }
+    __syncthreads();
+    for (int l = int(threadIdx.x); l < kBrickCells; l += NT) s_order[atomicAdd(&s_hist[bucket(l)], 1)] = uint16_t(l);
+    __syncthreads();
+  }
+
+  // ---- per wave from here on ----
   const float gx = a.s.goff[0][qa], gy = a.s.goff[1][qa], gz = a.s.goff[2][qa];
   const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
   const int per = 3 * n_u;
@@ -528,15 +582,14 @@ __device__ __forceinline__ void spread_brick(const BrickSpreadArgs& a, const dou
       const int le = s0 + lw;
       int ci = 0, cj = 0, ck = 0;
       int64_t e = 0;
-      const bool elem = lw < EPW && le < ng && brick_elem(B, g * EPW + le, ci, cj, ck, e);
+      const int l = lw < EPW && le < ng ? (STAGED ? int(s_order[g * EPW + le]) : g * EPW + le) : 0;
+      const bool elem = lw < EPW && le < ng && brick_elem(B, l, ci, cj, ck, e);
       int take = ng - s0, n = 0, base = 0;
       if constexpr (STAGED) {
-        // (1) each lane counts its rows' records; the exclusive scan over
-        // the wave is where its part of the lists starts (lists element
-        // by element, an element's rows in lane order)
-        int c = 0;
-        if (elem)
-          box_walk_rows(B, a.rows, rlo, rhi, ci, cj, ck, [&](int q0, int q1, uint32_t) { c += cst[q1 + 1] - cst[q0]; });
+        // (1) the lane's share of the element's list; the exclusive scan
+        // over the wave is where it starts (lists element by element, an
+        // element's rows in lane order)
+        const int c = elem ? int(s_cnt[l * NQ1 + qa]) : 0;
         int incl = c;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -714,7 +767,7 @@ struct BrickPlan {
   size_t lds;
 };
 
-constexpr int kBrickStaticLds = 64;  // k_spread_brick's static LDS, rounded up
+constexpr int kBrickStaticLds = 5248;  // k_spread_brick's static LDS (list lengths, element order), rounded up
 constexpr int kBrickListMin = 1536;  // list entries of a wave's buffer (a group's lists, 21 x ~40 at 1M beads)
 
 int lds_per_block(ggem_ctx* ctx) {
