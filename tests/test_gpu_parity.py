@@ -239,6 +239,33 @@ def test_cheb_reference_mode_matches_oracle(c1):
 
 
 @pytest.mark.parametrize("prec,tol", [(G.FP64, TOL64), (G.FP32C, TOL32)])
+def test_cheb_reference_mode_multi_rhs(c1, prec, tol):
+    """Reference mode with three right-hand sides: each stops where it
+    stops alone (its own device stop flag and statistics), with the same
+    bits as its single-vector call, and matches the oracle."""
+    cfg, x, f, ctx, m = c1
+    lmin, lmax = dense_bounds(m)
+    lo, hi = lmin / 1.05, lmax * 1.05
+    dw = G.normal(3 * 3 * len(x), 77).reshape(3, -1, 3)
+    ctx.set_precision(prec)
+    try:
+        for ctol in (1e-6, 0.1):
+            y, d, info = ctx.cheb_brownian(30, lo, hi, dw, tol=ctol, mode=G.CHEB_REFERENCE)
+            for r in range(3):
+                y1, d1, i1 = ctx.cheb_brownian(30, lo, hi, dw[r], tol=ctol, mode=G.CHEB_REFERENCE)
+                assert np.array_equal(y[r], y1[0]), (ctol, r)
+                assert (info[r].converged, info[r].terms, info[r].applies) == \
+                    (i1[0].converged, i1[0].terms, i1[0].applies), (ctol, r)
+                yo, do, io = m.cheb_brownian(30, lo, hi, ctol, 0, dw[r])
+                assert info[r].terms == io.iterations and info[r].converged == io.converged, (ctol, r)
+                assert rel_l2(y[r], yo) < tol and rel_l2(d[r], do) < tol, (ctol, r)
+    finally:
+        ctx.set_precision(G.FP64)
+        ctx.set_forces(f)
+        m.set_forces(f)
+
+
+@pytest.mark.parametrize("prec,tol", [(G.FP64, TOL64), (G.FP32C, TOL32)])
 def test_cheb_fixed_multi_rhs(c1, prec, tol):
     cfg, x, f, ctx, m = c1
     lmin, lmax = dense_bounds(m)
