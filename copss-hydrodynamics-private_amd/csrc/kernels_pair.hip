@@ -1008,6 +1008,137 @@ __global__ void __launch_bounds__(kBoxThreads) k_apply_box(PairArgs a, BoxView b
 }
 
 
+// k_apply_box for NR > 1 operands (the fixed-order step of several noise
+// vectors): the box's positions and the NR operands' records staged in LDS
+// ([NR][records] after the positions), each kept pair's table evaluated
+// once for all NR; per operand the sums, their merge points and the
+// epilogue are k_apply's for NR operands.  Every brick with targets is
+// staged (the host launches it only then).
+template <int EPI, bool ONE, int NR>
+__global__ void __launch_bounds__(kBoxThreads) k_apply_box_nr(PairArgs a, BoxView b, const float4* __restrict__ xg,
+                                                             const double* __restrict__ xsoa, SlotVec v, ChebEpi epi,
+                                                             const double* __restrict__ u_glob,
+                                                             double* __restrict__ out) {
+  static_assert(NR > 1, "one operand: k_apply_box");
+  extern __shared__ float4 lds[];
+  if (epi.skip && *epi.skip) return;  // uniform over the grid
+  const int64_t brick = block_index(a.xcd);
+  const int64_t c0 = brick * kBrickCells;
+  const int32_t sb = b.cs[c0], se = b.cs[c0 + kBrickCells];
+  if (sb >= se) return;  // no targets (uniform over the block)
+  const int64_t r0 = b.off[brick];
+  const int nrec = int(b.off[brick + 1] - r0);
+  const int nt = se - sb;
+  if (nrec == 0) return;  // not staged: never launched so (uniform over the block)
+  const int64_t n = a.n;
+  float4* s_tab = lds;                       // coefficient table
+  float4* rpos = lds + kTabStride * a.ct.n;  // [nrec] offset xyz, packed box cell
+  float4* rval = rpos + nrec;                // [NR][nrec] operands
+  load_tab(s_tab, a);
+  for (int i = int(threadIdx.x); i < nrec; i += kBoxThreads) {
+    const uint32_t t = b.slot[r0 + i];
+    rpos[i] = b.pos[r0 + i];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) rval[r * nrec + i] = xg[r * n + t];
+  }
+  __syncthreads();
+  BoxGeom B;
+  box_init(B, a, b.reach, brick, sb);
+  const float hx = B.h[0], hy = B.h[1], hz = B.h[2];
+  constexpr int U = 2;  // entries per group, the next group's record numbers loaded one ahead
+  for (int i = int(threadIdx.x); i < nt; i += kBoxThreads) {
+    const int64_t s = b.tgt[sb + i];
+    const int64_t bpk = b.base[sb + i];
+    const int64_t base = bpk & kBaseMask;
+    const float4 po = rpos[int((uint64_t(bpk) >> kSelfShift) & kField12)];  // the target's own record
+    const uint32_t kb = 0x808080u - __float_as_uint(po.w);
+    const int cnt = int((uint64_t(bpk) >> kKeptShift) & kField12);  // stored (kept) entries
+    float ps[NR][3];
+    KahanF kh[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) ps[r][0] = ps[r][1] = ps[r][2] = 0.f;
+    auto disp = [&](const float4& pt, float& dx, float& dy, float& dz, float& y) {
+      float di, dj, dk;
+      box_diff(__float_as_uint(pt.w), kb, di, dj, dk);
+      dx = __builtin_fmaf(di, hx, pt.x - po.x);
+      dy = __builtin_fmaf(dj, hy, pt.y - po.y);
+      dz = __builtin_fmaf(dk, hz, pt.z - po.z);
+      return ONE ? fp32c_interval1(a.ct, fp32c_r2(dx, dy, dz), y) : fp32c_interval(a.ct, fp32c_r2(dx, dy, dz), y);
+    };
+    auto merge = [&](int k) {
+      if ((k & (kMergeEvery - 1)) == kMergeEvery - 1) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            kh[r][q].add(ps[r][q]);
+            ps[r][q] = 0.f;
+          }
+      }
+    };
+    auto one = [&](uint32_t rec, int k) {
+      const float4 pt = rpos[rec];
+      float dx, dy, dz, y, A, Bx;
+      const int it = disp(pt, dx, dy, dz, y);
+      fp32c_eval(s_tab, it, y, A, Bx);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const float4 f = rval[r * nrec + rec];
+        fp32c_acc(A, Bx, dx, dy, dz, f.x, f.y, f.z, ps[r][0], ps[r][1], ps[r][2]);
+      }
+      merge(k);
+    };
+    if (cnt > 0) {
+      const uint16_t* __restrict__ ci = b.idx + base;
+      const int last = cnt - 1;
+      uint32_t en[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) en[u] = ci[int64_t(min(u, last)) * kSlice];
+      int k = 0;
+      for (; k + U <= cnt; k += U) {
+        uint32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          c[u] = en[u];
+          en[u] = ci[int64_t(min(k + U + u, last)) * kSlice];
+        }
+        float4 pt[U], f[U][NR];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          pt[u] = rpos[c[u]];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) f[u][r] = rval[r * nrec + c[u]];
+        }
+        float dx[U], dy[U], dz[U], y[U];
+        int it[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) it[u] = disp(pt[u], dx[u], dy[u], dz[u], y[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          float A, Bx;
+          fp32c_eval(s_tab, it[u], y[u], A, Bx);
+#pragma unroll
+          for (int r = 0; r < NR; ++r)
+            fp32c_acc(A, Bx, dx[u], dy[u], dz[u], f[u][r].x, f[u][r].y, f[u][r].z, ps[r][0], ps[r][1], ps[r][2]);
+          merge(k + u);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U - 1; ++u)
+        if (k + u < cnt) one(en[u], k + u);
+    }
+    double acc[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        kh[r][q].add(ps[r][q]);
+        acc[r][q] = kh[r][q].value();
+      }
+    apply_epilogue<NR, EPI>(a, s, acc, xsoa, v, epi, u_glob, out);
+  }
+}
+
 // FP64 pair sum with the operand records staged in LDS: one workgroup per
 // brick, its targets in slot order; per entry the stored tensor (list
 // layout) and the neighbour's record number in the brick's box (the same
@@ -1237,6 +1368,32 @@ hipError_t launch_box(ggem_ctx* ctx, PairArgs a, const float4* xg, const double*
   return hipGetLastError();
 }
 
+// staged FP32C pair sum of NR > 1 operands: every brick with targets staged
+// and the box's positions and NR operand tables within the block's LDS
+constexpr size_t kBoxNrLds = 81920;  // two blocks per CU
+size_t box_nr_lds(const ggem_ctx* ctx, int nr) {
+  return sizeof(float4) * (kTabStride * size_t(ctx->tab.n) + size_t(ctx->box_max) * size_t(1 + nr));
+}
+bool box_nr_ready(const ggem_ctx* ctx, int nr) {
+  return nr > 1 && nr <= 4 && ctx->box_valid && !ctx->box_fp64 && !ctx->box_unstaged && ctx->box_max > 0 &&
+         ctx->box_max <= kBoxMaxRecs && box_nr_lds(ctx, nr) <= kBoxNrLds;
+}
+
+template <int EPI, int NR>
+hipError_t launch_box_nr(ggem_ctx* ctx, PairArgs a, const float4* xg, const double* soa, const SlotVec& v,
+                         const ChebEpi& epi, const double* u_glob, double* out) {
+  if (!box_nr_ready(ctx, NR)) return hipErrorInvalidConfiguration;
+  a.xcd = 1;
+  const size_t lds = box_nr_lds(ctx, NR);
+  if (ctx->tab.n2 == 0)
+    hipLaunchKernelGGL((k_apply_box_nr<EPI, true, NR>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream, a,
+                       box_view(ctx), xg, soa, v, epi, u_glob, out);
+  else
+    hipLaunchKernelGGL((k_apply_box_nr<EPI, false, NR>), dim3(n_bricks(ctx)), dim3(kBoxThreads), lds, ctx->stream,
+                       a, box_view(ctx), xg, soa, v, epi, u_glob, out);
+  return hipGetLastError();
+}
+
 template <int EPI>
 hipError_t launch_box64(ggem_ctx* ctx, PairArgs a, const double4* xg, const double* soa, const SlotVec& v,
                         const ChebEpi& epi, const double* u_glob, double* out) {
@@ -1253,10 +1410,16 @@ hipError_t launch_box64(ggem_ctx* ctx, PairArgs a, const double4* xg, const doub
 // block shapes of k_apply); it is the default where some brick is staged
 constexpr int kBoxShape = kNApply;
 bool box_ready(const ggem_ctx* ctx, int precision, int nr) {
-  if (nr != 1 || !ctx->box_valid || ctx->box_max <= 0) return false;
+  if (nr > 1) return precision == GGEM_FP32C && box_nr_ready(ctx, nr);
+  if (!ctx->box_valid || ctx->box_max <= 0) return false;
   return precision == GGEM_FP32C ? !ctx->box_fp64 : (ctx->box_fp64 && ctx->coef64_valid);
 }
-int apply_default(const ggem_ctx* ctx, int precision, int nr) { return box_ready(ctx, precision, nr) ? kBoxShape : 0; }
+// the staged kernel by default for one operand; for two or four it is an
+// autotuner candidate only (256 systems x 16 operands in chunks of 4:
+// 3.53 ms per apply against 2.25 ms for the list-order kernel in chunks of 8)
+int apply_default(const ggem_ctx* ctx, int precision, int nr) {
+  return nr == 1 && box_ready(ctx, precision, nr) ? kBoxShape : 0;
+}
 int apply_candidates(const ggem_ctx* ctx, int precision, int nr) {
   return box_ready(ctx, precision, nr) ? kNApply + 1 : kNApply;
 }
@@ -1267,6 +1430,8 @@ hipError_t launch_k(ggem_ctx* ctx, PairArgs a, int c, const double4* g64, const 
                     const SlotVec& v, const ChebEpi& epi, const double* u_glob, double* out) {
   if (c == kBoxShape) {
     if constexpr (P == GGEM_FP32C && NR == 1) return launch_box<EPI>(ctx, a, g32, soa, v, epi, u_glob, out);
+    if constexpr (P == GGEM_FP32C && (NR == 2 || NR == 4))
+      return launch_box_nr<EPI, NR>(ctx, a, g32, soa, v, epi, u_glob, out);
     if constexpr (P == GGEM_FP64 && NR == 1) return launch_box64<EPI>(ctx, a, g64, soa, v, epi, u_glob, out);
     return hipErrorInvalidConfiguration;
   }

@@ -185,3 +185,33 @@ def test_lists_filled_on_demand_between_one_and_several_operands():
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("nrhs", [2, 3, 4, 6])
+def test_staged_kernel_several_operands_matches_list_order_kernel(nrhs):
+    """Several FP32C operands through the staged-record kernel (an autotuner
+    candidate for chunks of two and four operands) give the list-order
+    kernel's results bit for bit, pinned either way, and the oracle's to
+    the FP32C bound."""
+    cfg, x, f = W.make_inputs("beads20k")
+    cfg["precision"] = G.FP32C
+    dw = G.normal(nrhs * 3 * len(x), 31).reshape(nrhs, -1, 3)
+    lg = int(math.floor(math.log2(len(x))))
+    ctx = G.Context(cfg)
+    try:
+        ctx.reinit(x, f)
+        y0, d0, _ = ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)
+        res = []
+        for cand in (0, BOX):
+            for nr in (2, 4):
+                ctx.tune_set("apply/fp32c/%d/%d" % (nr, lg), cand)
+            ctx.set_autotune(True)
+            res.append(ctx.cheb_brownian(10, 0.05, 3.0, dw, mode=G.CHEB_FIXED)[:2])
+        for y, d in res:
+            assert np.array_equal(y, y0) and np.array_equal(d, d0)
+    finally:
+        ctx.close()
+    m = O.Mesh(dict(cfg, precision=G.FP64))
+    m.reinit(x, f)
+    y_ref, _, _ = m.cheb_brownian(10, 0.05, 3.0, 0.1, 1, dw[nrhs - 1])
+    assert rel_l2(y0[nrhs - 1], y_ref) < 1e-6
